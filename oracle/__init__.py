@@ -1,0 +1,297 @@
+"""CPU oracle of the equi-join hot path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s cpu_baseline leg import this package, and only as the checker
+or the timed CPU baseline; the product package ``pageflow_amd`` never imports
+it and has no CPU fallback.
+
+The restatement is plain C (``pf_oracle.c``, loaded through ctypes) for the
+hash programs, and numpy for the sort/partition/layout arithmetic.  Every
+function cites the reference file:line it follows (paths relative to the
+reference repository).
+
+Parity pinning (what checks each function besides the formula itself):
+
+* ``mix64_np`` / ``mix64``: golden (key, mix64) vectors written from the
+  reference's own numpy twin (``tests/mix64.py``) by
+  ``tests/golden/make_mix64_golden.py``; the two restatements (numpy and C)
+  agree; fmix64 is a bijection (checked against its inverse) and fmix64(0)=0.
+* ``partition``: the reference test's stable-sort definition
+  (``tests/operators/test_partition_sort.py:106-118``) and the padded layout
+  invariants (``:93-103``), plus brute force per bucket.
+* ``unique_hash_pairs``: the known answers of
+  ``tests/operators/test_unique_hash_join.py:86-151`` and a brute-force
+  dense comparison.
+* ``semi_anti``: the dense reference (``tests/operators/test_join_core.py:93-101``)
+  and the known answers ``:121-144``.
+* ``expand``: the ``repeat_interleave`` reference (``test_join_core.py:55-61``)
+  and its hand cases (``:64-77``).
+* ``sort_search_pairs``: the dense build x probe comparison
+  (``test_join_core.py:263-281``).
+* ``group_sum_count``: closed form (per-group sums by brute force).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_I64_MIN = np.iinfo(np.int64).min
+NULL = -64  # constants.py:9
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libpf_oracle.so")
+        if not os.path.exists(path):
+            import subprocess
+            subprocess.run(["make", "-s", "-C", _HERE], check=True)
+        lib = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        I64 = ctypes.c_int64
+        I32 = ctypes.c_int32
+        lib.o_mix64.argtypes = [P, P, I64]
+        lib.o_bucket_of.argtypes = [P, I64, ctypes.c_int, P]
+        lib.o_unique_hash_pairs.argtypes = [P, P, I64, P, P, I64, ctypes.c_int, P, P]
+        lib.o_unique_hash_pairs.restype = I64
+        lib.o_semi_anti.argtypes = [P, I64, P, I64, P, P, ctypes.c_int, ctypes.c_int, P, P]
+        lib.o_semi_anti.restype = I64
+        lib.o_expand.argtypes = [P, P, P, I64, P, P]
+        lib.o_expand.restype = I64
+        lib.o_group_sum_count.argtypes = [P, P, I64, I32, P, P]
+        lib.o_gather_i64.argtypes = [P, I64, P, I64, P]
+        _LIB = lib
+    return _LIB
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _i64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a), dtype=np.int64)
+
+
+# ---------------------------------------------------------------- mix64 --
+
+_M1 = np.uint64(0xff51afd7ed558ccd)
+_M2 = np.uint64(0xc4ceb9fe1a85ec53)
+_S33 = np.uint64(33)
+
+
+def mix64_np(keys) -> np.ndarray:
+    """mix64 of int64 keys as uint64, in numpy uint64 arithmetic
+    (kernel_utils.cuh:18-22; the reference's own twin tests/mix64.py:12-21)."""
+    k = np.ascontiguousarray(np.asarray(keys), dtype=np.int64).view(np.uint64)
+    with np.errstate(over="ignore"):
+        k = k ^ (k >> _S33)
+        k = k * _M1
+        k = k ^ (k >> _S33)
+        k = k * _M2
+        k = k ^ (k >> _S33)
+    return k
+
+
+def mix64(keys) -> np.ndarray:
+    """mix64 through the C restatement (uint64)."""
+    k = _i64(keys)
+    out = np.empty(k.shape[0], dtype=np.uint64)
+    _lib().o_mix64(_p(k), _p(out), k.shape[0])
+    return out
+
+
+def bucket_of(keys, B: int) -> np.ndarray:
+    """Each key's radix bucket among B: the top log2(B) bits of mix64
+    (partition_sort.cu:110-112; tests/mix64.py:30-36)."""
+    k = _i64(keys)
+    log_b = B.bit_length() - 1
+    out = np.empty(k.shape[0], dtype=np.int32)
+    _lib().o_bucket_of(_p(k), k.shape[0], log_b, _p(out))
+    return out
+
+
+# ------------------------------------------------------------ partition --
+
+def rows_per_unit(row_bytes: int) -> int:
+    """Rows of one padding unit, lcm(row bytes, 512) / row bytes
+    (partition_sort.cu:582; IO/codec/batched_compress.py:27-62)."""
+    from math import gcd
+    return 512 // gcd(row_bytes, 512)
+
+
+def partition(cols: Sequence[np.ndarray], key_pos: int, mask: Optional[np.ndarray],
+              goff: int, B: int) -> Tuple[List[int], List[List[np.ndarray]]]:
+    """The bucket counts and, per output (each column, then the gidx), each
+    bucket's rows: the kept rows stably ordered by bucket, gidx = goff + the
+    pre-mask row (operators/partition_sort.py:68-143, partition_sort.cu:1-31;
+    the reference test's definition tests/operators/test_partition_sort.py:106-118).
+    ``cols`` are raw (pre-mask) columns."""
+    n_pre = mask.shape[0] if mask is not None else cols[key_pos].shape[0]
+    gidx = np.arange(goff, goff + n_pre, dtype=np.int64)
+    if mask is not None:
+        keep = mask.astype(bool)
+        cols = [c[keep] for c in cols]
+        gidx = gidx[keep]
+    bucket = bucket_of(cols[key_pos], B)
+    order = np.argsort(bucket, kind="stable")
+    counts = np.bincount(bucket, minlength=B).astype(np.int64)
+    splits = np.cumsum(counts)[:-1]
+    outs = [list(np.split(c[order], splits)) for c in list(cols) + [gidx]]
+    return counts.tolist(), outs
+
+
+def padded_layout(counts: Sequence[int], row_bytes: int) -> Tuple[List[int], int]:
+    """Each bucket's first row and the padded total: every non-empty bucket
+    starts on a whole lcm(row bytes, 512)-byte unit
+    (IO/codec/batched_compress.py:27-62; IO/pipeline/common.py:36-45)."""
+    unit = rows_per_unit(row_bytes)
+    starts, at = [], 0
+    for c in counts:
+        starts.append(at)
+        at += -(-int(c) // unit) * unit
+    return starts, at
+
+
+def radix_partition(keys: np.ndarray, vals: np.ndarray, bits: int, shift: int,
+                    hashed: bool) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Stable partition of (key, value) rows by a digit of ``bits`` bits:
+    the top bits of mix64(key) below the top ``shift`` bits when ``hashed``
+    (partition_sort.cu:110-112 generalised to 2^12 buckets), else bits
+    [shift, shift + bits) of the unsigned key.  Returns (keys, values,
+    per-bucket counts)."""
+    k = np.ascontiguousarray(keys)
+    if hashed:
+        h = mix64(k.astype(np.int64))
+        d = ((h << np.uint64(shift)) >> np.uint64(64 - bits)).astype(np.int64) if bits else \
+            np.zeros(k.shape[0], np.int64)
+    else:
+        u = k.astype(np.int64).view(np.uint64) if k.dtype == np.int64 else \
+            k.astype(np.uint32).astype(np.uint64)
+        d = ((u >> np.uint64(shift)) & np.uint64((1 << bits) - 1)).astype(np.int64)
+    order = np.argsort(d, kind="stable")
+    counts = np.bincount(d, minlength=1 << bits).astype(np.int64)
+    return k[order], np.ascontiguousarray(vals)[order], counts
+
+
+# ------------------------------------------------------------ join core --
+
+def unique_hash_pairs(build_keys: Sequence[np.ndarray],
+                      probe_keys: Sequence[np.ndarray]) -> Tuple[np.ndarray, np.ndarray]:
+    """(build rows, probe rows) of an inner join on a unique build tuple of
+    1-2 int64 keys, in probe-row order (join_core.py:97-113,
+    join_core.cu:105-146, 301-351)."""
+    two = len(build_keys) == 2
+    b0, p0 = _i64(build_keys[0]), _i64(probe_keys[0])
+    b1 = _i64(build_keys[1]) if two else b0
+    p1 = _i64(probe_keys[1]) if two else p0
+    nb, np_ = b0.shape[0], p0.shape[0]
+    bi = np.empty(max(np_, 1), np.int64)
+    pi = np.empty(max(np_, 1), np.int64)
+    n = _lib().o_unique_hash_pairs(_p(b0), _p(b1), nb, _p(p0), _p(p1), np_, int(two),
+                                   _p(bi), _p(pi))
+    if n < 0:
+        raise MemoryError("oracle: unique_hash_pairs table allocation failed")
+    return bi[:n].copy(), pi[:n].copy()
+
+
+SUMMARY_OPS = {'<>': 1, '<': 2, '<=': 3, '>': 4, '>=': 5}  # join_core.py:35
+
+
+def semi_anti(build_key, probe_key, anti: bool, summary=None) -> Optional[np.ndarray]:
+    """Probe rows, ascending, with (semi) or without (anti) a match; None
+    when a build key is int64 min (join_core.py:116-143,
+    join_core.cu:154-212, 358-400)."""
+    bk, pk = _i64(build_key), _i64(probe_key)
+    bv = pv = None
+    op = 0
+    if summary is not None:
+        bv, pv, name = _i64(summary[0]), _i64(summary[1]), summary[2]
+        op = SUMMARY_OPS[name]
+    out = np.empty(max(pk.shape[0], 1), np.int64)
+    bad = np.zeros(1, np.int32)
+    n = _lib().o_semi_anti(_p(bk), bk.shape[0], _p(pk), pk.shape[0], _p(bv), _p(pv), op,
+                           int(bool(anti)), _p(out), _p(bad))
+    if n < 0:
+        raise MemoryError("oracle: semi_anti table allocation failed")
+    if bad[0]:
+        return None
+    return out[:n].copy()
+
+
+def expand(sorted_ids, start, length) -> Tuple[np.ndarray, np.ndarray]:
+    """(build rows, probe rows) of each probe row's run, probe-row then run
+    order (join_core.py:54-61, join_core.cu:86-101, 263-295)."""
+    s, st, ln = _i64(sorted_ids), _i64(start), _i64(length)
+    m = ln.shape[0]
+    n = _lib().o_expand(_p(s), _p(st), _p(ln), m, None, None)
+    cs = np.empty(max(n, 1), np.int64)
+    cr = np.empty(max(n, 1), np.int64)
+    _lib().o_expand(_p(s), _p(st), _p(ln), m, _p(cs), _p(cr))
+    return cs[:n].copy(), cr[:n].copy()
+
+
+def sort_search_pairs(left, right, bucket_counts=None) -> Tuple[np.ndarray, np.ndarray]:
+    """The sort/search core's (left rows, right rows) (hashjoin.py:380-441):
+    per bucket, the left keys sorted (stably here; the reference's sort is
+    unstable, so only the multiset is pinned), each right row's run found by
+    two searchsorted, every run expanded."""
+    lk, rk = _i64(left), _i64(right)
+    nL, nR = lk.shape[0], rk.shape[0]
+    lc, rc = bucket_counts if bucket_counts is not None else ([nL], [nR])
+    sorted_ids = np.empty(nL, np.int64)
+    start = np.zeros(nR, np.int64)
+    length = np.zeros(nR, np.int64)
+    ls = rs = 0
+    for lcnt, rcnt in zip(lc, rc):
+        le, re_ = ls + lcnt, rs + rcnt
+        if lcnt > 0:
+            perm = np.argsort(lk[ls:le], kind="stable")
+            sk = lk[ls:le][perm]
+            sorted_ids[ls:le] = perm + ls
+            if rcnt > 0:
+                rkb = rk[rs:re_]
+                lo = np.searchsorted(sk, rkb, side="left")
+                length[rs:re_] = np.searchsorted(sk, rkb, side="right") - lo
+                start[rs:re_] = lo + ls
+        ls, rs = le, re_
+    return expand(sorted_ids, start, length)
+
+
+def join_pairs_sorted(lk, rk) -> np.ndarray:
+    """All (left row, right row) pairs of an inner equi-join, as an int64
+    [n, 2] array sorted by (left, right) (the multiset of
+    hashjoin.py:380-441)."""
+    l, r = sort_search_pairs(lk, rk)
+    pairs = np.stack([l, r], axis=1) if l.size else np.empty((0, 2), np.int64)
+    if pairs.shape[0]:
+        order = np.lexsort((pairs[:, 1], pairs[:, 0]))
+        pairs = pairs[order]
+    return pairs
+
+
+# --------------------------------------------------------------- gather --
+
+def gather_with_null(src, idx) -> np.ndarray:
+    """conversion.py:39-50 for 1-D int64 rows."""
+    s, i = _i64(src), _i64(idx)
+    out = np.empty(max(i.shape[0], 1), np.int64)
+    _lib().o_gather_i64(_p(s), s.shape[0], _p(i), i.shape[0], _p(out))
+    return out[:i.shape[0]].copy()
+
+
+# ------------------------------------------------------------- group-by --
+
+def group_sum_count(gid, val, G: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Exact int64 SUM and COUNT per group (operator_utils.py:342-399)."""
+    g = np.ascontiguousarray(np.asarray(gid), dtype=np.int32)
+    v = _i64(val)
+    s = np.empty(G, np.int64)
+    c = np.empty(G, np.int64)
+    _lib().o_group_sum_count(_p(g), _p(v), g.shape[0], G, _p(s), _p(c))
+    return s, c

@@ -1,0 +1,112 @@
+"""The C ABI library (include/pageflow_join.h) through ctypes.
+
+The library is built in tree by ``pageflow_amd.build`` (or
+``__graft_entry__.build()``).  There is no fallback: a missing or stale
+library raises, so a call never silently runs anything but the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from typing import Optional
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpageflow_join.so")
+ABI_VERSION = 1
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+I32 = ctypes.c_int32
+SZ = ctypes.c_size_t
+PP = ctypes.POINTER(ctypes.c_void_p)
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "pf_version": (ctypes.c_int, []),
+    "pf_last_error": (ctypes.c_char_p, []),
+    "pf_mix64": (ctypes.c_int, [P, I64, P, P]),
+    "pf_bucket_ids": (ctypes.c_int, [P, I64, I32, P, P]),
+    "pf_partition_sort_workspace_bytes": (SZ, [I64, I64, I32, I32, I32, I32]),
+    "pf_partition_sort": (ctypes.c_int, [PP, PP, P, P, I32, I32, P, I64, I64, I64, I32, I32,
+                                         P, P, P, SZ, P]),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+_lock = threading.Lock()
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def declared_symbols():
+    return list(_SIGNATURES)
+
+
+def load() -> ctypes.CDLL:
+    """The library, loaded once; raises LibraryMissing when it is not
+    built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(
+                f"{LIB_PATH} is not built: run `python -m pageflow_amd.build` "
+                "(the join path has no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if lib.pf_version() != ABI_VERSION:
+            raise LibraryMissing(
+                f"{LIB_PATH} has ABI {lib.pf_version()}, expected {ABI_VERSION}: rebuild it")
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().pf_last_error().decode(errors="replace")
+        raise RuntimeError(msg or f"pageflow_join call failed ({rc})")
+
+
+def ptr(t: Optional[torch.Tensor]):
+    """A tensor's device address for the C ABI (None for None)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_of(device: torch.device):
+    """The current torch stream of ``device`` as a hipStream_t."""
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def require_cuda(t: torch.Tensor, what: str) -> None:
+    if not t.is_cuda:
+        raise ValueError(f"{what} must be a GPU tensor (the join path has no CPU fallback)")
+
+
+def workspace(nbytes: int, device: torch.device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def void_array(ptrs):
+    arr = (ctypes.c_void_p * max(len(ptrs), 1))()
+    for i, p in enumerate(ptrs):
+        arr[i] = p
+    return arr
+
+
+def i32_array(vals):
+    arr = (ctypes.c_int32 * max(len(vals), 1))()
+    for i, v in enumerate(vals):
+        arr[i] = int(v)
+    return arr
