@@ -74,6 +74,58 @@ int pf_partition_sort(const void* const* cols_in, void* const* cols_out,
                       int64_t* host_out, void* workspace, size_t workspace_bytes,
                       void* stream);
 
+/* ------------------------------------------------------- unique hash --
+ * Replaces tqpkernels.join_unique_hash (operators/kernels/join_core.cu:301-351;
+ * Python caller operators/join_core.py:110-111).
+ * Inner join on a unique build tuple of nkeys (1 or 2) int64 columns
+ * (b1/p1 ignored for 1 key), nb, np >= 1.  build_idx/probe_idx: int64[np];
+ * the first host_out[0] hold the matches in probe-row order, a probe row
+ * taking the lowest build row with its tuple.  host_out: pinned int64[2].
+ * Waits once. */
+size_t pf_join_unique_hash_workspace_bytes(int64_t nb, int64_t np, int32_t nkeys);
+int pf_join_unique_hash(const int64_t* b0, const int64_t* b1, int64_t nb, const int64_t* p0,
+                        const int64_t* p1, int64_t np, int32_t nkeys, int64_t* build_idx,
+                        int64_t* probe_idx, int64_t* host_out, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------- semi / anti --
+ * Replaces tqpkernels.join_semi_anti (operators/kernels/join_core.cu:358-400;
+ * Python caller operators/join_core.py:138-139).
+ * op: 0 none, 1 '<>', 2 '<', 3 '<=', 4 '>', 5 '>=' (build.v OP probe.v,
+ * bv/pv int64 value columns when op != 0).  out: int64[np], the first
+ * host_out[0] the kept probe rows ascending; host_out[1] = 1 when a build
+ * key equals int64 min (the result is then not valid).  Waits once. */
+size_t pf_join_semi_anti_workspace_bytes(int64_t nb, int64_t np, int32_t op);
+int pf_join_semi_anti(const int64_t* bk, int64_t nb, const int64_t* pk, int64_t np,
+                      const int64_t* bv, const int64_t* pv, int32_t op, int32_t anti,
+                      int64_t* out, int64_t* host_out, void* workspace, size_t workspace_bytes,
+                      void* stream);
+
+/* The same predicate as a row mask (no counterpart in the reference: the
+ * mask form of semi/anti output for the skewed-join workload): mask_out[i] =
+ * 1 when probe row i is kept.  bad_out (pinned, optional) receives the
+ * int64-min-key flag.  No host wait. */
+size_t pf_semi_anti_mask_workspace_bytes(int64_t nb, int64_t np, int32_t op);
+int pf_semi_anti_mask(const int64_t* bk, int64_t nb, const int64_t* pk, int64_t np,
+                      const int64_t* bv, const int64_t* pv, int32_t op, int32_t anti,
+                      uint8_t* mask_out, int64_t* bad_out, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------ expand --
+ * Replaces tqpkernels.join_expand (operators/kernels/join_core.cu:263-295;
+ * Python caller operators/join_core.py:60-61), split in two because the
+ * caller owns the outputs: the count call scans len (m rows) into the
+ * workspace and waits for the pair count (host_out[0]); the caller sizes
+ * cs/cr (int64[n_flat]) and the write call, on the same workspace, writes
+ * (sorted_ids[start[i] + j], i) for every probe row i and j < len[i], in
+ * probe-row then run order. */
+size_t pf_join_expand_workspace_bytes(int64_t m);
+int pf_join_expand_count(const int64_t* len, int64_t m, int64_t* host_out, void* workspace,
+                         size_t workspace_bytes, void* stream);
+int pf_join_expand_write(const int64_t* sorted_ids, const int64_t* start, int64_t m,
+                         int64_t n_flat, int64_t* cs, int64_t* cr, void* workspace,
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

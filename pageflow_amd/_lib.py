@@ -32,6 +32,15 @@ _SIGNATURES = {
     "pf_partition_sort_workspace_bytes": (SZ, [I64, I64, I32, I32, I32, I32]),
     "pf_partition_sort": (ctypes.c_int, [PP, PP, P, P, I32, I32, P, I64, I64, I64, I32, I32,
                                          P, P, P, SZ, P]),
+    "pf_join_unique_hash_workspace_bytes": (SZ, [I64, I64, I32]),
+    "pf_join_unique_hash": (ctypes.c_int, [P, P, I64, P, P, I64, I32, P, P, P, P, SZ, P]),
+    "pf_join_semi_anti_workspace_bytes": (SZ, [I64, I64, I32]),
+    "pf_join_semi_anti": (ctypes.c_int, [P, I64, P, I64, P, P, I32, I32, P, P, P, SZ, P]),
+    "pf_semi_anti_mask_workspace_bytes": (SZ, [I64, I64, I32]),
+    "pf_semi_anti_mask": (ctypes.c_int, [P, I64, P, I64, P, P, I32, I32, P, P, P, SZ, P]),
+    "pf_join_expand_workspace_bytes": (SZ, [I64]),
+    "pf_join_expand_count": (ctypes.c_int, [P, I64, P, P, SZ, P]),
+    "pf_join_expand_write": (ctypes.c_int, [P, P, I64, I64, P, P, P, P]),
 }
 
 _lib: Optional[ctypes.CDLL] = None

@@ -1,0 +1,582 @@
+// The join core's per-call kernels: drop-ins for the reference's
+// join_unique_hash, join_semi_anti and join_expand entry points
+// (operators/kernels/join_core.cu:263-400; Python contract
+// operators/join_core.py:54-143).
+//
+//   unique hash  inner join on a build tuple (1-2 int64 keys) proven unique:
+//                an open-addressing table at load <= 0.5, one partner per
+//                probe row (the lowest build row with the tuple), the matched
+//                probe rows compacted stably (probe-row order)
+//   semi / anti  a hash set of the build keys with an optional per-key
+//                min/max of a build value, the probe rows with (semi) or
+//                without (anti) a passing match compacted stably
+//   expand       each probe row's run of a sorted build written as (build
+//                row, probe row) pairs in probe-row order
+//
+// gfx950 design: 1-key tables keep the key in the slot (one 8-B probe load
+// per step, no dependent load of the build key), slots come from the low 32
+// bits of mix64 by multiply-shift, inserts read before their 64-bit CAS.
+// Compaction is two passes over 4096-row tiles (predicate + tile counts,
+// device scan, ballot-ranked writes), so the order is deterministic; no CUB.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "pf_common.hpp"
+#include "../../include/pageflow_join.h"
+
+namespace pf {
+namespace {
+
+constexpr int CT_NT = 256;          // threads a compaction tile
+constexpr int CT_IT = 16;           // rounds
+constexpr int CT_TILE = CT_NT * CT_IT;
+constexpr int CT_W = CT_NT / kWave;
+constexpr int64_t MAX_ROWS = (1LL << 31) - (1LL << 16);  // operators/join_core.py:30
+
+enum Op : int { OP_NONE = 0, OP_NE = 1, OP_LT = 2, OP_LE = 3, OP_GT = 4, OP_GE = 5 };
+
+// Block-wide sum (every thread gets it).
+__device__ __forceinline__ int block_sum_ct(int v, int* red) {
+  v = wave_sum(v);
+  if (lane_id() == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
+  int t = 0;
+#pragma unroll
+  for (int w = 0; w < CT_W; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+// -------------------------------------------------------- unique hash --
+
+// 1 key: the slot holds the key (empty = int64 min) and the lowest build
+// row with it.  The key int64 min itself cannot be stored; its lowest row
+// goes to *special.
+__global__ void k_uh_build1(const int64_t* __restrict__ b0, int64_t nb, int64_t* hkey,
+                            int32_t* hrow, uint32_t cap, int32_t* special) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int64_t k = b0[i];
+  if (k == kEmptyKey) {
+    atomicMin(special, (int32_t)i);
+    return;
+  }
+  uint32_t s = slot_of(mix64((uint64_t)k), cap);
+  for (;;) {
+    int64_t prev = hkey[s];
+    if (prev == kEmptyKey)
+      prev = (int64_t)atomicCAS((unsigned long long*)&hkey[s], (unsigned long long)kEmptyKey,
+                                (unsigned long long)k);
+    if (prev == kEmptyKey || prev == k) {
+      atomicMin(&hrow[s], (int32_t)i);
+      return;
+    }
+    s = next_slot(s, cap);
+  }
+}
+
+// 2 keys: the slot holds a build row (-1 empty); the tuple is compared
+// through the build columns.  A repeated tuple keeps its lowest row.
+__global__ void k_uh_build2(const int64_t* __restrict__ b0, const int64_t* __restrict__ b1,
+                            int64_t nb, int32_t* hrow, uint32_t cap) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int64_t k0 = b0[i], k1 = b1[i];
+  uint32_t s = slot_of(tuple_hash(k0, k1, true), cap);
+  for (;;) {
+    int32_t r = hrow[s];
+    if (r == -1) r = atomicCAS(&hrow[s], -1, (int32_t)i);
+    if (r == -1) return;
+    if (b0[r] == k0 && b1[r] == k1) {
+      atomicMin(&hrow[s], (int32_t)i);
+      return;
+    }
+    s = next_slot(s, cap);
+  }
+}
+
+// partner[i]: probe row i's build row or -1; tilecnt[t]: tile t's matches.
+template <bool TWO>
+__global__ __launch_bounds__(CT_NT) void k_uh_probe(
+    const int64_t* __restrict__ b0, const int64_t* __restrict__ b1,
+    const int64_t* __restrict__ p0, const int64_t* __restrict__ p1, int64_t np,
+    const int64_t* __restrict__ hkey, const int32_t* __restrict__ hrow, uint32_t cap,
+    const int32_t* __restrict__ special, int32_t* __restrict__ partner,
+    int32_t* __restrict__ tilecnt) {
+  __shared__ int red[CT_W];
+  const int64_t base = (int64_t)blockIdx.x * CT_TILE;
+  const int32_t sp = TWO ? -1 : *special;
+  int c = 0;
+#pragma unroll 4
+  for (int j = 0; j < CT_IT; ++j) {
+    const int64_t i = base + j * CT_NT + threadIdx.x;
+    if (i >= np) break;
+    const int64_t k0 = p0[i];
+    int32_t r = -1;
+    if (TWO) {
+      const int64_t k1 = p1[i];
+      uint32_t s = slot_of(tuple_hash(k0, k1, true), cap);
+      for (;;) {
+        const int32_t b = hrow[s];
+        if (b == -1) break;
+        if (b0[b] == k0 && b1[b] == k1) {
+          r = b;
+          break;
+        }
+        s = next_slot(s, cap);
+      }
+    } else if (k0 == kEmptyKey) {
+      r = sp == INT32_MAX ? -1 : sp;
+    } else {
+      uint32_t s = slot_of(mix64((uint64_t)k0), cap);
+      for (;;) {
+        const int64_t t = hkey[s];
+        if (t == k0) {
+          r = hrow[s];
+          break;
+        }
+        if (t == kEmptyKey) break;
+        s = next_slot(s, cap);
+      }
+    }
+    partner[i] = r;
+    c += r >= 0;
+  }
+  c = block_sum_ct(c, red);
+  if (threadIdx.x == 0) tilecnt[blockIdx.x] = c;
+}
+
+// --------------------------------------------------- stable compaction --
+
+// Positions of a tile's kept rows (row order): rounds j-major, lanes in
+// order.  keep[j]: row (base + j*CT_NT + tid) is kept.  Returns, per round,
+// the output index of the row (only meaningful where kept).
+__device__ __forceinline__ void tile_positions(const bool (&keep)[CT_IT], int64_t off,
+                                               int64_t (&pos)[CT_IT], int* cnt) {
+  const int lane = lane_id(), w = threadIdx.x / kWave;
+  int rank[CT_IT];
+#pragma unroll
+  for (int j = 0; j < CT_IT; ++j) {
+    const uint64_t bal = ballot(keep[j]);
+    rank[j] = mbcnt(bal);
+    if (lane == 0) cnt[j * CT_W + w] = __popcll(bal);
+  }
+  __syncthreads();
+  if (w == 0) {  // CT_IT * CT_W = 64 counts, (round, wave) order
+    const int v = cnt[lane];
+    const int incl = wave_incl_scan(v);
+    cnt[lane] = incl - v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < CT_IT; ++j) pos[j] = off + cnt[j * CT_W + w] + rank[j];
+}
+static_assert(CT_IT * CT_W == kWave, "one wave scans a tile's (round, wave) counts");
+
+// probe_idx / build_idx of the matched probe rows at their scanned tile
+// offsets.
+__global__ __launch_bounds__(CT_NT) void k_uh_compact(const int32_t* __restrict__ partner,
+                                                      int64_t np,
+                                                      const int64_t* __restrict__ offs,
+                                                      int64_t* __restrict__ probe_idx,
+                                                      int64_t* __restrict__ build_idx) {
+  __shared__ int cnt[kWave];
+  const int64_t base = (int64_t)blockIdx.x * CT_TILE;
+  bool keep[CT_IT];
+  int32_t r[CT_IT];
+#pragma unroll
+  for (int j = 0; j < CT_IT; ++j) {
+    const int64_t i = base + j * CT_NT + threadIdx.x;
+    r[j] = i < np ? partner[i] : -1;
+    keep[j] = r[j] >= 0;
+  }
+  int64_t pos[CT_IT];
+  tile_positions(keep, offs[blockIdx.x], pos, cnt);
+#pragma unroll
+  for (int j = 0; j < CT_IT; ++j) {
+    if (keep[j]) {
+      probe_idx[pos[j]] = base + j * CT_NT + threadIdx.x;
+      build_idx[pos[j]] = r[j];
+    }
+  }
+}
+
+// kept probe rows of a uint8 keep array at their scanned tile offsets.
+__global__ __launch_bounds__(CT_NT) void k_rows_compact(const uint8_t* __restrict__ keepv,
+                                                        int64_t n,
+                                                        const int64_t* __restrict__ offs,
+                                                        int64_t* __restrict__ out) {
+  __shared__ int cnt[kWave];
+  const int64_t base = (int64_t)blockIdx.x * CT_TILE;
+  bool keep[CT_IT];
+#pragma unroll
+  for (int j = 0; j < CT_IT; ++j) {
+    const int64_t i = base + j * CT_NT + threadIdx.x;
+    keep[j] = i < n && keepv[i];
+  }
+  int64_t pos[CT_IT];
+  tile_positions(keep, offs[blockIdx.x], pos, cnt);
+#pragma unroll
+  for (int j = 0; j < CT_IT; ++j)
+    if (keep[j]) out[pos[j]] = base + j * CT_NT + threadIdx.x;
+}
+
+__global__ void k_readback(const int64_t* __restrict__ dev, int n, int64_t* host) {
+  if ((int)threadIdx.x < n) store_host(&host[threadIdx.x], dev[threadIdx.x]);
+}
+
+// --------------------------------------------------------- semi / anti --
+
+// Insert the key (read before the CAS, so repeats neither CAS nor contend)
+// and fold the build value into the slot's min and/or max.  A key equal to
+// the empty-slot value sets *bad (the caller falls back).
+__global__ void k_set_build(const int64_t* __restrict__ key, const int64_t* __restrict__ val,
+                            int64_t n, int64_t* skey, int64_t* mn, int64_t* mx, uint32_t cap,
+                            int64_t* bad) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t k = key[i];
+  if (k == kEmptyKey) {
+    *bad = 1;
+    return;
+  }
+  uint32_t s = slot_of(mix64((uint64_t)k), cap);
+  for (;;) {
+    int64_t prev = skey[s];
+    if (prev == kEmptyKey)
+      prev = (int64_t)atomicCAS((unsigned long long*)&skey[s], (unsigned long long)kEmptyKey,
+                                (unsigned long long)k);
+    if (prev == kEmptyKey || prev == k) break;
+    s = next_slot(s, cap);
+  }
+  if (mn || mx) {
+    const int64_t v = val[i];
+    // a stale read is never past the atomic's result: min only falls
+    if (mn && v < mn[s]) atomicMin((long long*)&mn[s], (long long)v);
+    if (mx && v > mx[s]) atomicMax((long long*)&mx[s], (long long)v);
+  }
+}
+
+__device__ __forceinline__ bool set_hit(int64_t k, const int64_t* __restrict__ skey,
+                                        const int64_t* __restrict__ mn,
+                                        const int64_t* __restrict__ mx, uint32_t cap, int op,
+                                        int64_t v) {
+  if (k == kEmptyKey) return false;
+  uint32_t s = slot_of(mix64((uint64_t)k), cap);
+  for (;;) {
+    const int64_t t = skey[s];
+    if (t == k) break;
+    if (t == kEmptyKey) return false;
+    s = next_slot(s, cap);
+  }
+  switch (op) {
+    case OP_NONE: return true;
+    case OP_NE: return mn[s] != v || mx[s] != v;
+    case OP_LT: return mn[s] < v;
+    case OP_LE: return mn[s] <= v;
+    case OP_GT: return mx[s] > v;
+    default: return mx[s] >= v;
+  }
+}
+
+// keep[i] for probe row i, and tile counts (when tilecnt is not null).
+__global__ __launch_bounds__(CT_NT) void k_set_probe(
+    const int64_t* __restrict__ pk, const int64_t* __restrict__ pv, int64_t np,
+    const int64_t* __restrict__ skey, const int64_t* __restrict__ mn,
+    const int64_t* __restrict__ mx, uint32_t cap, int op, int anti,
+    uint8_t* __restrict__ keepv, int32_t* __restrict__ tilecnt) {
+  __shared__ int red[CT_W];
+  const int64_t base = (int64_t)blockIdx.x * CT_TILE;
+  int c = 0;
+#pragma unroll 4
+  for (int j = 0; j < CT_IT; ++j) {
+    const int64_t i = base + j * CT_NT + threadIdx.x;
+    if (i >= np) break;
+    const bool hit = set_hit(pk[i], skey, mn, mx, cap, op, op ? pv[i] : 0);
+    const bool keep = hit != (anti != 0);
+    keepv[i] = keep;
+    c += keep;
+  }
+  if (tilecnt) {
+    c = block_sum_ct(c, red);
+    if (threadIdx.x == 0) tilecnt[blockIdx.x] = c;
+  }
+}
+
+// -------------------------------------------------------------- expand --
+
+// Pair k belongs to the last probe row i with ex[i] <= k (that row has a
+// non-empty run); its build row is sorted_ids[start[i] + k - ex[i]].
+constexpr int EX_NT = 256, EX_PER = 8, EX_TILE = EX_NT * EX_PER;
+
+__device__ __forceinline__ int64_t last_le(const int64_t* __restrict__ ex, int64_t lo,
+                                           int64_t hi, int64_t k) {
+  // largest i in [lo, hi) with ex[i] <= k (ex[lo] <= k assumed)
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (ex[mid] <= k)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(EX_NT) void k_expand(const int64_t* __restrict__ sorted_ids,
+                                                  const int64_t* __restrict__ start,
+                                                  const int64_t* __restrict__ ex, int64_t m,
+                                                  int64_t n_flat, int64_t* __restrict__ cs,
+                                                  int64_t* __restrict__ cr) {
+  __shared__ int64_t range[2];
+  const int64_t k0 = (int64_t)blockIdx.x * EX_TILE;
+  const int64_t k1 = (k0 + EX_TILE < n_flat ? k0 + EX_TILE : n_flat) - 1;
+  if (threadIdx.x < 2) range[threadIdx.x] = last_le(ex, 0, m, threadIdx.x ? k1 : k0);
+  __syncthreads();
+  const int64_t r0 = range[0], r1 = range[1] + 1;
+#pragma unroll
+  for (int q = 0; q < EX_PER; ++q) {
+    const int64_t k = k0 + q * EX_NT + threadIdx.x;
+    if (k >= n_flat) break;
+    const int64_t i = last_le(ex, r0, r1, k);
+    cs[k] = sorted_ids[start[i] + (k - ex[i])];
+    cr[k] = i;
+  }
+}
+
+// ----------------------------------------------------------- layouts --
+
+struct UniqueWs {
+  int64_t* hkey;
+  int32_t* hrow;
+  int32_t* special;
+  int32_t* partner;
+  int32_t* tilecnt;
+  int64_t* offs;
+  int64_t* count;
+  void* scan;
+  size_t bytes;
+};
+
+UniqueWs carve_unique(void* ws, int64_t nb, int64_t np, bool two) {
+  Carve cv(ws);
+  UniqueWs u;
+  const int64_t cap = 2 * (nb > 0 ? nb : 1);
+  const int64_t tiles = (np + CT_TILE - 1) / CT_TILE;
+  u.hkey = cv.take<int64_t>(two ? 1 : cap);
+  u.hrow = cv.take<int32_t>(cap);
+  u.special = cv.take<int32_t>(1);
+  u.partner = cv.take<int32_t>(np > 0 ? np : 1);
+  u.tilecnt = cv.take<int32_t>(tiles > 0 ? tiles : 1);
+  u.offs = cv.take<int64_t>(tiles > 0 ? tiles : 1);
+  u.count = cv.take<int64_t>(2);
+  u.scan = cv.take<char>(scan_i32_scratch(tiles));
+  u.bytes = cv.bytes();
+  return u;
+}
+
+struct SetWs {
+  int64_t* skey;
+  int64_t* mn;
+  int64_t* mx;
+  uint8_t* keep;
+  int32_t* tilecnt;
+  int64_t* offs;
+  int64_t* dev;  // [0] kept rows, [1] bad
+  void* scan;
+  size_t bytes;
+};
+
+SetWs carve_set(void* ws, int64_t nb, int64_t np, int op, bool compact) {
+  Carve cv(ws);
+  SetWs s;
+  const int64_t cap = 2 * (nb > 0 ? nb : 1);
+  const int64_t tiles = (np + CT_TILE - 1) / CT_TILE;
+  s.skey = cv.take<int64_t>(cap);
+  const bool need_mn = op == OP_NE || op == OP_LT || op == OP_LE;
+  const bool need_mx = op == OP_NE || op == OP_GT || op == OP_GE;
+  s.mn = cv.take<int64_t>(need_mn ? cap : 1);
+  s.mx = cv.take<int64_t>(need_mx ? cap : 1);
+  if (!need_mn) s.mn = nullptr;
+  if (!need_mx) s.mx = nullptr;
+  s.keep = cv.take<uint8_t>(compact ? (np > 0 ? np : 1) : 1);
+  s.tilecnt = cv.take<int32_t>(tiles > 0 ? tiles : 1);
+  s.offs = cv.take<int64_t>(tiles > 0 ? tiles : 1);
+  s.dev = cv.take<int64_t>(2);
+  s.scan = cv.take<char>(scan_i32_scratch(tiles));
+  s.bytes = cv.bytes();
+  return s;
+}
+
+// Builds a semi/anti set into s (fills, inserts); shared by the compacted
+// and the mask entry points.
+int build_set(const SetWs& s, const int64_t* bk, const int64_t* bv, int64_t nb, int op,
+              hipStream_t st) {
+  const int64_t cap = 2 * nb;
+  if (fill_i64(s.skey, cap, kEmptyKey, st)) return 1;
+  if (s.mn && fill_i64(s.mn, cap, INT64_MAX, st)) return 1;
+  if (s.mx && fill_i64(s.mx, cap, INT64_MIN, st)) return 1;
+  if (fill_i64(s.dev, 2, 0, st)) return 1;
+  k_set_build<<<grid_for(nb, 256), 256, 0, st>>>(bk, op ? bv : nullptr, nb, s.skey, s.mn,
+                                                   s.mx, (uint32_t)cap, s.dev + 1);
+  return launch_check("semi_anti build");
+}
+
+}  // namespace
+}  // namespace pf
+
+extern "C" {
+
+size_t pf_join_unique_hash_workspace_bytes(int64_t nb, int64_t np, int32_t nkeys) {
+  return pf::carve_unique(nullptr, nb, np, nkeys == 2).bytes;
+}
+
+int pf_join_unique_hash(const int64_t* b0, const int64_t* b1, int64_t nb, const int64_t* p0,
+                        const int64_t* p1, int64_t np, int32_t nkeys, int64_t* build_idx,
+                        int64_t* probe_idx, int64_t* host_out, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (nkeys != 1 && nkeys != 2) return fail("join_core: one or two aligned build/probe keys");
+  if (nb <= 0 || np <= 0) return fail("join_core: unique hash needs rows on both sides");
+  if (nb > MAX_ROWS || np > MAX_ROWS)
+    return fail("join_core: a call takes at most " + std::to_string(MAX_ROWS) + " rows a side");
+  const bool two = nkeys == 2;
+  UniqueWs u = carve_unique(workspace, nb, np, two);
+  if (workspace_bytes < u.bytes) return fail("join_core: workspace too small");
+  void* hdev = nullptr;
+  if (host_alias(host_out, &hdev)) return 1;
+  const uint32_t cap = (uint32_t)(2 * nb);
+  const int64_t tiles = (np + CT_TILE - 1) / CT_TILE;
+  if (two) {
+    if (fill_i32(u.hrow, cap, -1, st)) return 1;
+    k_uh_build2<<<grid_for(nb, 256), 256, 0, st>>>(b0, b1, nb, u.hrow, cap);
+    k_uh_probe<true><<<(unsigned)tiles, CT_NT, 0, st>>>(b0, b1, p0, p1, np, u.hkey, u.hrow,
+                                                        cap, u.special, u.partner, u.tilecnt);
+  } else {
+    if (fill_i64(u.hkey, cap, kEmptyKey, st)) return 1;
+    if (fill_i32(u.hrow, cap, INT32_MAX, st)) return 1;
+    if (fill_i32(u.special, 1, INT32_MAX, st)) return 1;
+    k_uh_build1<<<grid_for(nb, 256), 256, 0, st>>>(b0, nb, u.hkey, u.hrow, cap, u.special);
+    k_uh_probe<false><<<(unsigned)tiles, CT_NT, 0, st>>>(b0, b0, p0, p0, np, u.hkey, u.hrow,
+                                                         cap, u.special, u.partner,
+                                                         u.tilecnt);
+  }
+  if (launch_check("join_unique_hash")) return 1;
+  if (scan_i32_to_i64_excl(u.tilecnt, u.offs, tiles, u.count, u.scan, st)) return 1;
+  k_readback<<<1, 64, 0, st>>>(u.count, 1, (int64_t*)hdev);
+  k_uh_compact<<<(unsigned)tiles, CT_NT, 0, st>>>(u.partner, np, u.offs, probe_idx, build_idx);
+  if (launch_check("join_unique_hash")) return 1;
+  if (wait_stream(st, "join_unique_hash")) return 1;
+  return ok();
+}
+
+size_t pf_join_semi_anti_workspace_bytes(int64_t nb, int64_t np, int32_t op) {
+  return pf::carve_set(nullptr, nb, np, op, true).bytes;
+}
+
+int pf_join_semi_anti(const int64_t* bk, int64_t nb, const int64_t* pk, int64_t np,
+                      const int64_t* bv, const int64_t* pv, int32_t op, int32_t anti,
+                      int64_t* out, int64_t* host_out, void* workspace, size_t workspace_bytes,
+                      void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (nb <= 0 || np <= 0) return fail("join_core: semi/anti needs rows on both sides");
+  if (np > MAX_ROWS || nb > MAX_ROWS)
+    return fail("join_core: a call takes at most " + std::to_string(MAX_ROWS) + " rows a side");
+  if (op < OP_NONE || op > OP_GE)
+    return fail("join_core: unknown residual operator " + std::to_string(op));
+  if (op != OP_NONE && (!bv || !pv)) return fail("join_core: a residual needs both value columns");
+  SetWs s = carve_set(workspace, nb, np, op, true);
+  if (workspace_bytes < s.bytes) return fail("join_core: workspace too small");
+  void* hdev = nullptr;
+  if (host_alias(host_out, &hdev)) return 1;
+  if (build_set(s, bk, bv, nb, op, st)) return 1;
+  const int64_t tiles = (np + CT_TILE - 1) / CT_TILE;
+  k_set_probe<<<(unsigned)tiles, CT_NT, 0, st>>>(pk, pv, np, s.skey, s.mn, s.mx,
+                                                 (uint32_t)(2 * nb), op, anti, s.keep,
+                                                 s.tilecnt);
+  if (launch_check("join_semi_anti")) return 1;
+  if (scan_i32_to_i64_excl(s.tilecnt, s.offs, tiles, s.dev, s.scan, st)) return 1;
+  k_readback<<<1, 64, 0, st>>>(s.dev, 2, (int64_t*)hdev);
+  k_rows_compact<<<(unsigned)tiles, CT_NT, 0, st>>>(s.keep, np, s.offs, out);
+  if (launch_check("join_semi_anti")) return 1;
+  if (wait_stream(st, "join_semi_anti")) return 1;
+  return ok();
+}
+
+size_t pf_semi_anti_mask_workspace_bytes(int64_t nb, int64_t np, int32_t op) {
+  return pf::carve_set(nullptr, nb, np, op, false).bytes;
+}
+
+int pf_semi_anti_mask(const int64_t* bk, int64_t nb, const int64_t* pk, int64_t np,
+                      const int64_t* bv, const int64_t* pv, int32_t op, int32_t anti,
+                      uint8_t* mask_out, int64_t* bad_out, void* workspace,
+                      size_t workspace_bytes, void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (nb <= 0 || np < 0) return fail("semi_anti_mask: needs build rows");
+  if (nb > MAX_ROWS) return fail("semi_anti_mask: too many build rows");
+  if (op < OP_NONE || op > OP_GE)
+    return fail("semi_anti_mask: unknown residual operator " + std::to_string(op));
+  if (op != OP_NONE && (!bv || !pv)) return fail("semi_anti_mask: a residual needs both columns");
+  SetWs s = carve_set(workspace, nb, np, op, false);
+  if (workspace_bytes < s.bytes) return fail("semi_anti_mask: workspace too small");
+  if (build_set(s, bk, bv, nb, op, st)) return 1;
+  if (np > 0) {
+    const int64_t tiles = (np + CT_TILE - 1) / CT_TILE;
+    k_set_probe<<<(unsigned)tiles, CT_NT, 0, st>>>(pk, pv, np, s.skey, s.mn, s.mx,
+                                                   (uint32_t)(2 * nb), op, anti, mask_out,
+                                                   nullptr);
+    if (launch_check("semi_anti_mask")) return 1;
+  }
+  if (bad_out) {
+    k_readback<<<1, 64, 0, st>>>(s.dev + 1, 1, bad_out);
+    if (launch_check("semi_anti_mask")) return 1;
+  }
+  return ok();
+}
+
+size_t pf_join_expand_workspace_bytes(int64_t m) {
+  pf::Carve cv(nullptr);
+  cv.take<int64_t>(m > 0 ? m : 1);
+  cv.take<int64_t>(2);
+  cv.take<char>(pf::scan_i64_scratch(m));
+  return cv.bytes();
+}
+
+int pf_join_expand_count(const int64_t* len, int64_t m, int64_t* host_out, void* workspace,
+                         size_t workspace_bytes, void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (m < 0 || m > MAX_ROWS) return fail("join_core: start/len rows out of range");
+  if (workspace_bytes < pf_join_expand_workspace_bytes(m))
+    return fail("join_core: workspace too small");
+  void* hdev = nullptr;
+  if (host_alias(host_out, &hdev)) return 1;
+  Carve cv(workspace);
+  int64_t* ex = cv.take<int64_t>(m > 0 ? m : 1);
+  int64_t* total = cv.take<int64_t>(2);
+  void* scan = cv.take<char>(scan_i64_scratch(m));
+  if (scan_i64_excl_from(len, ex, m, total, scan, st)) return 1;
+  k_readback<<<1, 64, 0, st>>>(total, 1, (int64_t*)hdev);
+  if (launch_check("join_expand")) return 1;
+  // the one wait: the pair count sizes the outputs
+  if (wait_stream(st, "join_expand")) return 1;
+  return ok();
+}
+
+int pf_join_expand_write(const int64_t* sorted_ids, const int64_t* start, int64_t m,
+                         int64_t n_flat, int64_t* cs, int64_t* cr, void* workspace,
+                         void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (n_flat <= 0 || m <= 0) return ok();
+  const int64_t* ex = (const int64_t*)workspace;  // the count call's exclusive offsets
+  k_expand<<<grid_for(n_flat, EX_TILE), EX_NT, 0, st>>>(sorted_ids, start, ex, m, n_flat, cs,
+                                                        cr);
+  if (launch_check("join_expand")) return 1;
+  return ok();
+}
+
+}  // extern "C"

@@ -158,6 +158,8 @@ inline unsigned grid_for(int64_t n, int per_block) {
 // the scratch a call takes.
 size_t scan_i64_scratch(int64_t n);
 int scan_i64_excl(int64_t* data, int64_t n, int64_t* total, void* scratch, hipStream_t st);
+int scan_i64_excl_from(const int64_t* in, int64_t* out, int64_t n, int64_t* total,
+                       void* scratch, hipStream_t st);
 // The same over int32 counts, producing int64 offsets (out may not alias in).
 size_t scan_i32_scratch(int64_t n);
 int scan_i32_to_i64_excl(const int32_t* in, int64_t* out, int64_t n, int64_t* total,

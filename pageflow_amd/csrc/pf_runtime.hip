@@ -200,6 +200,11 @@ int scan_i64_excl(int64_t* data, int64_t n, int64_t* total, void* scratch, hipSt
   return scan_impl<int64_t>(data, data, n, total, (int64_t*)scratch, st);
 }
 
+int scan_i64_excl_from(const int64_t* in, int64_t* out, int64_t n, int64_t* total,
+                       void* scratch, hipStream_t st) {
+  return scan_impl<int64_t>(in, out, n, total, (int64_t*)scratch, st);
+}
+
 int scan_i32_to_i64_excl(const int32_t* in, int64_t* out, int64_t n, int64_t* total,
                          void* scratch, hipStream_t st) {
   return scan_impl<int32_t>(in, out, n, total, (int64_t*)scratch, st);

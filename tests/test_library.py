@@ -1,0 +1,30 @@
+"""The C-ABI library: it loads, and it exports every entry point
+include/pageflow_join.h declares (no compute calls: this runs without a
+GPU)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+from pageflow_amd import _lib
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "pageflow_join.h")).read()
+    return sorted(set(re.findall(r"\b(pf_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = _declared()
+    assert names, "no entry point declared"
+    for name in names:
+        assert hasattr(lib, name), name
+    assert lib.pf_version() == _lib.ABI_VERSION
+
+
+def test_every_declared_symbol_has_a_binding():
+    assert set(_declared()) == set(_lib.declared_symbols())

@@ -1,0 +1,243 @@
+"""The CPU oracle pinned before it is trusted: against the golden vectors
+written from the reference's own mix64 twin, the known answers the
+reference's tests hold (tests/operators/test_unique_hash_join.py,
+tests/operators/test_join_core.py of the reference repository), brute force
+on small inputs, and what the mathematics fixes (fmix64 is a bijection)."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+I64 = np.iinfo(np.int64)
+
+
+# ---------------------------------------------------------------- mix64 --
+
+def test_mix64_matches_the_reference_twin_golden_vectors():
+    d = np.load(os.path.join(HERE, "golden", "mix64_golden.npz"))
+    assert np.array_equal(oracle.mix64(d["keys"]), d["mix64"])
+    assert np.array_equal(oracle.mix64_np(d["keys"]), d["mix64"])
+    assert np.array_equal(oracle.bucket_of(d["keys"], 512), d["bucket512"])
+    assert np.array_equal(oracle.bucket_of(d["keys"], 1), d["bucket1"])
+
+
+def _unmix64(h: np.ndarray) -> np.ndarray:
+    """The inverse of fmix64: xorshift-33 is its own inverse on 64 bits, and
+    each multiply is undone by the constant's inverse mod 2^64."""
+    inv1 = np.uint64(pow(0xff51afd7ed558ccd, -1, 1 << 64))
+    inv2 = np.uint64(pow(0xc4ceb9fe1a85ec53, -1, 1 << 64))
+    s = np.uint64(33)
+    k = h.copy()
+    with np.errstate(over="ignore"):
+        k ^= k >> s
+        k *= inv2
+        k ^= k >> s
+        k *= inv1
+        k ^= k >> s
+    return k
+
+
+def test_mix64_is_a_bijection_with_fixed_point_zero():
+    keys = np.random.default_rng(3).integers(I64.min, I64.max, 50_000, dtype=np.int64)
+    keys[:5] = [0, 1, -1, I64.min, I64.max]
+    h = oracle.mix64(keys)
+    assert np.array_equal(_unmix64(h).view(np.int64), keys)
+    assert int(oracle.mix64(np.array([0]))[0]) == 0
+    assert len(np.unique(h)) == len(np.unique(keys))
+
+
+def test_buckets_spread_a_linear_hash():
+    # the reference's spread check (test_agg_partition.py:70-80): 2^20
+    # multiples of a coefficient over 256 buckets within a quarter of the mean
+    h = np.arange(1, (1 << 20) + 1, dtype=np.int64) * 9_999_991
+    counts = np.bincount(oracle.bucket_of(h, 256), minlength=256)
+    assert counts.min() > 0.75 * 4096 and counts.max() < 1.25 * 4096
+
+
+# ------------------------------------------------------------ unique hash --
+
+def _first_slots(keys, cap):
+    """The reference's slot function (join_core.cu:56-59), as its test
+    computes it (test_unique_hash_join.py:80-83)."""
+    return [((int(h) << 9) % (1 << 64)) * cap >> 64 for h in oracle.mix64_np(keys)]
+
+
+def test_unique_hash_colliding_keys_known_answer():
+    cand = np.arange(1, 2048, dtype=np.int64)
+    slots = _first_slots(cand, 12)
+    bucket = next(s for s in range(12) if slots.count(s) >= 6)
+    coll = [int(cand[i]) for i, s in enumerate(slots) if s == bucket][:6]
+    probe = [coll[5], -999, coll[0], coll[3], coll[0]]
+    bi, pi = oracle.unique_hash_pairs([np.array(coll)], [np.array(probe)])
+    assert pi.tolist() == [0, 2, 3, 4]
+    assert bi.tolist() == [5, 0, 3, 0]
+
+
+def test_unique_hash_two_key_known_answer():
+    bi, pi = oracle.unique_hash_pairs(
+        [np.array([1, 1, 2, 2]), np.array([7, 8, 7, 8])],
+        [np.array([2, 1, 2, 3, 1, 1]), np.array([8, 8, 9, 7, 7, 7])])
+    assert pi.tolist() == [0, 1, 4, 5]
+    assert bi.tolist() == [3, 1, 0, 0]
+
+
+@pytest.mark.parametrize("b,p,eb,ep", [
+    ([], [1, 2], [], []), ([1, 2], [], [], []), ([1, 2], [3, 4], [], []),
+    ([1, 2, 3], [3, 2, 1], [2, 1, 0], [0, 1, 2])])
+def test_unique_hash_edges(b, p, eb, ep):
+    bi, pi = oracle.unique_hash_pairs([np.array(b, np.int64)], [np.array(p, np.int64)])
+    assert bi.tolist() == eb and pi.tolist() == ep
+
+
+def test_unique_hash_capacity_boundaries():
+    for nb in (7, 8, 9, 15, 16, 17, 31, 32, 33):
+        build = np.arange(nb, dtype=np.int64) * 17 + 3
+        probe = np.concatenate((build[::-1], [-1], build[:3]))
+        bi, pi = oracle.unique_hash_pairs([build], [probe])
+        assert bi.tolist() == list(range(nb - 1, -1, -1)) + [0, 1, 2]
+        assert pi.tolist() == list(range(nb)) + [nb + 1, nb + 2, nb + 3]
+
+
+def test_unique_hash_brute_force():
+    rng = np.random.default_rng(5)
+    build = rng.permutation(4000)[:1500].astype(np.int64) - 2000
+    probe = rng.integers(-2500, 2500, 3000)
+    bi, pi = oracle.unique_hash_pairs([build], [probe])
+    where = {int(k): i for i, k in enumerate(build)}
+    want = [(where[int(k)], i) for i, k in enumerate(probe) if int(k) in where]
+    assert list(zip(bi.tolist(), pi.tolist())) == want
+
+
+# ------------------------------------------------------------ semi / anti --
+
+_OPS = {'<>': np.not_equal, '<': np.less, '<=': np.less_equal, '>': np.greater,
+        '>=': np.greater_equal}
+
+
+def _dense_semi_anti(bk, pk, anti, bv=None, pv=None, op=None):
+    hit = bk[:, None] == pk[None, :]
+    if op is not None:
+        hit &= _OPS[op](bv[:, None], pv[None, :])
+    keep = hit.any(axis=0) if bk.size else np.zeros(pk.size, bool)
+    if anti:
+        keep = ~keep
+    return np.nonzero(keep)[0]
+
+
+@pytest.mark.parametrize("anti", [False, True])
+@pytest.mark.parametrize("op", [None, '<>', '<', '<=', '>', '>='])
+@pytest.mark.parametrize("nb,np_", [(0, 5), (5, 0), (1, 1), (7, 300), (2000, 1500)])
+def test_semi_anti_dense(anti, op, nb, np_):
+    rng = np.random.default_rng(nb * 7 + np_)
+    bk = rng.integers(-50, max(nb // 3, 1), nb)
+    pk = rng.integers(-60, 2 * max(nb // 3, 1), np_)
+    bv = rng.integers(0, 4, nb)
+    pv = rng.integers(0, 4, np_)
+    got = oracle.semi_anti(bk, pk, anti, (bv, pv, op) if op else None)
+    assert np.array_equal(got, _dense_semi_anti(bk, pk, anti, bv, pv, op))
+
+
+@pytest.mark.parametrize("anti", [False, True])
+def test_semi_anti_summary_known_answers(anti):
+    bk = np.array([1, 1, 1, 2, 2, 3])
+    pk = np.array([1, 2, 3, 4, 1])
+    same = np.full(6, 9)
+    got = oracle.semi_anti(bk, pk, anti, (same, np.full(5, 9), '<>'))
+    assert got.tolist() == ([0, 1, 2, 3, 4] if anti else [])
+    got = oracle.semi_anti(bk, pk, anti, (same, np.full(5, 5), '<>'))
+    assert got.tolist() == ([3] if anti else [0, 1, 2, 4])
+
+
+def test_semi_anti_extreme_keys_and_empty_slot_key():
+    keys = [I64.min + 1, -1, 0, -64, I64.max, 42]
+    bk = np.array(keys[:4], np.int64)
+    pk = np.array(keys + [I64.min], np.int64)
+    assert oracle.semi_anti(bk, pk, False).tolist() == [0, 1, 2, 3]
+    assert oracle.semi_anti(bk, pk, True).tolist() == [4, 5, 6]
+    assert oracle.semi_anti(np.array([3, I64.min]), np.array([3]), False) is None
+
+
+# ------------------------------------------------------------------ expand --
+
+def _expand_reference(sorted_ids, start, length):
+    rows = np.arange(length.size)
+    cr = np.repeat(rows, length)
+    first = np.repeat(start, length)
+    run_start = np.repeat(np.cumsum(length) - length, length)
+    within = np.arange(cr.size) - run_start
+    return sorted_ids[first + within], cr
+
+
+@pytest.mark.parametrize("lengths", [[], [0], [0, 0, 0], [1], [3], [1, 0, 2, 0, 0, 3, 0],
+                                     [0, 0, 5], [5, 0, 0], [2, 2, 2, 2]])
+def test_expand_hand_cases(lengths):
+    length = np.array(lengths, np.int64)
+    nb = max(8, int(length.sum()) + 3)
+    sorted_ids = np.random.default_rng(1).permutation(nb).astype(np.int64)
+    start = np.array([(3 * i) % (nb - n) if nb > n else 0 for i, n in enumerate(lengths)],
+                     np.int64)
+    cs, cr = oracle.expand(sorted_ids, start, length)
+    rcs, rcr = _expand_reference(sorted_ids, start, length)
+    assert np.array_equal(cs, rcs) and np.array_equal(cr, rcr)
+
+
+def test_sort_search_pairs_is_the_dense_join():
+    rng = np.random.default_rng(9)
+    lk = rng.integers(0, 40, 300)
+    rk = rng.integers(0, 60, 200)
+    got = oracle.join_pairs_sorted(lk, rk)
+    li, ri = np.nonzero(lk[:, None] == rk[None, :])
+    want = np.stack([li, ri], 1)
+    want = want[np.lexsort((want[:, 1], want[:, 0]))]
+    assert np.array_equal(got, want)
+
+
+# ------------------------------------------------------- partition/layout --
+
+def test_partition_is_a_stable_bucket_sort():
+    rng = np.random.default_rng(2)
+    keys = rng.integers(-(1 << 40), 1 << 40, 5000)
+    pay = rng.integers(0, 100, 5000)
+    mask = rng.random(5000) < 0.4
+    counts, outs = oracle.partition([keys, pay], 0, mask, 77, 16)
+    assert sum(counts) == mask.sum()
+    gidx = np.concatenate(outs[2])
+    kept = np.nonzero(mask)[0]
+    b = oracle.bucket_of(keys[kept], 16)
+    for bucket, rows in enumerate(outs[2]):
+        src = rows - 77
+        assert np.all(np.diff(src) > 0)
+        assert np.all(oracle.bucket_of(keys[src], 16) == bucket)
+    assert sorted(gidx.tolist()) == (kept + 77).tolist()
+    assert np.array_equal(np.bincount(b, minlength=16), counts)
+
+
+def test_padded_layout():
+    starts, total = oracle.padded_layout([0, 3, 64, 65, 0], 8)
+    assert starts == [0, 0, 64, 128, 256] and total == 256
+    starts, total = oracle.padded_layout([1, 1], 25)
+    assert starts == [0, 512] and total == 1024
+
+
+# --------------------------------------------------------------- group-by --
+
+def test_group_sum_count_brute_force():
+    rng = np.random.default_rng(4)
+    g = rng.integers(0, 25, 10_000).astype(np.int32)
+    v = rng.integers(100, 10_000_000, 10_000)
+    v[::97] = -64
+    s, c = oracle.group_sum_count(g, v, 25)
+    for k in range(25):
+        assert s[k] == int(sum(int(x) for x, gg in zip(v, g) if gg == k))
+        assert c[k] == sum(1 for x, gg in zip(v, g) if gg == k and x != -64)
+
+
+def test_gather_with_null():
+    src = np.array([10, 11, 12], np.int64)
+    assert oracle.gather_with_null(src, [2, -64, 0]).tolist() == [12, -64, 10]
+    assert oracle.gather_with_null(np.array([], np.int64), [-64, -64]).tolist() == [-64, -64]
