@@ -126,6 +126,48 @@ int pf_join_expand_write(const int64_t* sorted_ids, const int64_t* start, int64_
                          int64_t n_flat, int64_t* cs, int64_t* cr, void* workspace,
                          void* stream);
 
+/* --------------------------------------------------- radix partition --
+ * No single counterpart in the reference: the legacy partition
+ * (pf_partition_sort, B <= 512, padded, int64 keys) generalised for the
+ * batched and large-join tiers to 2^12 buckets, int32 or int64 keys and
+ * int32 row ids, unpadded.  Stable: rows keep their input order in a bucket.
+ * keys: n keys of key_bytes (8 or 4); vals: int32 row ids (NULL: the row
+ * index).  mode 0: digit = the `bits` bits of mix64(key) below its top
+ * `shift` bits, keys_out gets the keys; mode 1: the same digit, keys_out
+ * gets mix64(key) as int64; mode 2: digit = bits [shift, shift + bits) of
+ * the unsigned key.  bucket_offsets: device int64[2^bits + 1], each bucket's
+ * first row and n.  No host wait. */
+size_t pf_radix_partition_workspace_bytes(int64_t n, int32_t bits);
+int pf_radix_partition(const void* keys, int32_t key_bytes, const int32_t* vals, int64_t n,
+                       int32_t bits, int32_t shift, int32_t mode, void* keys_out,
+                       int32_t* vals_out, int64_t* bucket_offsets, void* workspace,
+                       size_t workspace_bytes, void* stream);
+
+/* --------------------------------------------------- partitioned join --
+ * The large-join tier (no single counterpart in the reference; it replaces
+ * the per-call unique-hash core, join_core.cu:301-351, and the sort/search
+ * core, operators/hashjoin.py:380-441 + join_core.cu:263-295, for whole
+ * sides): both sides radix-partitioned by the top `bits` bits of mix64(key)
+ * (bits <= 12), each bucket joined against an LDS hash table of its build
+ * rows.  multi = 0: a unique build (a probe row takes the lowest build row
+ * with its key); multi = 1: every matching build row.  Two calls on one
+ * workspace: prepare partitions, counts and waits (twice), leaving in
+ * host_out (pinned int64[3]) [0] the pair count (-1: a build bucket is too
+ * large for the multi path, [1] its rows; the caller takes another path),
+ * [1] the largest build bucket, [2] the work items; write fills
+ * out_probe/out_build (int64, at least the pair count rows) and gathers
+ * npay (<= 2) int64 build payload columns by build row into pay_out.  Pairs
+ * come in a deterministic order (bucket, then probe row in partition
+ * order, then build row ascending).  Keys are int64; np, nb < 2^31. */
+size_t pf_hash_join_workspace_bytes(int64_t nb, int64_t np, int32_t bits, int32_t multi);
+int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys, int64_t np,
+                         int32_t bits, int32_t multi, int64_t* host_out, void* workspace,
+                         size_t workspace_bytes, void* stream);
+int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int64_t n_items,
+                       const int64_t* const* bpay, int32_t npay, int64_t* out_probe,
+                       int64_t* out_build, int64_t* const* pay_out, void* workspace,
+                       size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

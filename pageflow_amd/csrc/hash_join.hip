@@ -1,0 +1,479 @@
+// Partitioned equi-join with LDS hash tables: the large-join tier (16M-row
+// build, 640M-row probe) and the multi-match join that replaces the
+// reference's sort/search core (operators/hashjoin.py:380-441 +
+// join_core.expand) for non-unique builds.
+//
+// Both sides are radix-partitioned by the top `bits` bits of mix64(key)
+// (pf_radix_partition in its hash-out mode: each row becomes (mix64(key),
+// row id)).  Because mix64 is a bijection on 64 bits, equal hashes are equal
+// keys, so the join compares hashes only and never reads a key again.
+//
+// Work items are (bucket, probe chunk of 64K rows).  A workgroup loads its
+// bucket's build hashes into LDS (a chunk at a time when the bucket is
+// larger than the LDS table), builds an open-addressing table over them
+// (u32 slots, LDS CAS), and streams the probe chunk against it.
+//   unique   each hash's lowest build row answers (a proven-unique build)
+//   multi    the build bucket is first sorted by (hash, row) in LDS
+//            (hj_sort_buckets), so a key's rows are one run; the table maps
+//            a hash to its run's head and the run's length
+// Pairs are counted per item (hj_probe<COUNT>), the counts scanned, then
+// written (hj_probe<WRITE>) at deterministic positions: items in order,
+// probe rows in order inside an item, a probe row's build rows ascending.
+// The payload columns of the build side are gathered in the write pass.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "pf_common.hpp"
+#include "../../include/pageflow_join.h"
+
+namespace pf {
+
+size_t radix_partition_ws(int64_t n, int bits);
+int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
+                    int shift, int mode, void* keys_out, int32_t* vals_out,
+                    int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st);
+
+namespace {
+
+constexpr int HJ_NT = 1024;
+constexpr int HJ_W = HJ_NT / kWave;
+constexpr int HJ_IT = 4;                   // probe rows a thread per round
+constexpr int HJ_ROUND = HJ_NT * HJ_IT;
+constexpr int CAP_U = 6144;                // unique: build rows an LDS chunk
+constexpr int TS_U = 16384;                // unique: table slots
+constexpr int CAP_M = 4096;                // multi: build rows an LDS chunk
+constexpr int TS_M = 8192;
+constexpr int SORT_CAP = 8192;             // multi: largest build bucket sorted in LDS
+constexpr int SORT_NT = 512;
+constexpr int PCH = 65536;                 // probe rows a work item
+constexpr int MAX_PAY = 2;
+
+struct HjParams {
+  const uint64_t* bh;     // build (hash, row) partitioned [nb]
+  const int32_t* brow;
+  const int64_t* boff;    // [P + 1]
+  const uint64_t* ph;     // probe (hash, row) partitioned [np]
+  const int32_t* prow;
+  const int64_t* poff;    // [P + 1]
+  const int64_t* item_off;  // [P + 1] each bucket's first work item
+  int P;
+  int64_t n_items;
+  int64_t* item_cnt;      // COUNT: pairs per item
+  const int64_t* item_pos;  // WRITE: each item's first output row
+  int64_t* out_probe;
+  int64_t* out_build;
+  const int64_t* pay_src[MAX_PAY];
+  int64_t* pay_out[MAX_PAY];
+  int npay;
+};
+
+// ------------------------------------------------------------- items --
+
+// One block: each bucket's work items (its probe chunks; none when either
+// side is empty), their prefix, the total and the largest build bucket
+// into info[0..1].
+__global__ __launch_bounds__(1024) void hj_items(const int64_t* __restrict__ boff,
+                                                 const int64_t* __restrict__ poff, int P,
+                                                 int cap, int64_t* __restrict__ item_off,
+                                                 int64_t* __restrict__ info) {
+  __shared__ int64_t tmp[1024 / kWave + 1];
+  __shared__ int64_t mx[1024 / kWave];
+  constexpr int PER = 4;  // P <= 4096
+  int64_t v[PER], s = 0, m = 0;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = threadIdx.x * PER + q;
+    v[q] = 0;
+    if (b < P) {
+      const int64_t nb = boff[b + 1] - boff[b], np = poff[b + 1] - poff[b];
+      if (nb > 0 && np > 0) v[q] = (np + PCH - 1) / PCH;
+      m = nb > m ? nb : m;
+    }
+    s += v[q];
+  }
+  int64_t tot;
+  int64_t ex = block_excl_scan<1024>(s, tmp, &tot);
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = threadIdx.x * PER + q;
+    if (b < P) item_off[b] = ex;
+    ex += v[q];
+  }
+  // largest build bucket
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const int64_t y = __shfl_xor(m, o, kWave);
+    m = y > m ? y : m;
+  }
+  if (lane_id() == 0) mx[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t mm = 0;
+    for (int w = 0; w < 1024 / kWave; ++w) mm = mx[w] > mm ? mx[w] : mm;
+    item_off[P] = tot;
+    info[0] = tot;
+    info[1] = mm;
+  }
+}
+
+// ---------------------------------------------------------- LDS sort --
+
+// Block b sorts build bucket b's (hash, row) pairs by hash, then row
+// (bitonic network in LDS; the bucket holds at most SORT_CAP rows).
+__global__ __launch_bounds__(SORT_NT) void hj_sort_buckets(uint64_t* __restrict__ bh,
+                                                           int32_t* __restrict__ brow,
+                                                           const int64_t* __restrict__ boff) {
+  __shared__ uint64_t sh[SORT_CAP];
+  __shared__ int32_t sr[SORT_CAP];
+  const int64_t b0 = boff[blockIdx.x], n = boff[blockIdx.x + 1] - b0;
+  if (n <= 1) return;
+  int m = 2;
+  while (m < n) m <<= 1;
+  for (int i = threadIdx.x; i < m; i += SORT_NT) {
+    if (i < n) {
+      sh[i] = bh[b0 + i];
+      sr[i] = brow[b0 + i];
+    } else {
+      sh[i] = ~0ULL;
+      sr[i] = INT32_MAX;
+    }
+  }
+  __syncthreads();
+  for (int k = 2; k <= m; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < (m >> 1); i += SORT_NT) {
+        const int lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+        const int hi = lo + j;
+        const bool up = (lo & k) == 0;
+        const uint64_t a = sh[lo], c = sh[hi];
+        const int32_t ra = sr[lo], rc = sr[hi];
+        const bool gt = a > c || (a == c && ra > rc);
+        if (gt == up) {
+          sh[lo] = c;
+          sh[hi] = a;
+          sr[lo] = rc;
+          sr[hi] = ra;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < n; i += SORT_NT) {
+    bh[b0 + i] = sh[i];
+    brow[b0 + i] = sr[i];
+  }
+}
+
+// ------------------------------------------------------ probe / write --
+
+struct Item {
+  int64_t bs, be, ps, pe;  // the bucket's build rows, the item's probe rows
+};
+
+__device__ __forceinline__ Item item_of(const HjParams& p, int64_t item) {
+  // bucket: last b with item_off[b] <= item
+  int lo = 0, hi = p.P;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (p.item_off[mid] <= item)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  // skip buckets without items (equal offsets): the last such index holds it
+  const int b = lo;
+  const int64_t c = item - p.item_off[b];
+  Item it;
+  it.bs = p.boff[b];
+  it.be = p.boff[b + 1];
+  it.ps = p.poff[b] + c * PCH;
+  it.pe = it.ps + PCH < p.poff[b + 1] ? it.ps + PCH : p.poff[b + 1];
+  return it;
+}
+
+template <bool MULTI>
+struct Lds;
+template <>
+struct Lds<false> {
+  static constexpr int CAP = CAP_U, TS = TS_U;
+};
+template <>
+struct Lds<true> {
+  static constexpr int CAP = CAP_M, TS = TS_M;
+};
+
+// MODE 0: count pairs per item; MODE 1: write them.
+template <bool MULTI, int MODE>
+__global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
+  constexpr int CAP = Lds<MULTI>::CAP, TS = Lds<MULTI>::TS;
+  __shared__ uint64_t keys[CAP];
+  __shared__ uint32_t slot[TS];
+  __shared__ uint16_t glen[MULTI ? CAP : 1];
+  __shared__ int64_t tmp[HJ_W + 1];
+  const int tid = threadIdx.x;
+  const int64_t item = blockIdx.x;
+  if (item >= p.n_items) return;
+  const Item it = item_of(p, item);
+  int64_t pos = 0;
+  if (MODE == 1) pos = p.item_pos[item];
+  int64_t count = 0;
+  // a bucket larger than an LDS chunk is joined one build chunk at a time
+  for (int64_t cb = it.bs; cb < it.be; cb += CAP) {
+  const int nloc = (int)(it.be - cb < CAP ? it.be - cb : CAP);
+  __syncthreads();  // the previous chunk's lookups are done
+  for (int s = tid; s < TS; s += HJ_NT) slot[s] = 0;
+  for (int i = tid; i < nloc; i += HJ_NT) keys[i] = p.bh[cb + i];
+  __syncthreads();
+  for (int i = tid; i < nloc; i += HJ_NT) {
+    const uint64_t h = keys[i];
+    if (MULTI && i > 0 && keys[i - 1] == h) continue;  // not a run head
+    if (MULTI) {
+      int len = 1;
+      while (i + len < nloc && keys[i + len] == h) ++len;
+      glen[i] = (uint16_t)(len > 65535 ? 65535 : len);
+    }
+    uint32_t s = (uint32_t)h & (TS - 1);
+    for (;;) {
+      const uint32_t cur = atomicCAS(&slot[s], 0u, (uint32_t)(i + 1));
+      if (cur == 0) break;
+      if (keys[cur - 1] == h) {  // a repeated hash (unique mode): lowest row
+        atomicMin(&slot[s], (uint32_t)(i + 1));
+        break;
+      }
+      s = (s + 1) & (TS - 1);
+    }
+  }
+  __syncthreads();
+
+  for (int64_t r0 = it.ps; r0 < it.pe; r0 += HJ_ROUND) {
+    int hidx[HJ_IT];
+    int m[HJ_IT];
+    int msum = 0;
+#pragma unroll
+    for (int k = 0; k < HJ_IT; ++k) {
+      const int64_t r = r0 + (int64_t)tid * HJ_IT + k;
+      hidx[k] = -1;
+      m[k] = 0;
+      if (r < it.pe) {
+        const uint64_t h = p.ph[r];
+        uint32_t s = (uint32_t)h & (TS - 1);
+        for (;;) {
+          const uint32_t v = slot[s];
+          if (v == 0) break;
+          if (keys[v - 1] == h) {
+            hidx[k] = (int)v - 1;
+            break;
+          }
+          s = (s + 1) & (TS - 1);
+        }
+        if (hidx[k] >= 0) {
+          if (MULTI) {
+            // a run longer than glen's range is counted to its end
+            int len = glen[hidx[k]];
+            if (len == 65535)
+              while (hidx[k] + len < nloc && keys[hidx[k] + len] == h) ++len;
+            m[k] = len;
+          } else {
+            m[k] = 1;
+          }
+        }
+      }
+      msum += m[k];
+    }
+    if (MODE == 0) {
+      count += msum;
+      continue;
+    }
+    int64_t tot;
+    int64_t o = pos + block_excl_scan<HJ_NT>((int64_t)msum, tmp, &tot);
+    pos += tot;
+#pragma unroll
+    for (int k = 0; k < HJ_IT; ++k) {
+      if (m[k] == 0) continue;
+      const int64_t r = r0 + (int64_t)tid * HJ_IT + k;
+      const int64_t prow = p.prow[r];
+      for (int q = 0; q < m[k]; ++q) {
+        const int32_t brow = p.brow[cb + hidx[k] + q];
+        p.out_probe[o] = prow;
+        p.out_build[o] = brow;
+        for (int c = 0; c < p.npay; ++c) p.pay_out[c][o] = p.pay_src[c][brow];
+        ++o;
+      }
+    }
+  }
+  }  // build chunks
+  if (MODE == 0) {
+    const int64_t t = wave_sum(count);
+    if (lane_id() == 0) tmp[tid / kWave] = t;
+    __syncthreads();
+    if (tid == 0) {
+      int64_t s = 0;
+      for (int w = 0; w < HJ_W; ++w) s += tmp[w];
+      p.item_cnt[item] = s;
+    }
+  }
+}
+
+__global__ void hj_readback(const int64_t* __restrict__ a, const int64_t* __restrict__ b,
+                            int64_t* host) {
+  if (threadIdx.x == 0) store_host(&host[0], a[0]);
+  if (threadIdx.x == 1) store_host(&host[1], a[1]);
+  if (threadIdx.x == 2) store_host(&host[2], b ? b[0] : 0);
+}
+
+struct HjWs {
+  uint64_t* bh;
+  int32_t* brow;
+  int64_t* boff;
+  uint64_t* ph;
+  int32_t* prow;
+  int64_t* poff;
+  int64_t* item_off;
+  int64_t* info;       // [0] items, [1] largest build bucket
+  int64_t* total;
+  int64_t* item_cnt;   // [max items] (counts, then scanned in place)
+  void* part_ws;
+  void* scan;
+  size_t bytes;
+};
+
+// Upper bound of work items: every bucket's probe chunks.
+int64_t max_items(int64_t np, int bits) { return np / PCH + (1LL << bits) + 1; }
+
+HjWs carve_hj(void* ws, int64_t nb, int64_t np, int bits, bool multi) {
+  Carve cv(ws);
+  HjWs w;
+  const int64_t P = 1LL << bits;
+  w.bh = cv.take<uint64_t>(nb > 0 ? nb : 1);
+  w.brow = cv.take<int32_t>(nb > 0 ? nb : 1);
+  w.boff = cv.take<int64_t>(P + 1);
+  w.ph = cv.take<uint64_t>(np > 0 ? np : 1);
+  w.prow = cv.take<int32_t>(np > 0 ? np : 1);
+  w.poff = cv.take<int64_t>(P + 1);
+  w.item_off = cv.take<int64_t>(P + 1);
+  w.info = cv.take<int64_t>(4);
+  w.total = cv.take<int64_t>(2);
+  const int64_t mi = max_items(np, bits);
+  w.item_cnt = cv.take<int64_t>(mi);
+  const size_t pws = radix_partition_ws(nb > np ? nb : np, bits);
+  w.part_ws = cv.take<char>(pws);
+  w.scan = cv.take<char>(scan_i64_scratch(mi));
+  w.bytes = cv.bytes();
+  return w;
+}
+
+HjParams params_of(const HjWs& w, int bits, int64_t n_items) {
+  HjParams p{};
+  p.bh = w.bh;
+  p.brow = w.brow;
+  p.boff = w.boff;
+  p.ph = w.ph;
+  p.prow = w.prow;
+  p.poff = w.poff;
+  p.item_off = w.item_off;
+  p.P = 1 << bits;
+  p.n_items = n_items;
+  p.item_cnt = w.item_cnt;
+  p.item_pos = w.item_cnt;
+  return p;
+}
+
+}  // namespace
+}  // namespace pf
+
+extern "C" {
+
+size_t pf_hash_join_workspace_bytes(int64_t nb, int64_t np, int32_t bits, int32_t multi) {
+  if (bits < 0) bits = 0;
+  if (bits > 12) bits = 12;
+  return pf::carve_hj(nullptr, nb, np, bits, multi != 0).bytes;
+}
+
+int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys, int64_t np,
+                         int32_t bits, int32_t multi, int64_t* host_out, void* workspace,
+                         size_t workspace_bytes, void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (bits < 0 || bits > 12) return fail("hash_join: bits must be in [0, 12]");
+  if (nb < 0 || np < 0 || nb >= (1LL << 31) || np >= (1LL << 31))
+    return fail("hash_join: a side takes fewer than 2^31 rows");
+  HjWs w = carve_hj(workspace, nb, np, bits, multi != 0);
+  if (workspace_bytes < w.bytes) return fail("hash_join: workspace too small");
+  void* hdev = nullptr;
+  if (host_alias(host_out, &hdev)) return 1;
+  const int P = 1 << bits;
+  const int cap = multi ? CAP_M : CAP_U;
+  const size_t pws = radix_partition_ws(nb > np ? nb : np, bits);
+  // both sides as (mix64(key), row) by the top `bits` bits of the hash
+  if (radix_partition(bkeys, 8, nullptr, nb, bits, 0, 1, w.bh, w.brow, w.boff, w.part_ws, pws,
+                      st))
+    return 1;
+  if (radix_partition(pkeys, 8, nullptr, np, bits, 0, 1, w.ph, w.prow, w.poff, w.part_ws, pws,
+                      st))
+    return 1;
+  hj_items<<<1, 1024, 0, st>>>(w.boff, w.poff, P, cap, w.item_off, w.info);
+  hj_readback<<<1, 64, 0, st>>>(w.info, nullptr, (int64_t*)hdev);
+  if (launch_check("hash_join items")) return 1;
+  if (wait_stream(st, "hash_join")) return 1;
+  const int64_t n_items = host_out[0];
+  const int64_t largest = host_out[1];
+  if (multi && largest > SORT_CAP) {
+    // a build bucket too large to sort in LDS: the caller takes another path
+    host_out[0] = -1;
+    host_out[1] = largest;
+    host_out[2] = 0;
+    return ok();
+  }
+  if (n_items > max_items(np, bits)) return fail("hash_join: work items overflow");
+  if (multi && nb > 0) {
+    hj_sort_buckets<<<(unsigned)P, SORT_NT, 0, st>>>(w.bh, w.brow, w.boff);
+    if (launch_check("hash_join sort")) return 1;
+  }
+  HjParams p = params_of(w, bits, n_items);
+  if (n_items > 0) {
+    if (multi)
+      hj_probe<true, 0><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
+    else
+      hj_probe<false, 0><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
+    if (launch_check("hash_join count")) return 1;
+  }
+  if (scan_i64_excl(w.item_cnt, n_items, w.total, w.scan, st)) return 1;
+  hj_readback<<<1, 64, 0, st>>>(w.total, w.info, (int64_t*)hdev);
+  if (launch_check("hash_join count")) return 1;
+  if (wait_stream(st, "hash_join")) return 1;
+  // host_out: [0] pairs, [1] (unused), [2] work items
+  const int64_t pairs = host_out[0];
+  host_out[1] = largest;
+  host_out[2] = n_items;
+  (void)pairs;
+  return ok();
+}
+
+int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int64_t n_items,
+                       const int64_t* const* bpay, int32_t npay, int64_t* out_probe,
+                       int64_t* out_build, int64_t* const* pay_out, void* workspace,
+                       size_t workspace_bytes, void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (npay < 0 || npay > MAX_PAY) return fail("hash_join: at most 2 payload columns");
+  HjWs w = carve_hj(workspace, nb, np, bits, multi != 0);
+  if (workspace_bytes < w.bytes) return fail("hash_join: workspace too small");
+  if (n_items <= 0) return ok();
+  HjParams p = params_of(w, bits, n_items);
+  p.out_probe = out_probe;
+  p.out_build = out_build;
+  p.npay = npay;
+  for (int c = 0; c < npay; ++c) {
+    p.pay_src[c] = bpay[c];
+    p.pay_out[c] = pay_out[c];
+  }
+  if (multi)
+    hj_probe<true, 1><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
+  else
+    hj_probe<false, 1><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
+  if (launch_check("hash_join write")) return 1;
+  return ok();
+}
+
+}  // extern "C"

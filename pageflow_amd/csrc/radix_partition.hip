@@ -1,0 +1,347 @@
+// Stable radix partition of (key, row id) pairs into up to 2^12 buckets: the
+// partition of the batched and large-join tiers (one pass, no padding), and
+// the per-digit pass of an LSD sort.
+//
+// A bucket is the top `bits` bits of mix64(key) below its top `shift` bits
+// (hashed modes; the join's partition function generalised past the legacy
+// 512 buckets), or bits [shift, shift + bits) of the unsigned key (raw mode).
+// Rows keep their input order inside a bucket.
+//
+// gfx950 design.  Each workgroup owns a chunk of rows (at least 64 rows a
+// bucket on average) and walks it in tiles: per tile, wave64 ballots rank
+// rows inside their bucket (one ballot per digit bit plus validity), per-wave
+// counts in LDS give each wave its start, the tile is staged in LDS in bucket
+// order and each bucket's run is written at the chunk's running cursor for
+// that bucket.  A chunk's cursors come from a device-wide scan of the chunk
+// histograms laid out bucket-major, so the whole output is one stable
+// partition and a bucket's runs from consecutive tiles of a chunk are
+// contiguous in HBM.
+//
+//   rp_hist     per-chunk bucket counts (LDS atomics), hist[b][chunk]
+//   scan        exclusive scan of hist in bucket-major order -> cursors
+//   rp_scatter  per tile: rank, stage, write runs; advance the cursors
+//   rp_offsets  each bucket's first row (and n) into bucket_offsets
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "pf_common.hpp"
+#include "../../include/pageflow_join.h"
+
+namespace pf {
+namespace {
+
+enum RpMode : int { RP_HASH_KEY = 0, RP_HASH_OUT = 1, RP_RAW = 2 };
+
+template <typename K>
+struct Unsigned;
+template <>
+struct Unsigned<int64_t> { typedef uint64_t T; };
+template <>
+struct Unsigned<int32_t> { typedef uint32_t T; };
+
+// digit of a key and the value written in its place (the key, or its hash)
+template <typename K, typename KO, int MODE>
+__device__ __forceinline__ uint32_t rp_digit(K key, int bits, int shift, KO& outv) {
+  if (MODE == RP_RAW) {
+    const uint64_t u = (uint64_t)(typename Unsigned<K>::T)key;
+    outv = (KO)key;
+    return (uint32_t)((u >> shift) & ((1ULL << bits) - 1));
+  }
+  const uint64_t h = mix64((uint64_t)(int64_t)key);
+  outv = MODE == RP_HASH_OUT ? (KO)h : (KO)key;
+  return bits ? (uint32_t)((h << shift) >> (64 - bits)) : 0u;
+}
+
+// Rows of a chunk: at least 64 a bucket on average, a multiple of the tile.
+int64_t chunk_rows(int bits, int tile) {
+  int64_t c = (int64_t)64 << bits;
+  if (c < 32768) c = 32768;
+  return ((c + tile - 1) / tile) * tile;
+}
+
+template <int LOGB>
+struct RpShape;
+template <>
+struct RpShape<8> {
+  static constexpr int NT = 256, IT = 8;
+};
+template <>
+struct RpShape<12> {
+  static constexpr int NT = 512, IT = 8;
+};
+
+template <typename K, typename KO, int MODE, int LOGB>
+__global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_hist(const K* __restrict__ keys,
+                                                            int64_t n, int64_t crows, int bits,
+                                                            int shift, int64_t nchunks,
+                                                            int32_t* __restrict__ hist) {
+  constexpr int NT = RpShape<LOGB>::NT;
+  __shared__ int32_t h[1 << LOGB];
+  const int B = 1 << bits;
+  for (int b = threadIdx.x; b < B; b += NT) h[b] = 0;
+  __syncthreads();
+  const int64_t c0 = (int64_t)blockIdx.x * crows;
+  const int64_t c1 = c0 + crows < n ? c0 + crows : n;
+  for (int64_t i = c0 + threadIdx.x; i < c1; i += NT) {
+    KO o;
+    atomicAdd(&h[rp_digit<K, KO, MODE>(keys[i], bits, shift, o)], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < B; b += NT) hist[(int64_t)b * nchunks + blockIdx.x] = h[b];
+}
+
+// Lanes of the wave holding the same digit (d == ~0u: not a row).
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits) {
+  const bool valid = d != ~0u;
+  const uint64_t v = ballot(valid);
+  uint64_t m = valid ? v : ~v;
+  for (int bit = 0; bit < bits; ++bit) {
+    const bool on = (d >> bit) & 1;
+    const uint64_t bal = ballot(on);
+    m &= on ? bal : ~bal;
+  }
+  return m;
+}
+
+template <typename K, typename KO, int MODE, int LOGB>
+__global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_scatter(
+    const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t n, int64_t crows,
+    int bits, int shift, int64_t nchunks, const int64_t* __restrict__ offs,
+    KO* __restrict__ kout, int32_t* __restrict__ vout) {
+  constexpr int NT = RpShape<LOGB>::NT, IT = RpShape<LOGB>::IT;
+  constexpr int W = NT / kWave, TILE = NT * IT, BMAX = 1 << LOGB;
+  constexpr int WROWS = kWave * IT;
+  __shared__ uint16_t wh[W * BMAX];
+  __shared__ uint16_t tstart[BMAX];
+  __shared__ uint32_t cursor[BMAX];
+  __shared__ KO stage_k[TILE];
+  __shared__ int32_t stage_v[TILE];
+  __shared__ uint16_t sb[TILE];
+  __shared__ int32_t tmp[W + 1];
+
+  const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
+  const int B = 1 << bits;
+  const int64_t c0 = (int64_t)blockIdx.x * crows;
+  const int64_t c1 = c0 + crows < n ? c0 + crows : n;
+  for (int b = tid; b < B; b += NT) {
+    cursor[b] = (uint32_t)offs[(int64_t)b * nchunks + blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < W; ++q) wh[q * B + b] = 0;
+  }
+  __syncthreads();
+  for (int64_t t0 = c0; t0 < c1; t0 += TILE) {
+    uint32_t d[IT];
+    KO kv[IT];
+    int32_t vv[IT];
+    const int64_t seg = t0 + (int64_t)w * WROWS;
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const int64_t i = seg + k * kWave + lane;
+      if (i < c1) {
+        d[k] = rp_digit<K, KO, MODE>(keys[i], bits, shift, kv[k]);
+        vv[k] = vals ? vals[i] : (int32_t)i;
+      } else {
+        d[k] = ~0u;
+      }
+    }
+    // stable rank inside (wave, bucket)
+    int rk[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const uint64_t peers = match_digit(d[k], bits);
+      const int below = mbcnt(peers);
+      const bool valid = d[k] != ~0u;
+      const int base = valid ? (int)wh[w * B + d[k]] : 0;
+      rk[k] = base + below;
+      if (valid && below == 0) wh[w * B + d[k]] = (uint16_t)(base + __popcll(peers));
+    }
+    __syncthreads();
+    // per bucket: each wave's start in the tile's run; the run's length
+    for (int b = tid; b < B; b += NT) {
+      int run = 0;
+#pragma unroll
+      for (int q = 0; q < W; ++q) {
+        const int t = wh[q * B + b];
+        wh[q * B + b] = (uint16_t)run;
+        run += t;
+      }
+      tstart[b] = (uint16_t)run;
+    }
+    __syncthreads();
+    // the runs' starts in bucket order: B / NT consecutive buckets a thread
+    {
+      constexpr int PER = BMAX / NT > 0 ? BMAX / NT : 1;
+      const int per = B >= NT ? B / NT : 1;
+      const int b0 = tid * per;
+      int loc[PER];
+      int s = 0;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        loc[q] = (q < per && b0 + q < B) ? tstart[b0 + q] : 0;
+        s += loc[q];
+      }
+      int tot;
+      int ex = block_excl_scan<NT>(s, tmp, &tot);
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (q < per && b0 + q < B) tstart[b0 + q] = (uint16_t)ex;
+        ex += loc[q];
+      }
+    }
+    __syncthreads();
+    int tvalid = 0;
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      if (d[k] != ~0u) {
+        const int pos = tstart[d[k]] + wh[w * B + d[k]] + rk[k];
+        stage_k[pos] = kv[k];
+        stage_v[pos] = vv[k];
+        sb[pos] = (uint16_t)d[k];
+      }
+    }
+    {
+      const int64_t left = c1 - t0;
+      tvalid = left < TILE ? (int)left : TILE;
+    }
+    __syncthreads();
+    for (int s = tid; s < tvalid; s += NT) {
+      const int b = sb[s];
+      const uint32_t o = cursor[b] + (uint32_t)(s - tstart[b]);
+      kout[o] = stage_k[s];
+      vout[o] = stage_v[s];
+    }
+    __syncthreads();
+    for (int b = tid; b < B; b += NT) {
+      const int next = b + 1 < B ? tstart[b + 1] : tvalid;
+      cursor[b] += (uint32_t)(next - tstart[b]);
+#pragma unroll
+      for (int q = 0; q < W; ++q) wh[q * B + b] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void rp_offsets(const int64_t* __restrict__ offs, int64_t nchunks, int B, int64_t n,
+                           int64_t* __restrict__ bucket_offsets) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b <= B; b += gridDim.x * blockDim.x)
+    bucket_offsets[b] = b < B ? offs[(int64_t)b * nchunks] : n;
+}
+
+struct RpWs {
+  int32_t* hist;
+  int64_t* offs;
+  void* scan;
+  size_t bytes;
+};
+
+int tile_of(int bits) {
+  return bits <= 8 ? RpShape<8>::NT * RpShape<8>::IT : RpShape<12>::NT * RpShape<12>::IT;
+}
+
+RpWs carve_rp(void* ws, int64_t n, int bits) {
+  Carve cv(ws);
+  RpWs r;
+  const int64_t crows = chunk_rows(bits, tile_of(bits));
+  const int64_t nchunks = n > 0 ? (n + crows - 1) / crows : 1;
+  const int64_t entries = nchunks << bits;
+  r.hist = cv.take<int32_t>(entries);
+  r.offs = cv.take<int64_t>(entries);
+  r.scan = cv.take<char>(scan_i32_scratch(entries));
+  r.bytes = cv.bytes();
+  return r;
+}
+
+template <typename K, typename KO, int MODE, int LOGB>
+int rp_launch(const K* keys, const int32_t* vals, int64_t n, int bits, int shift, KO* kout,
+              int32_t* vout, int64_t* bucket_offsets, const RpWs& r, hipStream_t st) {
+  const int tile = RpShape<LOGB>::NT * RpShape<LOGB>::IT;
+  const int64_t crows = chunk_rows(bits, tile_of(bits));
+  const int64_t nchunks = n > 0 ? (n + crows - 1) / crows : 1;
+  (void)tile;
+  const int64_t entries = nchunks << bits;
+  rp_hist<K, KO, MODE, LOGB><<<(unsigned)nchunks, RpShape<LOGB>::NT, 0, st>>>(
+      keys, n, crows, bits, shift, nchunks, r.hist);
+  if (launch_check("radix_partition hist")) return 1;
+  if (scan_i32_to_i64_excl(r.hist, r.offs, entries, nullptr, r.scan, st)) return 1;
+  if (n > 0) {
+    rp_scatter<K, KO, MODE, LOGB><<<(unsigned)nchunks, RpShape<LOGB>::NT, 0, st>>>(
+        keys, vals, n, crows, bits, shift, nchunks, r.offs, kout, vout);
+    if (launch_check("radix_partition scatter")) return 1;
+  }
+  rp_offsets<<<grid_for((1 << bits) + 1, 256), 256, 0, st>>>(r.offs, nchunks, 1 << bits, n,
+                                                             bucket_offsets);
+  return launch_check("radix_partition offsets");
+}
+
+template <typename K>
+int rp_dispatch(const K* keys, const int32_t* vals, int64_t n, int bits, int shift, int mode,
+                void* kout, int32_t* vout, int64_t* bucket_offsets, const RpWs& r,
+                hipStream_t st) {
+  if (mode == RP_HASH_OUT) {
+    if (bits <= 8)
+      return rp_launch<K, int64_t, RP_HASH_OUT, 8>(keys, vals, n, bits, shift, (int64_t*)kout,
+                                                   vout, bucket_offsets, r, st);
+    return rp_launch<K, int64_t, RP_HASH_OUT, 12>(keys, vals, n, bits, shift, (int64_t*)kout,
+                                                  vout, bucket_offsets, r, st);
+  }
+  if (mode == RP_HASH_KEY) {
+    if (bits <= 8)
+      return rp_launch<K, K, RP_HASH_KEY, 8>(keys, vals, n, bits, shift, (K*)kout, vout,
+                                             bucket_offsets, r, st);
+    return rp_launch<K, K, RP_HASH_KEY, 12>(keys, vals, n, bits, shift, (K*)kout, vout,
+                                            bucket_offsets, r, st);
+  }
+  if (bits <= 8)
+    return rp_launch<K, K, RP_RAW, 8>(keys, vals, n, bits, shift, (K*)kout, vout,
+                                      bucket_offsets, r, st);
+  return rp_launch<K, K, RP_RAW, 12>(keys, vals, n, bits, shift, (K*)kout, vout,
+                                     bucket_offsets, r, st);
+}
+
+}  // namespace
+
+// Host-side use by the other tiers of the library.
+size_t radix_partition_ws(int64_t n, int bits) { return carve_rp(nullptr, n, bits).bytes; }
+
+int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
+                    int shift, int mode, void* keys_out, int32_t* vals_out,
+                    int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st) {
+  if (key_bytes != 8 && key_bytes != 4) return fail("radix_partition: keys are int64 or int32");
+  if (bits < 0 || bits > 12) return fail("radix_partition: bits must be in [0, 12]");
+  if (n < 0 || n >= (1LL << 32)) return fail("radix_partition: n out of range");
+  if (mode < RP_HASH_KEY || mode > RP_RAW) return fail("radix_partition: unknown mode");
+  const int kbits = mode == RP_RAW ? key_bytes * 8 : 64;
+  if (shift < 0 || shift + bits > kbits) return fail("radix_partition: shift + bits too large");
+  if (mode == RP_HASH_OUT && key_bytes != 8 && keys_out == keys)
+    return fail("radix_partition: a hashed output cannot overwrite int32 keys");
+  RpWs r = carve_rp(ws, n, bits);
+  if (ws_bytes < r.bytes) return fail("radix_partition: workspace too small");
+  if (key_bytes == 8)
+    return rp_dispatch<int64_t>((const int64_t*)keys, vals, n, bits, shift, mode, keys_out,
+                                vals_out, bucket_offsets, r, st);
+  return rp_dispatch<int32_t>((const int32_t*)keys, vals, n, bits, shift, mode, keys_out,
+                              vals_out, bucket_offsets, r, st);
+}
+
+}  // namespace pf
+
+extern "C" {
+
+size_t pf_radix_partition_workspace_bytes(int64_t n, int32_t bits) {
+  if (bits < 0) bits = 0;
+  if (bits > 12) bits = 12;
+  return pf::radix_partition_ws(n, bits);
+}
+
+int pf_radix_partition(const void* keys, int32_t key_bytes, const int32_t* vals, int64_t n,
+                       int32_t bits, int32_t shift, int32_t mode, void* keys_out,
+                       int32_t* vals_out, int64_t* bucket_offsets, void* workspace,
+                       size_t workspace_bytes, void* stream) {
+  if (pf::radix_partition(keys, key_bytes, vals, n, bits, shift, mode, keys_out, vals_out,
+                          bucket_offsets, workspace, workspace_bytes, (hipStream_t)stream))
+    return 1;
+  return pf::ok();
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+"""Whole-side equi-joins on the partitioned LDS-table kernels
+(``pf_hash_join_*``, include/pageflow_join.h): the large-join tier and the
+multi-match join that stands in for the reference's sort/search core
+(operators/hashjoin.py:380-441) on non-unique builds.
+
+``HashJoin`` keeps its workspace between calls (a serving loop or a
+benchmark reuses it); ``hash_join`` is the one-shot form.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence, Tuple
+
+import torch
+
+from . import _lib
+
+CAP_UNIQUE = 6144   # build rows an LDS table holds (unique builds)
+CAP_MULTI = 4096    # multi: build rows an LDS chunk; buckets sorted in LDS up to 8192 rows
+MAX_BITS = 12
+
+
+def default_bits(n_build: int, unique: bool) -> int:
+    """Bucket bits so the mean build bucket sits well inside one LDS table
+    (about 2/3 of it for a unique build, 60% of a multi chunk)."""
+    target = CAP_UNIQUE * 2 // 3 if unique else CAP_MULTI * 3 // 5
+    bits = 0
+    while bits < MAX_BITS and (n_build >> bits) > target:
+        bits += 1
+    return bits
+
+
+def _check_keys(t: torch.Tensor, what: str) -> None:
+    _lib.require_cuda(t, what)
+    if t.dtype != torch.int64 or t.dim() != 1 or not t.is_contiguous():
+        raise ValueError(f"{what} must be a contiguous 1-D int64 tensor")
+    if t.numel() >= (1 << 31):
+        raise ValueError(f"{what}: a side takes fewer than 2^31 rows")
+
+
+class HashJoin:
+    """One join's plan: sizes, bucket bits and the workspace."""
+
+    def __init__(self, n_build: int, n_probe: int, unique: bool = True,
+                 bits: Optional[int] = None, device=None):
+        self.nb, self.np = int(n_build), int(n_probe)
+        self.unique = bool(unique)
+        self.bits = default_bits(self.nb, self.unique) if bits is None else int(bits)
+        if not 0 <= self.bits <= MAX_BITS:
+            raise ValueError(f"bits must be in [0, {MAX_BITS}]")
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+        lib = _lib.load()
+        self.ws_bytes = int(lib.pf_hash_join_workspace_bytes(self.nb, self.np, self.bits,
+                                                             int(not self.unique)))
+        self.ws = _lib.workspace(self.ws_bytes, self.device)
+        self.host = torch.empty(3, dtype=torch.int64, pin_memory=True)
+        self.n_items = 0
+        self.pairs = 0
+
+    def prepare(self, build_keys: torch.Tensor, probe_keys: torch.Tensor) -> int:
+        """Partitions both sides and counts the pairs (two host waits).
+        Returns the pair count, or -1 when a build bucket is too large for
+        the multi path."""
+        _check_keys(build_keys, "build keys")
+        _check_keys(probe_keys, "probe keys")
+        if build_keys.numel() != self.nb or probe_keys.numel() != self.np:
+            raise ValueError("HashJoin: sides differ from the plan's sizes")
+        lib = _lib.load()
+        with torch.cuda.device(self.device):
+            _lib.check(lib.pf_hash_join_prepare(
+                _lib.ptr(build_keys), self.nb, _lib.ptr(probe_keys), self.np, self.bits,
+                int(not self.unique), ctypes.c_void_p(self.host.data_ptr()), _lib.ptr(self.ws),
+                self.ws_bytes, _lib.stream_of(self.device)))
+        self.pairs, _, self.n_items = (int(v) for v in self.host.tolist())
+        return self.pairs
+
+    def write(self, out_probe: torch.Tensor, out_build: torch.Tensor,
+              payloads: Sequence[torch.Tensor] = (),
+              pay_out: Sequence[torch.Tensor] = ()) -> None:
+        """Writes the prepared pairs (and gathered build payloads); no wait."""
+        if self.pairs < 0:
+            raise RuntimeError("HashJoin: the prepared join took no LDS path")
+        if out_probe.numel() < self.pairs or out_build.numel() < self.pairs:
+            raise ValueError("HashJoin: outputs smaller than the pair count")
+        if len(payloads) != len(pay_out) or len(payloads) > 2:
+            raise ValueError("HashJoin: at most two payload columns, one output each")
+        for t in payloads:
+            if t.dtype != torch.int64 or t.numel() != self.nb or not t.is_contiguous():
+                raise ValueError("HashJoin: payloads are contiguous int64 build columns")
+        for t in pay_out:
+            if t.dtype != torch.int64 or t.numel() < self.pairs:
+                raise ValueError("HashJoin: payload outputs are int64 of the pair count")
+        lib = _lib.load()
+        with torch.cuda.device(self.device):
+            _lib.check(lib.pf_hash_join_write(
+                self.nb, self.np, self.bits, int(not self.unique), self.n_items,
+                _lib.void_array([t.data_ptr() for t in payloads]), len(payloads),
+                _lib.ptr(out_probe), _lib.ptr(out_build),
+                _lib.void_array([t.data_ptr() for t in pay_out]), _lib.ptr(self.ws),
+                self.ws_bytes, _lib.stream_of(self.device)))
+
+
+def hash_join(build_keys: torch.Tensor, probe_keys: torch.Tensor,
+              payloads: Sequence[torch.Tensor] = (), unique: bool = True,
+              bits: Optional[int] = None
+              ) -> Optional[Tuple[torch.Tensor, torch.Tensor, List[torch.Tensor]]]:
+    """(probe rows, build rows, gathered build payloads) of an inner
+    equi-join of int64 keys, in the deterministic partition order; None when
+    the multi path cannot take a build bucket (too many rows of one
+    bucket)."""
+    plan = HashJoin(build_keys.numel(), probe_keys.numel(), unique, bits, build_keys.device)
+    n = plan.prepare(build_keys, probe_keys)
+    if n < 0:
+        return None
+    dev = build_keys.device
+    out_p = torch.empty(n, dtype=torch.int64, device=dev)
+    out_b = torch.empty(n, dtype=torch.int64, device=dev)
+    outs = [torch.empty(n, dtype=torch.int64, device=dev) for _ in payloads]
+    plan.write(out_p, out_b, list(payloads), outs)
+    return out_p, out_b, outs

@@ -1,0 +1,122 @@
+"""The partitioned LDS-table join (pf_hash_join_*) against the oracle:
+sorted (probe row, build row) pairs bit-exact, payloads gathered by build
+row, for unique and multi-match builds, empty and tiny sides, extreme keys,
+oversized build buckets (chunked LDS tables), skewed probes, determinism."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from pageflow_amd.join import HashJoin, hash_join
+
+pytestmark = pytest.mark.gpu
+
+I64 = np.iinfo(np.int64)
+
+
+def _sorted_pairs(probe_idx, build_idx):
+    p = probe_idx.cpu().numpy()
+    b = build_idx.cpu().numpy()
+    pairs = np.stack([b, p], 1) if p.size else np.empty((0, 2), np.int64)
+    if pairs.shape[0]:
+        pairs = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
+    return pairs
+
+
+def _check(bk, pk, unique, bits=None, pay=True):
+    g = torch.Generator().manual_seed(len(bk))
+    payload = torch.randint(-(1 << 60), 1 << 60, (len(bk),), generator=g, dtype=torch.int64)
+    res = hash_join(torch.as_tensor(bk).cuda(), torch.as_tensor(pk).cuda(),
+                    [payload.cuda()] if pay else [], unique=unique, bits=bits)
+    assert res is not None
+    pi, bi, outs = res
+    got = _sorted_pairs(pi, bi)
+    want = oracle.join_pairs_sorted(np.asarray(bk), np.asarray(pk))
+    if unique:
+        # a unique build: the oracle's unique-hash pairs are the same multiset
+        rb, rp = oracle.unique_hash_pairs([np.asarray(bk)], [np.asarray(pk)])
+        w2 = np.stack([rb, rp], 1) if rb.size else np.empty((0, 2), np.int64)
+        w2 = w2[np.lexsort((w2[:, 1], w2[:, 0]))] if w2.shape[0] else w2
+        assert np.array_equal(w2, want)
+    assert np.array_equal(got, want)
+    if pay:
+        assert np.array_equal(outs[0].cpu().numpy(), payload.numpy()[bi.cpu().numpy()])
+    return pi, bi
+
+
+@pytest.mark.parametrize("bits", [0, 4, 8, 12])
+def test_unique_random(bits):
+    rng = np.random.default_rng(bits)
+    bk = rng.permutation(200_000).astype(np.int64) * 7 - 5
+    pk = bk[rng.integers(0, bk.size, 1_000_000)]
+    pk[rng.random(pk.size) < 0.1] = 1 << 50
+    _check(bk, pk, True, bits)
+
+
+@pytest.mark.parametrize("bits", [0, 6, 12])
+def test_multi_random(bits):
+    rng = np.random.default_rng(100 + bits)
+    bk = rng.integers(0, 50_000, 150_000)
+    pk = rng.integers(-100, 60_000, 400_000)
+    _check(bk, pk, False, bits)
+
+
+def test_edges():
+    for bk, pk in (([], [1, 2]), ([1, 2], []), ([1, 2], [3, 4]), ([5], [5, 5, 5])):
+        for unique in (True, False):
+            _check(np.array(bk, np.int64), np.array(pk, np.int64), unique, pay=False)
+
+
+def test_extreme_keys():
+    keys = np.array([I64.min, I64.max, 0, -1, -64, 1], np.int64)
+    probe = np.array([I64.min, 5, I64.max, -64, I64.min, 1, 0], np.int64)
+    _check(keys, probe, True, bits=2)
+    _check(np.concatenate([keys, keys]), probe, False, bits=2)
+
+
+def test_oversized_build_buckets_take_chunked_tables():
+    # 60K unique build rows in one bucket (bits 0): ten LDS chunks
+    rng = np.random.default_rng(7)
+    bk = rng.permutation(60_000).astype(np.int64)
+    pk = rng.integers(-1000, 61_000, 300_000)
+    _check(bk, pk, True, bits=0)
+    # multi: 6000 rows in one bucket (sorted in LDS), two LDS chunks
+    bk = rng.integers(0, 1500, 6000)
+    _check(bk, rng.integers(0, 1600, 50_000), False, bits=0)
+
+
+def test_multi_refuses_a_bucket_past_the_sort_capacity():
+    bk = torch.arange(20_000, dtype=torch.int64, device="cuda")
+    assert hash_join(bk, bk.clone(), unique=False, bits=0) is None
+
+
+def test_skewed_probe_and_duplicate_heavy_build():
+    rng = np.random.default_rng(3)
+    bk = np.repeat(np.arange(2000, dtype=np.int64), rng.integers(1, 9, 2000))
+    pk = np.where(rng.random(500_000) < 0.3, 17, rng.integers(0, 2500, 500_000))
+    _check(bk, pk, False, bits=5)
+
+
+def test_repeat_is_byte_identical():
+    rng = np.random.default_rng(11)
+    bk = torch.from_numpy(rng.integers(0, 10_000, 30_000)).cuda()
+    pk = torch.from_numpy(rng.integers(0, 12_000, 200_000)).cuda()
+    a = hash_join(bk, pk, unique=False, bits=6)
+    b = hash_join(bk, pk, unique=False, bits=6)
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+
+
+def test_plan_reuse():
+    rng = np.random.default_rng(12)
+    bk = torch.from_numpy(rng.permutation(100_000).astype(np.int64)).cuda()
+    plan = HashJoin(100_000, 400_000, unique=True)
+    for seed in range(2):
+        pk = torch.from_numpy(np.random.default_rng(seed).integers(0, 110_000, 400_000)).cuda()
+        n = plan.prepare(bk, pk)
+        op = torch.empty(n, dtype=torch.int64, device="cuda")
+        ob = torch.empty(n, dtype=torch.int64, device="cuda")
+        plan.write(op, ob)
+        got = _sorted_pairs(op, ob)
+        assert np.array_equal(got, oracle.join_pairs_sorted(bk.cpu().numpy(), pk.cpu().numpy()))
