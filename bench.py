@@ -1,0 +1,208 @@
+"""Benchmark of the equi-join hot path on one MI355X.
+
+Workload (the large plain join): build 16M unique int64 keys (a seeded
+permutation of 1..16M) with two int64 payload columns; probe 640M int64 keys
+uniform over the build keys, 10% replaced by absent keys drawn from
+[16M+1, 2^62).  A step is one whole join over inputs resident in HBM: both
+sides radix-partitioned by mix64, each bucket joined against an LDS hash
+table, the matched (probe row, build row) pairs written with one gathered
+build payload column.  tuples/s = (build rows + probe rows) / step time.
+
+Prints one JSON line (see the fields below): the throughput, the roofline of
+the dominant kernel (its algorithmic bytes over its average launch time,
+measured with HIP events over the timed steps on the launch stream), and the
+CPU baseline: the oracle's C restatement of the reference's unique-hash
+join program timed on this host on a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_c2(nb: int, np_: int, seed: int, device):
+    import torch
+    g = torch.Generator(device=device).manual_seed(seed)
+    build = torch.randperm(nb, generator=g, device=device) + 1
+    pay0 = torch.randint(-(1 << 62), 1 << 62, (nb,), generator=g, device=device)
+    pay1 = torch.randint(-(1 << 62), 1 << 62, (nb,), generator=g, device=device)
+    idx = torch.randint(0, nb, (np_,), generator=g, device=device)
+    probe = build[idx]
+    del idx
+    absent = torch.rand(np_, generator=g, device=device) < 0.1
+    n_abs = int(absent.sum())
+    probe[absent] = torch.randint(nb + 1, 1 << 62, (n_abs,), generator=g, device=device)
+    del absent
+    return build, (pay0, pay1), probe, np_ - n_abs
+
+
+def kernel_bytes(name: str, nb: int, np_: int, pairs: int) -> float:
+    """Algorithmic bytes of one step's launches of a kernel: what the kernel
+    must read once and write once for its function (inputs, its own outputs;
+    no table or re-read traffic)."""
+    table = {
+        # key read (8 B) per row
+        "hj_probe_hist": np_ * 8, "hj_build_hist": nb * 8,
+        # key read 8 B, (hash 8 B, row id 4 B) written
+        "hj_probe_scatter": np_ * 20, "hj_build_scatter": nb * 20,
+        # partitioned probe hashes read, build hashes read
+        "hj_count": np_ * 8 + nb * 8,
+        # hashes + row ids read, pairs (2 x 8 B) + payload (8 B) written,
+        # payload read (8 B) per pair
+        "hj_write": np_ * 12 + nb * 8 + pairs * 32,
+    }
+    return float(table.get(name, 0))
+
+
+def cpu_baseline(build_cpu, probe_cpu, sample_rows: int):
+    import numpy as np
+    import oracle
+    bk = build_cpu.numpy()
+    pk = probe_cpu[:sample_rows].numpy()
+    oracle.unique_hash_pairs([bk[:1000]], [pk[:1000]])  # warm the library
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        bi, pi = oracle.unique_hash_pairs([bk], [pk])
+        oracle.gather_with_null(bk, bi)
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[1]
+    return {"value": (bk.shape[0] + pk.shape[0]) / t, "unit": "tuples/s", "cores": 1,
+            "kind": "port",
+            "sample": f"the full {bk.shape[0]}-row build and the first {pk.shape[0]} probe "
+                      f"rows of the same inputs; oracle C restatement of the reference's "
+                      f"unique-hash join (table build, probe, probe-order pairs) plus the "
+                      f"payload gather, one thread, median of 3 ({t:.2f} s)"}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--build-rows", type=int, default=16_000_000)
+    ap.add_argument("--probe-rows", type=int, default=640_000_000)
+    ap.add_argument("--bits", type=int, default=None)
+    ap.add_argument("--cpu-sample", type=int, default=16_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the join's multi-GPU sharding is a design, not a measured path)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pageflow_amd import _lib
+    from pageflow_amd.join import HashJoin
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    nb, np_ = args.build_rows, args.probe_rows
+    log(f"generating {nb} build / {np_} probe rows")
+    build, pays, probe, expect = make_c2(nb, np_, 2, dev)
+    torch.cuda.synchronize()
+    plan = HashJoin(nb, np_, unique=True, bits=args.bits, device=dev)
+    out_p = torch.empty(np_, dtype=torch.int64, device=dev)
+    out_b = torch.empty(np_, dtype=torch.int64, device=dev)
+    out_pay = torch.empty(np_, dtype=torch.int64, device=dev)
+    log(f"plan: bits={plan.bits} workspace={plan.ws_bytes / 2**30:.2f} GiB")
+
+    def step():
+        n = plan.prepare(build, probe)
+        plan.write(out_p, out_b, [pays[0]], [out_pay])
+        return n
+
+    for i in range(args.warmup):
+        n = step()
+        log(f"warmup {i}: {n} pairs")
+    torch.cuda.synchronize()
+    _lib.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        n = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = _lib.profile_read()
+    _lib.profile(False)
+    ms = (t1 - t0) * 1e3 / args.steps
+    log(f"timed {args.steps} steps: {ms:.3f} ms/step, {n} pairs")
+
+    # size-independent checks at full size: every present probe row matched
+    # once, each pair's keys equal, the payload gathered by build row
+    pairs = n
+    assert pairs == expect, (pairs, expect)
+    op, ob = out_p[:pairs], out_b[:pairs]
+    assert bool((build[ob] == probe[op]).all())
+    assert bool((out_pay[:pairs] == pays[0][ob]).all())
+    assert int(torch.unique(op).numel()) == pairs
+    log("checks passed: pair count, key equality, payload gather, unique probe rows")
+
+    kernels = {}
+    for name, (tot_ms, launches) in prof.items():
+        per_step = launches / args.steps
+        kernels[name] = {"avg_ms": tot_ms / launches, "launches_per_step": per_step}
+    dom = max(prof, key=lambda k: prof[k][0])
+    dom_ms = prof[dom][0] / prof[dom][1]
+    per_launch = kernel_bytes(dom, nb, np_, pairs) / max(1.0, prof[dom][1] / args.steps)
+    achieved = per_launch / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{dom}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    e2e_bytes = nb * 8 + np_ * 8 + pairs * 16 + pairs * 16
+    tuples = (nb + np_) / (ms * 1e-3)
+
+    cpu = None
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        cpu = cpu_baseline(build.cpu(), probe[:args.cpu_sample].cpu(), args.cpu_sample)
+
+    line = {
+        "metric": "join tuples/s + % HBM roofline, batched vs per-join launch; p99 ms (serving)",
+        "value": tuples,
+        "unit": "tuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": "large plain join: build 16M unique int64 keys + 2 int64 "
+                               "payloads, probe 640M int64 keys (10% absent); pairs + 1 "
+                               "gathered payload",
+                   "build_rows": nb, "probe_rows": np_, "pairs": pairs, "bits": plan.bits},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": traffic},
+        "e2e": {"algorithmic_bytes": e2e_bytes,
+                "GBps": e2e_bytes / (ms * 1e-3) / 1e9,
+                "frac": e2e_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -38,6 +38,16 @@ int pf_version(void);
 /* The calling thread's last error message ("" after a successful call). */
 const char* pf_last_error(void);
 
+/* Per-kernel timing with HIP events (no counterpart in the reference, whose
+ * timing is Nsight's): pf_profile(1) clears and starts recording an event
+ * pair around every kernel the library launches, pf_profile(0) stops.
+ * pf_profile_read waits for the recorded events and fills, per kernel name
+ * (NUL-separated in `names`), the total milliseconds and the launches;
+ * returns the kernels filled. */
+int pf_profile(int32_t enable);
+int32_t pf_profile_read(char* names, int64_t names_bytes, double* total_ms, int64_t* launches,
+                        int32_t max_kernels);
+
 /* ------------------------------------------------------------ hashing --
  * mix64 (kernel_utils.cuh:18-22) of n int64 keys into out (int64 bits).
  * Replaces the torch-op twin operators/aggregate.py:95-104. */

@@ -27,6 +27,8 @@ PP = ctypes.POINTER(ctypes.c_void_p)
 _SIGNATURES = {
     "pf_version": (ctypes.c_int, []),
     "pf_last_error": (ctypes.c_char_p, []),
+    "pf_profile": (ctypes.c_int, [I32]),
+    "pf_profile_read": (I32, [ctypes.c_char_p, I64, P, P, I32]),
     "pf_mix64": (ctypes.c_int, [P, I64, P, P]),
     "pf_bucket_ids": (ctypes.c_int, [P, I64, I32, P, P]),
     "pf_partition_sort_workspace_bytes": (SZ, [I64, I64, I32, I32, I32, I32]),
@@ -125,3 +127,20 @@ def i32_array(vals):
     for i, v in enumerate(vals):
         arr[i] = int(v)
     return arr
+
+
+def profile(enable: bool) -> None:
+    """Starts (clearing) or stops the library's per-kernel event timing."""
+    check(load().pf_profile(int(bool(enable))))
+
+
+def profile_read(max_kernels: int = 64):
+    """{kernel name: (total ms, launches)} recorded since profile(True)."""
+    lib = load()
+    names = ctypes.create_string_buffer(max_kernels * 96)
+    ms = (ctypes.c_double * max_kernels)()
+    cnt = (ctypes.c_int64 * max_kernels)()
+    k = lib.pf_profile_read(names, len(names), ctypes.cast(ms, ctypes.c_void_p),
+                            ctypes.cast(cnt, ctypes.c_void_p), max_kernels)
+    raw = names.raw.split(b"\0")
+    return {raw[i].decode(): (ms[i], int(cnt[i])) for i in range(k)}

@@ -32,7 +32,8 @@ namespace pf {
 size_t radix_partition_ws(int64_t n, int bits);
 int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
                     int shift, int mode, void* keys_out, int32_t* vals_out,
-                    int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st);
+                    int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st,
+                    int who);
 
 namespace {
 
@@ -407,10 +408,10 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
   const size_t pws = radix_partition_ws(nb > np ? nb : np, bits);
   // both sides as (mix64(key), row) by the top `bits` bits of the hash
   if (radix_partition(bkeys, 8, nullptr, nb, bits, 0, 1, w.bh, w.brow, w.boff, w.part_ws, pws,
-                      st))
+                      st, 1))
     return 1;
   if (radix_partition(pkeys, 8, nullptr, np, bits, 0, 1, w.ph, w.prow, w.poff, w.part_ws, pws,
-                      st))
+                      st, 2))
     return 1;
   hj_items<<<1, 1024, 0, st>>>(w.boff, w.poff, P, cap, w.item_off, w.info);
   hj_readback<<<1, 64, 0, st>>>(w.info, nullptr, (int64_t*)hdev);
@@ -427,15 +428,21 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
   }
   if (n_items > max_items(np, bits)) return fail("hash_join: work items overflow");
   if (multi && nb > 0) {
+    {
+    KTimer _kt("hj_sort_buckets", st);
     hj_sort_buckets<<<(unsigned)P, SORT_NT, 0, st>>>(w.bh, w.brow, w.boff);
+  }
     if (launch_check("hash_join sort")) return 1;
   }
   HjParams p = params_of(w, bits, n_items);
   if (n_items > 0) {
+    {
+    KTimer _kt("hj_count", st);
     if (multi)
       hj_probe<true, 0><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
     else
       hj_probe<false, 0><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
+  }
     if (launch_check("hash_join count")) return 1;
   }
   if (scan_i64_excl(w.item_cnt, n_items, w.total, w.scan, st)) return 1;
@@ -468,10 +475,13 @@ int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int6
     p.pay_src[c] = bpay[c];
     p.pay_out[c] = pay_out[c];
   }
-  if (multi)
+  {
+    KTimer _kt("hj_write", st);
+    if (multi)
     hj_probe<true, 1><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
   else
     hj_probe<false, 1><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
+  }
   if (launch_check("hash_join write")) return 1;
   return ok();
 }

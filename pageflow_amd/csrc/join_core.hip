@@ -458,14 +458,20 @@ int pf_join_unique_hash(const int64_t* b0, const int64_t* b1, int64_t nb, const 
     if (fill_i32(u.hrow, cap, INT32_MAX, st)) return 1;
     if (fill_i32(u.special, 1, INT32_MAX, st)) return 1;
     k_uh_build1<<<grid_for(nb, 256), 256, 0, st>>>(b0, nb, u.hkey, u.hrow, cap, u.special);
+    {
+    KTimer _kt("uh_probe", st);
     k_uh_probe<false><<<(unsigned)tiles, CT_NT, 0, st>>>(b0, b0, p0, p0, np, u.hkey, u.hrow,
                                                          cap, u.special, u.partner,
                                                          u.tilecnt);
   }
+  }
   if (launch_check("join_unique_hash")) return 1;
   if (scan_i32_to_i64_excl(u.tilecnt, u.offs, tiles, u.count, u.scan, st)) return 1;
   k_readback<<<1, 64, 0, st>>>(u.count, 1, (int64_t*)hdev);
-  k_uh_compact<<<(unsigned)tiles, CT_NT, 0, st>>>(u.partner, np, u.offs, probe_idx, build_idx);
+  {
+    KTimer _kt("uh_compact", st);
+    k_uh_compact<<<(unsigned)tiles, CT_NT, 0, st>>>(u.partner, np, u.offs, probe_idx, build_idx);
+  }
   if (launch_check("join_unique_hash")) return 1;
   if (wait_stream(st, "join_unique_hash")) return 1;
   return ok();
@@ -519,7 +525,8 @@ int pf_semi_anti_mask(const int64_t* bk, int64_t nb, const int64_t* pk, int64_t 
   if (nb > MAX_ROWS) return fail("semi_anti_mask: too many build rows");
   if (op < OP_NONE || op > OP_GE)
     return fail("semi_anti_mask: unknown residual operator " + std::to_string(op));
-  if (op != OP_NONE && (!bv || !pv)) return fail("semi_anti_mask: a residual needs both columns");
+  if (op != OP_NONE && (!bv || (np > 0 && !pv)))
+    return fail("semi_anti_mask: a residual needs both columns");
   SetWs s = carve_set(workspace, nb, np, op, false);
   if (workspace_bytes < s.bytes) return fail("semi_anti_mask: workspace too small");
   if (build_set(s, bk, bv, nb, op, st)) return 1;

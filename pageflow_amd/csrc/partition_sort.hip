@@ -655,10 +655,13 @@ int pf_partition_sort(const void* const* cols_in, void* const* cols_out,
   }
   p.hist = l.hist;
   p.bases = l.bases;
-  if (use_sel)
+  {
+    KTimer _kt("ps_hist", st);
+    if (use_sel)
     ps_hist<true><<<(unsigned)nblk, NT, 0, st>>>(p);
   else
     ps_hist<false><<<(unsigned)nblk, NT, 0, st>>>(p);
+  }
   ps_bucket_scan<<<(unsigned)B, NT, 0, st>>>(l.hist, nblk, l.counts);
   Rpa rpa;
   for (int c = 0; c < MAXCLS; ++c) rpa.v[c] = 1;
@@ -669,10 +672,13 @@ int pf_partition_sort(const void* const* cols_in, void* const* cols_out,
   hipEvent_t ev = stream_event(st);
   if (!ev) return fail("partition_sort: cannot create an event");
   if (hip_check(hipEventRecord(ev, st), "partition_sort")) return 1;
-  if (use_sel)
+  {
+    KTimer _kt("ps_scatter", st);
+    if (use_sel)
     ps_scatter<true><<<(unsigned)nblk, NT, 0, st>>>(p);
   else
     ps_scatter<false><<<(unsigned)nblk, NT, 0, st>>>(p);
+  }
   if (launch_check("partition_sort scatter")) return 1;
   // the call's one host wait: the counts, while the scatter runs
   if (hip_check(hipEventSynchronize(ev), "partition_sort")) return 1;

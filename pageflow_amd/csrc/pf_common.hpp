@@ -131,6 +131,17 @@ int wait_stream(hipStream_t st, const char* what);
 // it).
 int host_alias(void* host, void** dev);
 
+// Per-kernel timing (pf_profile): while enabled, a KTimer scope records a
+// HIP event pair around the launches it encloses on `st`, accumulated by
+// kernel name when read back.  Disabled, it costs one flag test.
+struct KTimer {
+  KTimer(const char* name, hipStream_t st);
+  ~KTimer();
+  const char* name;
+  hipStream_t st;
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
 // Bump allocator over a caller-provided workspace (every piece 256-B
 // aligned).  Passing base == nullptr only measures.
 struct Carve {

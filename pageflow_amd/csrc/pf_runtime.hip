@@ -4,8 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "pf_common.hpp"
 #include "../../include/pageflow_join.h"
@@ -61,6 +64,50 @@ int host_alias(void* host, void** dev) {
     return fail("host_out is not pinned, device-mapped host memory");
   }
   return 0;
+}
+
+// ----------------------------------------------------------- profile --
+
+namespace {
+struct Pending {
+  std::string name;
+  hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<Pending> g_pending;
+std::map<std::string, std::pair<double, int64_t>> g_prof;  // ms, launches
+
+void drain_locked() {
+  for (auto& p : g_pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto& e = g_prof[p.name];
+      e.first += ms;
+      e.second += 1;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  g_pending.clear();
+}
+}  // namespace
+
+KTimer::KTimer(const char* n, hipStream_t s) : name(n), st(s) {
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  if (!g_prof_on) return;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    a = b = nullptr;
+    return;
+  }
+  (void)hipEventRecord(a, st);
+}
+
+KTimer::~KTimer() {
+  if (!a) return;
+  (void)hipEventRecord(b, st);
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  g_pending.push_back({name, a, b});
 }
 
 // ------------------------------------------------------------- fills --
@@ -240,6 +287,33 @@ unsigned stream_grid(int64_t n) {
 extern "C" {
 
 int pf_version(void) { return PF_ABI_VERSION; }
+
+int pf_profile(int32_t enable) {
+  std::lock_guard<std::mutex> lock(pf::g_prof_mu);
+  pf::drain_locked();
+  pf::g_prof.clear();
+  pf::g_prof_on = enable != 0;
+  return pf::ok();
+}
+
+int32_t pf_profile_read(char* names, int64_t names_bytes, double* total_ms, int64_t* launches,
+                        int32_t max_kernels) {
+  std::lock_guard<std::mutex> lock(pf::g_prof_mu);
+  pf::drain_locked();
+  int32_t k = 0;
+  int64_t at = 0;
+  for (auto& e : pf::g_prof) {
+    if (k >= max_kernels) break;
+    const int64_t len = (int64_t)e.first.size() + 1;
+    if (at + len > names_bytes) break;
+    memcpy(names + at, e.first.c_str(), (size_t)len);
+    at += len;
+    total_ms[k] = e.second.first;
+    launches[k] = e.second.second;
+    ++k;
+  }
+  return k;
+}
 
 const char* pf_last_error(void) { return pf::g_error.c_str(); }
 

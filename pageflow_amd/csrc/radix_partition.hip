@@ -252,21 +252,33 @@ RpWs carve_rp(void* ws, int64_t n, int bits) {
   return r;
 }
 
+// kernel names for the profiler, by caller: the C entry point, the join's
+// build side, the join's probe side
+const char* const kHistName[3] = {"rp_hist", "hj_build_hist", "hj_probe_hist"};
+const char* const kScatterName[3] = {"rp_scatter", "hj_build_scatter", "hj_probe_scatter"};
+
 template <typename K, typename KO, int MODE, int LOGB>
 int rp_launch(const K* keys, const int32_t* vals, int64_t n, int bits, int shift, KO* kout,
-              int32_t* vout, int64_t* bucket_offsets, const RpWs& r, hipStream_t st) {
+              int32_t* vout, int64_t* bucket_offsets, const RpWs& r, hipStream_t st,
+              int who) {
   const int tile = RpShape<LOGB>::NT * RpShape<LOGB>::IT;
   const int64_t crows = chunk_rows(bits, tile_of(bits));
   const int64_t nchunks = n > 0 ? (n + crows - 1) / crows : 1;
   (void)tile;
   const int64_t entries = nchunks << bits;
-  rp_hist<K, KO, MODE, LOGB><<<(unsigned)nchunks, RpShape<LOGB>::NT, 0, st>>>(
+  {
+    KTimer _kt(kHistName[who], st);
+    rp_hist<K, KO, MODE, LOGB><<<(unsigned)nchunks, RpShape<LOGB>::NT, 0, st>>>(
       keys, n, crows, bits, shift, nchunks, r.hist);
+  }
   if (launch_check("radix_partition hist")) return 1;
   if (scan_i32_to_i64_excl(r.hist, r.offs, entries, nullptr, r.scan, st)) return 1;
   if (n > 0) {
+    {
+    KTimer _kt(kScatterName[who], st);
     rp_scatter<K, KO, MODE, LOGB><<<(unsigned)nchunks, RpShape<LOGB>::NT, 0, st>>>(
         keys, vals, n, crows, bits, shift, nchunks, r.offs, kout, vout);
+  }
     if (launch_check("radix_partition scatter")) return 1;
   }
   rp_offsets<<<grid_for((1 << bits) + 1, 256), 256, 0, st>>>(r.offs, nchunks, 1 << bits, n,
@@ -277,26 +289,26 @@ int rp_launch(const K* keys, const int32_t* vals, int64_t n, int bits, int shift
 template <typename K>
 int rp_dispatch(const K* keys, const int32_t* vals, int64_t n, int bits, int shift, int mode,
                 void* kout, int32_t* vout, int64_t* bucket_offsets, const RpWs& r,
-                hipStream_t st) {
+                hipStream_t st, int who) {
   if (mode == RP_HASH_OUT) {
     if (bits <= 8)
       return rp_launch<K, int64_t, RP_HASH_OUT, 8>(keys, vals, n, bits, shift, (int64_t*)kout,
-                                                   vout, bucket_offsets, r, st);
+                                                   vout, bucket_offsets, r, st, who);
     return rp_launch<K, int64_t, RP_HASH_OUT, 12>(keys, vals, n, bits, shift, (int64_t*)kout,
-                                                  vout, bucket_offsets, r, st);
+                                                  vout, bucket_offsets, r, st, who);
   }
   if (mode == RP_HASH_KEY) {
     if (bits <= 8)
       return rp_launch<K, K, RP_HASH_KEY, 8>(keys, vals, n, bits, shift, (K*)kout, vout,
-                                             bucket_offsets, r, st);
+                                             bucket_offsets, r, st, who);
     return rp_launch<K, K, RP_HASH_KEY, 12>(keys, vals, n, bits, shift, (K*)kout, vout,
-                                            bucket_offsets, r, st);
+                                            bucket_offsets, r, st, who);
   }
   if (bits <= 8)
     return rp_launch<K, K, RP_RAW, 8>(keys, vals, n, bits, shift, (K*)kout, vout,
-                                      bucket_offsets, r, st);
+                                      bucket_offsets, r, st, who);
   return rp_launch<K, K, RP_RAW, 12>(keys, vals, n, bits, shift, (K*)kout, vout,
-                                     bucket_offsets, r, st);
+                                     bucket_offsets, r, st, who);
 }
 
 }  // namespace
@@ -306,7 +318,8 @@ size_t radix_partition_ws(int64_t n, int bits) { return carve_rp(nullptr, n, bit
 
 int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
                     int shift, int mode, void* keys_out, int32_t* vals_out,
-                    int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st) {
+                    int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st,
+                    int who) {
   if (key_bytes != 8 && key_bytes != 4) return fail("radix_partition: keys are int64 or int32");
   if (bits < 0 || bits > 12) return fail("radix_partition: bits must be in [0, 12]");
   if (n < 0 || n >= (1LL << 32)) return fail("radix_partition: n out of range");
@@ -319,9 +332,9 @@ int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_
   if (ws_bytes < r.bytes) return fail("radix_partition: workspace too small");
   if (key_bytes == 8)
     return rp_dispatch<int64_t>((const int64_t*)keys, vals, n, bits, shift, mode, keys_out,
-                                vals_out, bucket_offsets, r, st);
+                                vals_out, bucket_offsets, r, st, who);
   return rp_dispatch<int32_t>((const int32_t*)keys, vals, n, bits, shift, mode, keys_out,
-                              vals_out, bucket_offsets, r, st);
+                              vals_out, bucket_offsets, r, st, who);
 }
 
 }  // namespace pf
@@ -339,7 +352,7 @@ int pf_radix_partition(const void* keys, int32_t key_bytes, const int32_t* vals,
                        int32_t* vals_out, int64_t* bucket_offsets, void* workspace,
                        size_t workspace_bytes, void* stream) {
   if (pf::radix_partition(keys, key_bytes, vals, n, bits, shift, mode, keys_out, vals_out,
-                          bucket_offsets, workspace, workspace_bytes, (hipStream_t)stream))
+                          bucket_offsets, workspace, workspace_bytes, (hipStream_t)stream, 0))
     return 1;
   return pf::ok();
 }

@@ -200,7 +200,7 @@ def semi_anti_mask(build_key: torch.Tensor, probe_key: torch.Tensor, anti: bool,
     None when a build key equals int64 min."""
     device = probe_key.device
     n_build, n_probe = build_key.numel(), probe_key.numel()
-    if n_build == 0:
+    if n_build == 0 or n_probe == 0:
         return torch.full((n_probe,), bool(anti), dtype=torch.bool, device=device)
     _check_column(build_key, "the build key", build_key)
     _check_column(probe_key, "the probe key", build_key)

@@ -55,11 +55,12 @@ def test_unique_random(bits):
     _check(bk, pk, True, bits)
 
 
-@pytest.mark.parametrize("bits", [0, 6, 12])
-def test_multi_random(bits):
+@pytest.mark.parametrize("bits,nb", [(0, 8000), (6, 150_000), (12, 150_000)])
+def test_multi_random(bits, nb):
+    # multi-match builds are sorted per bucket in LDS: at most 8192 rows a bucket
     rng = np.random.default_rng(100 + bits)
-    bk = rng.integers(0, 50_000, 150_000)
-    pk = rng.integers(-100, 60_000, 400_000)
+    bk = rng.integers(0, nb // 3, nb)
+    pk = rng.integers(-100, nb // 3 + 10_000, 400_000)
     _check(bk, pk, False, bits)
 
 
