@@ -90,6 +90,66 @@ def cpu_baseline(build_cpu, probe_cpu, sample_rows: int):
                       f"payload gather, one thread, median of 3 ({t:.2f} s)"}
 
 
+def bench_partition(args, dev) -> None:
+    """The standalone radix partition of the large join's 640M probe keys
+    (int64) with int32 row ids into 2^bits buckets, for each bits given.
+    Algorithmic bytes: keys and row ids read once and written once (24 B a
+    row)."""
+    import torch
+    from pageflow_amd import _lib, radix
+
+    nb, np_ = args.build_rows, args.probe_rows
+    log(f"generating {np_} probe keys")
+    _, _, probe, _ = make_c2(nb, np_, 2, dev)
+    rows = torch.arange(np_, dtype=torch.int32, device=dev)
+    kout = torch.empty_like(probe)
+    vout = torch.empty_like(rows)
+    sweep = {}
+    for bits in [int(b) for b in args.part_bits.split(",")]:
+        offs = torch.empty((1 << bits) + 1, dtype=torch.int64, device=dev)
+        ws = _lib.workspace(radix.workspace_bytes(np_, bits), dev)
+        run = lambda: radix.radix_partition(probe, bits, rows, out=(kout, vout),
+                                            bucket_offsets=offs, ws=ws)
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        _lib.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        prof = _lib.profile_read()
+        _lib.profile(False)
+        ms = (t1 - t0) * 1e3 / args.steps
+        sc = prof["rp_scatter"][0] / prof["rp_scatter"][1]
+        hi = prof["rp_hist"][0] / prof["rp_hist"][1]
+        # stability and counts: offsets end at n, a sample of buckets checked
+        assert int(offs[-1]) == np_
+        sweep[bits] = {"ms": ms, "rows_per_s": np_ / (ms * 1e-3),
+                       "GBps": np_ * 24 / (ms * 1e-3) / 1e9,
+                       "scatter_ms": sc, "hist_ms": hi,
+                       "scatter_GBps": np_ * 24 / (sc * 1e-3) / 1e9}
+        log(f"bits={bits}: {ms:.3f} ms ({sweep[bits]['GBps']:.0f} GB/s), scatter {sc:.3f} ms, "
+            f"hist {hi:.3f} ms")
+    best = min(sweep, key=lambda b: sweep[b]["ms"])
+    line = {
+        "metric": "radix partition rows/s (640M int64 keys + int32 row ids)",
+        "value": sweep[best]["rows_per_s"], "unit": "rows/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": sweep[best]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": "standalone radix partition of 640M int64 keys with int32 row "
+                               "ids", "bits": best},
+        "roofline": {"bound": "hbm", "kernel": "rp_scatter",
+                     "achieved": sweep[best]["scatter_GBps"], "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": sweep[best]["scatter_GBps"] / HBM_PEAK_GBPS,
+                     "traffic": None},
+        "sweep": sweep,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +160,8 @@ def main() -> None:
     ap.add_argument("--bits", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["join", "partition"], default="join")
+    ap.add_argument("--part-bits", default="8,9,10,11,12")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -113,6 +175,9 @@ def main() -> None:
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.workload == "partition":
+        bench_partition(args, dev)
+        return
     nb, np_ = args.build_rows, args.probe_rows
     log(f"generating {nb} build / {np_} probe rows")
     build, pays, probe, expect = make_c2(nb, np_, 2, dev)
