@@ -105,11 +105,13 @@ def bench_partition(args, dev) -> None:
     kout = torch.empty_like(probe)
     vout = torch.empty_like(rows)
     sweep = {}
-    for bits in [int(b) for b in args.part_bits.split(",")]:
+    cases = [(int(b), int(sh)) for b in args.part_bits.split(",")
+             for sh in args.part_shapes.split(",") if int(sh) < 0 or int(b) <= 8]
+    for bits, shape in cases:
         offs = torch.empty((1 << bits) + 1, dtype=torch.int64, device=dev)
         ws = _lib.workspace(radix.workspace_bytes(np_, bits), dev)
         run = lambda: radix.radix_partition(probe, bits, rows, out=(kout, vout),
-                                            bucket_offsets=offs, ws=ws)
+                                            bucket_offsets=offs, ws=ws, shape=shape)
         for _ in range(args.warmup):
             run()
         torch.cuda.synchronize()
@@ -126,12 +128,13 @@ def bench_partition(args, dev) -> None:
         hi = prof["rp_hist"][0] / prof["rp_hist"][1]
         # stability and counts: offsets end at n, a sample of buckets checked
         assert int(offs[-1]) == np_
-        sweep[bits] = {"ms": ms, "rows_per_s": np_ / (ms * 1e-3),
+        key = f"{bits}/{shape}"
+        sweep[key] = {"bits": bits, "shape": shape, "ms": ms, "rows_per_s": np_ / (ms * 1e-3),
                        "GBps": np_ * 24 / (ms * 1e-3) / 1e9,
                        "scatter_ms": sc, "hist_ms": hi,
                        "scatter_GBps": np_ * 24 / (sc * 1e-3) / 1e9}
-        log(f"bits={bits}: {ms:.3f} ms ({sweep[bits]['GBps']:.0f} GB/s), scatter {sc:.3f} ms, "
-            f"hist {hi:.3f} ms")
+        log(f"bits={bits} shape={shape}: {ms:.3f} ms ({sweep[key]['GBps']:.0f} GB/s), "
+            f"scatter {sc:.3f} ms, hist {hi:.3f} ms")
     best = min(sweep, key=lambda b: sweep[b]["ms"])
     line = {
         "metric": "radix partition rows/s (640M int64 keys + int32 row ids)",
@@ -140,7 +143,7 @@ def bench_partition(args, dev) -> None:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
         "data": "synthetic",
         "config": {"workload": "standalone radix partition of 640M int64 keys with int32 row "
-                               "ids", "bits": best},
+                               "ids", "bits": sweep[best]["bits"], "shape": sweep[best]["shape"]},
         "roofline": {"bound": "hbm", "kernel": "rp_scatter",
                      "achieved": sweep[best]["scatter_GBps"], "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": sweep[best]["scatter_GBps"] / HBM_PEAK_GBPS,
@@ -162,6 +165,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["join", "partition"], default="join")
     ap.add_argument("--part-bits", default="8,9,10,11,12")
+    ap.add_argument("--part-shapes", default="-1")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

@@ -45,6 +45,8 @@ _SIGNATURES = {
     "pf_join_expand_write": (ctypes.c_int, [P, P, I64, I64, P, P, P, P]),
     "pf_radix_partition_workspace_bytes": (SZ, [I64, I32]),
     "pf_radix_partition": (ctypes.c_int, [P, I32, P, I64, I32, I32, I32, P, P, P, P, SZ, P]),
+    "pf_radix_partition_shape": (ctypes.c_int, [P, I32, P, I64, I32, I32, I32, P, P, P, P, SZ,
+                                                I32, P]),
     "pf_hash_join_workspace_bytes": (SZ, [I64, I64, I32, I32]),
     "pf_hash_join_prepare": (ctypes.c_int, [P, I64, P, I64, I32, I32, P, P, SZ, P]),
     "pf_hash_join_write": (ctypes.c_int, [I64, I64, I32, I32, I64, PP, I32, P, P, PP, P, SZ,

@@ -33,7 +33,7 @@ size_t radix_partition_ws(int64_t n, int bits);
 int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
                     int shift, int mode, void* keys_out, int32_t* vals_out,
                     int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st,
-                    int who);
+                    int who, int shape);
 
 namespace {
 
@@ -42,7 +42,7 @@ constexpr int HJ_W = HJ_NT / kWave;
 constexpr int HJ_IT = 4;                   // probe rows a thread per round
 constexpr int HJ_ROUND = HJ_NT * HJ_IT;
 constexpr int CAP_U = 6144;                // unique: build rows an LDS chunk
-constexpr int TS_U = 16384;                // unique: table slots
+constexpr int TS_U = 8192;                 // unique: table slots
 constexpr int CAP_M = 4096;                // multi: build rows an LDS chunk
 constexpr int TS_M = 8192;
 constexpr int SORT_CAP = 8192;             // multi: largest build bucket sorted in LDS
@@ -204,116 +204,165 @@ struct Lds<true> {
   static constexpr int CAP = CAP_M, TS = TS_M;
 };
 
-// MODE 0: count pairs per item; MODE 1: write them.
+// MODE 0: count pairs per item; MODE 1: write them.  Probe rows are taken
+// in rounds of HJ_ROUND, row r0 + k * HJ_NT + tid in slot k of a thread, so a
+// wave's lanes touch consecutive rows: loads coalesce and, in the write
+// pass, consecutive matches land on consecutive output rows.  A match's
+// build row and first payload come from LDS (staged with the chunk).
+static_assert(HJ_IT * HJ_W == kWave, "one wave scans a round's (slot, wave) totals");
+
 template <bool MULTI, int MODE>
 __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
   constexpr int CAP = Lds<MULTI>::CAP, TS = Lds<MULTI>::TS;
+  constexpr bool W1 = MODE == 1;
   __shared__ uint64_t keys[CAP];
   __shared__ uint32_t slot[TS];
+  __shared__ int32_t rows[W1 ? CAP : 1];
+  __shared__ int64_t pay0[W1 ? CAP : 1];
   __shared__ uint16_t glen[MULTI ? CAP : 1];
-  __shared__ int64_t tmp[HJ_W + 1];
-  const int tid = threadIdx.x;
+  __shared__ int64_t cnt[HJ_IT * HJ_W];
+  const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   const int64_t item = blockIdx.x;
   if (item >= p.n_items) return;
   const Item it = item_of(p, item);
   int64_t pos = 0;
-  if (MODE == 1) pos = p.item_pos[item];
+  if (W1) pos = p.item_pos[item];
   int64_t count = 0;
+  const bool has_pay = W1 && p.npay > 0;
   // a bucket larger than an LDS chunk is joined one build chunk at a time
   for (int64_t cb = it.bs; cb < it.be; cb += CAP) {
-  const int nloc = (int)(it.be - cb < CAP ? it.be - cb : CAP);
-  __syncthreads();  // the previous chunk's lookups are done
-  for (int s = tid; s < TS; s += HJ_NT) slot[s] = 0;
-  for (int i = tid; i < nloc; i += HJ_NT) keys[i] = p.bh[cb + i];
-  __syncthreads();
-  for (int i = tid; i < nloc; i += HJ_NT) {
-    const uint64_t h = keys[i];
-    if (MULTI && i > 0 && keys[i - 1] == h) continue;  // not a run head
-    if (MULTI) {
-      int len = 1;
-      while (i + len < nloc && keys[i + len] == h) ++len;
-      glen[i] = (uint16_t)(len > 65535 ? 65535 : len);
+    const int nloc = (int)(it.be - cb < CAP ? it.be - cb : CAP);
+    __syncthreads();  // the previous chunk's lookups are done
+    for (int s = tid; s < TS; s += HJ_NT) slot[s] = 0;
+    for (int i = tid; i < nloc; i += HJ_NT) {
+      keys[i] = p.bh[cb + i];
+      if (W1) rows[i] = p.brow[cb + i];
+      if (has_pay) pay0[i] = p.pay_src[0][cb + i];
     }
-    uint32_t s = (uint32_t)h & (TS - 1);
-    for (;;) {
-      const uint32_t cur = atomicCAS(&slot[s], 0u, (uint32_t)(i + 1));
-      if (cur == 0) break;
-      if (keys[cur - 1] == h) {  // a repeated hash (unique mode): lowest row
-        atomicMin(&slot[s], (uint32_t)(i + 1));
-        break;
+    __syncthreads();
+    for (int i = tid; i < nloc; i += HJ_NT) {
+      const uint64_t h = keys[i];
+      if (MULTI && i > 0 && keys[i - 1] == h) continue;  // not a run head
+      if (MULTI) {
+        int len = 1;
+        while (i + len < nloc && keys[i + len] == h) ++len;
+        glen[i] = (uint16_t)(len > 65535 ? 65535 : len);
       }
-      s = (s + 1) & (TS - 1);
+      uint32_t s = (uint32_t)h & (TS - 1);
+      for (;;) {
+        const uint32_t cur = atomicCAS(&slot[s], 0u, (uint32_t)(i + 1));
+        if (cur == 0) break;
+        if (keys[cur - 1] == h) {  // a repeated hash (unique mode): lowest row
+          atomicMin(&slot[s], (uint32_t)(i + 1));
+          break;
+        }
+        s = (s + 1) & (TS - 1);
+      }
     }
-  }
-  __syncthreads();
+    __syncthreads();
 
-  for (int64_t r0 = it.ps; r0 < it.pe; r0 += HJ_ROUND) {
-    int hidx[HJ_IT];
-    int m[HJ_IT];
-    int msum = 0;
+    for (int64_t r0 = it.ps; r0 < it.pe; r0 += HJ_ROUND) {
+      int hidx[HJ_IT];
+      int m[HJ_IT];
 #pragma unroll
-    for (int k = 0; k < HJ_IT; ++k) {
-      const int64_t r = r0 + (int64_t)tid * HJ_IT + k;
-      hidx[k] = -1;
-      m[k] = 0;
-      if (r < it.pe) {
-        const uint64_t h = p.ph[r];
-        uint32_t s = (uint32_t)h & (TS - 1);
-        for (;;) {
-          const uint32_t v = slot[s];
-          if (v == 0) break;
-          if (keys[v - 1] == h) {
-            hidx[k] = (int)v - 1;
-            break;
+      for (int k = 0; k < HJ_IT; ++k) {
+        const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
+        hidx[k] = -1;
+        m[k] = 0;
+        if (r < it.pe) {
+          const uint64_t h = p.ph[r];
+          uint32_t s = (uint32_t)h & (TS - 1);
+          for (;;) {
+            const uint32_t v = slot[s];
+            if (v == 0) break;
+            if (keys[v - 1] == h) {
+              hidx[k] = (int)v - 1;
+              break;
+            }
+            s = (s + 1) & (TS - 1);
           }
-          s = (s + 1) & (TS - 1);
-        }
-        if (hidx[k] >= 0) {
-          if (MULTI) {
-            // a run longer than glen's range is counted to its end
-            int len = glen[hidx[k]];
-            if (len == 65535)
-              while (hidx[k] + len < nloc && keys[hidx[k] + len] == h) ++len;
-            m[k] = len;
-          } else {
-            m[k] = 1;
+          if (hidx[k] >= 0) {
+            if (MULTI) {
+              int len = glen[hidx[k]];
+              if (len == 65535)  // a run longer than glen's range, counted to its end
+                while (hidx[k] + len < nloc && keys[hidx[k] + len] == h) ++len;
+              m[k] = len;
+            } else {
+              m[k] = 1;
+            }
           }
         }
       }
-      msum += m[k];
-    }
-    if (MODE == 0) {
-      count += msum;
-      continue;
-    }
-    int64_t tot;
-    int64_t o = pos + block_excl_scan<HJ_NT>((int64_t)msum, tmp, &tot);
-    pos += tot;
+      if (!W1) {
 #pragma unroll
-    for (int k = 0; k < HJ_IT; ++k) {
-      if (m[k] == 0) continue;
-      const int64_t r = r0 + (int64_t)tid * HJ_IT + k;
-      const int64_t prow = p.prow[r];
-      for (int q = 0; q < m[k]; ++q) {
-        const int32_t brow = p.brow[cb + hidx[k] + q];
-        p.out_probe[o] = prow;
-        p.out_build[o] = brow;
-        for (int c = 0; c < p.npay; ++c) p.pay_out[c][o] = p.pay_src[c][brow];
-        ++o;
+        for (int k = 0; k < HJ_IT; ++k) count += m[k];
+        continue;
       }
+      // output offsets in row order: within a wave by lane, across (round
+      // slot, wave) by one wave's scan of the HJ_IT * HJ_W totals
+      int64_t inw[HJ_IT];
+#pragma unroll
+      for (int k = 0; k < HJ_IT; ++k) {
+        int64_t incl, tot;
+        if (MULTI) {
+          incl = wave_incl_scan((int64_t)m[k]);
+          tot = __shfl(incl, kWave - 1, kWave);
+        } else {
+          const uint64_t bal = ballot(m[k] != 0);
+          incl = mbcnt(bal) + m[k];
+          tot = __popcll(bal);
+        }
+        inw[k] = incl - m[k];
+        if (lane == 0) cnt[k * HJ_W + w] = tot;
+      }
+      __syncthreads();
+      if (w == 0) {
+        const int64_t v = cnt[lane];
+        cnt[lane] = wave_incl_scan(v) - v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < HJ_IT; ++k) {
+        if (m[k] == 0) continue;
+        const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
+        const int64_t prow = p.prow[r];
+        int64_t o = pos + cnt[k * HJ_W + w] + inw[k];
+        for (int q = 0; q < m[k]; ++q, ++o) {
+          const int li = hidx[k] + q;
+          p.out_probe[o] = prow;
+          p.out_build[o] = rows[li];
+          if (has_pay) p.pay_out[0][o] = pay0[li];
+          if (p.npay > 1) p.pay_out[1][o] = p.pay_src[1][cb + li];
+        }
+      }
+      // the round's last (slot, wave) total, from the wave that holds it
+      __shared__ int64_t last_tot;
+      if (tid == HJ_NT - 1) {
+        const int k = HJ_IT - 1;
+        last_tot = cnt[k * HJ_W + HJ_W - 1] + inw[k] + m[k];
+      }
+      __syncthreads();
+      pos += last_tot;
     }
-  }
   }  // build chunks
-  if (MODE == 0) {
+  if (!W1) {
     const int64_t t = wave_sum(count);
-    if (lane_id() == 0) tmp[tid / kWave] = t;
+    if (lane == 0) cnt[w] = t;
     __syncthreads();
     if (tid == 0) {
       int64_t s = 0;
-      for (int w = 0; w < HJ_W; ++w) s += tmp[w];
+      for (int q = 0; q < HJ_W; ++q) s += cnt[q];
       p.item_cnt[item] = s;
     }
   }
+}
+
+// dst[i] = src[idx[i]]: a build payload column in partition order.
+__global__ void hj_gather_pay(const int64_t* __restrict__ src, const int32_t* __restrict__ idx,
+                              int64_t n, int64_t* __restrict__ dst) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[idx[i]];
 }
 
 __global__ void hj_readback(const int64_t* __restrict__ a, const int64_t* __restrict__ b,
@@ -334,6 +383,7 @@ struct HjWs {
   int64_t* info;       // [0] items, [1] largest build bucket
   int64_t* total;
   int64_t* item_cnt;   // [max items] (counts, then scanned in place)
+  int64_t* pay_part[MAX_PAY];  // build payloads in partition order
   void* part_ws;
   void* scan;
   size_t bytes;
@@ -357,6 +407,7 @@ HjWs carve_hj(void* ws, int64_t nb, int64_t np, int bits, bool multi) {
   w.total = cv.take<int64_t>(2);
   const int64_t mi = max_items(np, bits);
   w.item_cnt = cv.take<int64_t>(mi);
+  for (int c = 0; c < MAX_PAY; ++c) w.pay_part[c] = cv.take<int64_t>(nb > 0 ? nb : 1);
   const size_t pws = radix_partition_ws(nb > np ? nb : np, bits);
   w.part_ws = cv.take<char>(pws);
   w.scan = cv.take<char>(scan_i64_scratch(mi));
@@ -408,10 +459,10 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
   const size_t pws = radix_partition_ws(nb > np ? nb : np, bits);
   // both sides as (mix64(key), row) by the top `bits` bits of the hash
   if (radix_partition(bkeys, 8, nullptr, nb, bits, 0, 1, w.bh, w.brow, w.boff, w.part_ws, pws,
-                      st, 1))
+                      st, 1, -1))
     return 1;
   if (radix_partition(pkeys, 8, nullptr, np, bits, 0, 1, w.ph, w.prow, w.poff, w.part_ws, pws,
-                      st, 2))
+                      st, 2, -1))
     return 1;
   hj_items<<<1, 1024, 0, st>>>(w.boff, w.poff, P, cap, w.item_off, w.info);
   hj_readback<<<1, 64, 0, st>>>(w.info, nullptr, (int64_t*)hdev);
@@ -472,7 +523,13 @@ int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int6
   p.out_build = out_build;
   p.npay = npay;
   for (int c = 0; c < npay; ++c) {
-    p.pay_src[c] = bpay[c];
+    {
+      KTimer _kt("hj_gather_pay", st);
+      hj_gather_pay<<<grid_for(nb, 256) < 16384 ? grid_for(nb, 256) : 16384, 256, 0, st>>>(
+          bpay[c], w.brow, nb, w.pay_part[c]);
+    }
+    if (launch_check("hash_join payload")) return 1;
+    p.pay_src[c] = w.pay_part[c];
     p.pay_out[c] = pay_out[c];
   }
   {

@@ -222,6 +222,125 @@ __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_scatter(
   }
 }
 
+// Up to 256 buckets: the same stable partition with the next tile's keys
+// loaded while the current one is ranked (registers double-buffered), each
+// bucket owned by one thread across the per-tile bookkeeping (wave prefix,
+// tile scan, global base, cursor), and the tile's bucket ids staged as bytes.
+template <typename K, typename KO, int MODE, int NT, int IT>
+__global__ __launch_bounds__(NT) void rp_scatter8(
+    const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t n, int64_t crows,
+    int bits, int shift, int64_t nchunks, const int64_t* __restrict__ offs,
+    KO* __restrict__ kout, int32_t* __restrict__ vout) {
+  constexpr int W = NT / kWave, TILE = NT * IT, BMAX = 256, WROWS = kWave * IT;
+  static_assert(BMAX <= NT, "a thread owns each bucket");
+  __shared__ uint16_t wh[W * BMAX];
+  __shared__ uint16_t tstart[BMAX];
+  __shared__ uint32_t gb[BMAX];
+  __shared__ KO stage_k[TILE];
+  __shared__ int32_t stage_v[TILE];
+  __shared__ uint8_t sb[TILE];
+  __shared__ int32_t tmp[W + 1];
+
+  const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
+  const int B = 1 << bits;
+  const int64_t c0 = (int64_t)blockIdx.x * crows;
+  const int64_t c1 = c0 + crows < n ? c0 + crows : n;
+  uint32_t cursor = tid < B ? (uint32_t)offs[(int64_t)tid * nchunks + blockIdx.x] : 0u;
+  for (int x = tid; x < W * B; x += NT) wh[x] = 0;
+
+  K kn[IT];
+  int32_t vn[IT];
+  auto load_tile = [&](int64_t t0) {
+    const int64_t seg = t0 + (int64_t)w * WROWS;
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const int64_t i = seg + k * kWave + lane;
+      if (i < c1) {
+        kn[k] = keys[i];
+        vn[k] = vals ? vals[i] : (int32_t)i;
+      }
+    }
+  };
+  load_tile(c0);
+  __syncthreads();
+  for (int64_t t0 = c0; t0 < c1; t0 += TILE) {
+    K kc[IT];
+    int32_t vc[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      kc[k] = kn[k];
+      vc[k] = vn[k];
+    }
+    if (t0 + TILE < c1) load_tile(t0 + TILE);
+    const int64_t seg = t0 + (int64_t)w * WROWS;
+    uint32_t d[IT];
+    KO ov[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const int64_t i = seg + k * kWave + lane;
+      d[k] = i < c1 ? rp_digit<K, KO, MODE>(kc[k], bits, shift, ov[k]) : ~0u;
+    }
+    int rk[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const uint64_t peers = match_digit(d[k], bits);
+      const int below = mbcnt(peers);
+      const bool valid = d[k] != ~0u;
+      const int base = valid ? (int)wh[w * B + d[k]] : 0;
+      rk[k] = base + below;
+      if (valid && below == 0) wh[w * B + d[k]] = (uint16_t)(base + __popcll(peers));
+    }
+    __syncthreads();
+    // thread b: bucket b's wave starts, its tile run's start and global base
+    int tot = 0;
+    if (tid < B) {
+#pragma unroll
+      for (int q = 0; q < W; ++q) {
+        const int t = wh[q * B + tid];
+        wh[q * B + tid] = (uint16_t)tot;
+        tot += t;
+      }
+    }
+    {
+      const int incl = wave_incl_scan(tot);
+      if (lane == kWave - 1) tmp[w] = incl;
+      __syncthreads();
+      if (w == 0) {
+        const int x = lane < W ? tmp[lane] : 0;
+        const int xs = wave_incl_scan(x);
+        if (lane < W) tmp[lane] = xs - x;
+      }
+      __syncthreads();
+      if (tid < B) {
+        const int ex = tmp[w] + incl - tot;
+        tstart[tid] = (uint16_t)ex;
+        gb[tid] = cursor - (uint32_t)ex;
+        cursor += (uint32_t)tot;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      if (d[k] != ~0u) {
+        const int pos = tstart[d[k]] + wh[w * B + d[k]] + rk[k];
+        stage_k[pos] = ov[k];
+        stage_v[pos] = vc[k];
+        sb[pos] = (uint8_t)d[k];
+      }
+    }
+    const int64_t left = c1 - t0;
+    const int tvalid = left < TILE ? (int)left : TILE;
+    __syncthreads();
+    for (int s = tid; s < tvalid; s += NT) {
+      const uint32_t o = gb[sb[s]] + (uint32_t)s;
+      kout[o] = stage_k[s];
+      vout[o] = stage_v[s];
+    }
+    for (int x = tid; x < W * B; x += NT) wh[x] = 0;
+    __syncthreads();
+  }
+}
+
 __global__ void rp_offsets(const int64_t* __restrict__ offs, int64_t nchunks, int B, int64_t n,
                            int64_t* __restrict__ bucket_offsets) {
   for (int b = blockIdx.x * blockDim.x + threadIdx.x; b <= B; b += gridDim.x * blockDim.x)
@@ -235,14 +354,29 @@ struct RpWs {
   size_t bytes;
 };
 
-int tile_of(int bits) {
-  return bits <= 8 ? RpShape<8>::NT * RpShape<8>::IT : RpShape<12>::NT * RpShape<12>::IT;
+// Shapes of the <= 8-bit scatter (threads, rows a thread per tile); the
+// default is the measured best on MI355X, a caller (the autotuner) may pick
+// another.
+struct Shape8 {
+  int nt, it;
+};
+constexpr Shape8 kShapes8[4] = {{256, 8}, {256, 16}, {512, 8}, {512, 16}};
+constexpr int kDefaultShape8 = 2;
+
+int tile_of(int bits, int shape) {
+  if (bits > 8) return RpShape<12>::NT * RpShape<12>::IT;
+  const Shape8 sh = kShapes8[shape < 0 || shape > 3 ? kDefaultShape8 : shape];
+  return sh.nt * sh.it;
 }
+
+// The largest tile of any shape fixes the chunk rows, so one workspace size
+// serves every shape.
+int64_t chunk_rows_of(int bits) { return chunk_rows(bits, 8192); }
 
 RpWs carve_rp(void* ws, int64_t n, int bits) {
   Carve cv(ws);
   RpWs r;
-  const int64_t crows = chunk_rows(bits, tile_of(bits));
+  const int64_t crows = chunk_rows_of(bits);
   const int64_t nchunks = n > 0 ? (n + crows - 1) / crows : 1;
   const int64_t entries = nchunks << bits;
   r.hist = cv.take<int32_t>(entries);
@@ -257,30 +391,46 @@ RpWs carve_rp(void* ws, int64_t n, int bits) {
 const char* const kHistName[3] = {"rp_hist", "hj_build_hist", "hj_probe_hist"};
 const char* const kScatterName[3] = {"rp_scatter", "hj_build_scatter", "hj_probe_scatter"};
 
+template <typename K, typename KO, int MODE, int NT, int IT>
+void launch8(const K* keys, const int32_t* vals, int64_t n, int64_t crows, int bits, int shift,
+             int64_t nchunks, const int64_t* offs, KO* kout, int32_t* vout, hipStream_t st) {
+  rp_scatter8<K, KO, MODE, NT, IT><<<(unsigned)nchunks, NT, 0, st>>>(
+      keys, vals, n, crows, bits, shift, nchunks, offs, kout, vout);
+}
+
 template <typename K, typename KO, int MODE, int LOGB>
 int rp_launch(const K* keys, const int32_t* vals, int64_t n, int bits, int shift, KO* kout,
               int32_t* vout, int64_t* bucket_offsets, const RpWs& r, hipStream_t st,
-              int who) {
-  const int tile = RpShape<LOGB>::NT * RpShape<LOGB>::IT;
-  const int64_t crows = chunk_rows(bits, tile_of(bits));
+              int who, int shape) {
+  const int64_t crows = chunk_rows_of(bits);
   const int64_t nchunks = n > 0 ? (n + crows - 1) / crows : 1;
-  (void)tile;
   const int64_t entries = nchunks << bits;
   {
     KTimer _kt(kHistName[who], st);
     rp_hist<K, KO, MODE, LOGB><<<(unsigned)nchunks, RpShape<LOGB>::NT, 0, st>>>(
-      keys, n, crows, bits, shift, nchunks, r.hist);
+        keys, n, crows, bits, shift, nchunks, r.hist);
   }
   if (launch_check("radix_partition hist")) return 1;
   if (scan_i32_to_i64_excl(r.hist, r.offs, entries, nullptr, r.scan, st)) return 1;
   if (n > 0) {
-    {
     KTimer _kt(kScatterName[who], st);
-    rp_scatter<K, KO, MODE, LOGB><<<(unsigned)nchunks, RpShape<LOGB>::NT, 0, st>>>(
-        keys, vals, n, crows, bits, shift, nchunks, r.offs, kout, vout);
+    if (LOGB > 8) {
+      rp_scatter<K, KO, MODE, 12><<<(unsigned)nchunks, RpShape<12>::NT, 0, st>>>(
+          keys, vals, n, crows, bits, shift, nchunks, r.offs, kout, vout);
+    } else {
+      switch (shape < 0 || shape > 3 ? kDefaultShape8 : shape) {
+        case 0: launch8<K, KO, MODE, 256, 8>(keys, vals, n, crows, bits, shift, nchunks, r.offs,
+                                             kout, vout, st); break;
+        case 1: launch8<K, KO, MODE, 256, 16>(keys, vals, n, crows, bits, shift, nchunks,
+                                              r.offs, kout, vout, st); break;
+        case 2: launch8<K, KO, MODE, 512, 8>(keys, vals, n, crows, bits, shift, nchunks, r.offs,
+                                             kout, vout, st); break;
+        default: launch8<K, KO, MODE, 512, 16>(keys, vals, n, crows, bits, shift, nchunks,
+                                               r.offs, kout, vout, st); break;
+      }
+    }
   }
-    if (launch_check("radix_partition scatter")) return 1;
-  }
+  if (launch_check("radix_partition scatter")) return 1;
   rp_offsets<<<grid_for((1 << bits) + 1, 256), 256, 0, st>>>(r.offs, nchunks, 1 << bits, n,
                                                              bucket_offsets);
   return launch_check("radix_partition offsets");
@@ -289,26 +439,26 @@ int rp_launch(const K* keys, const int32_t* vals, int64_t n, int bits, int shift
 template <typename K>
 int rp_dispatch(const K* keys, const int32_t* vals, int64_t n, int bits, int shift, int mode,
                 void* kout, int32_t* vout, int64_t* bucket_offsets, const RpWs& r,
-                hipStream_t st, int who) {
+                hipStream_t st, int who, int shape) {
   if (mode == RP_HASH_OUT) {
     if (bits <= 8)
       return rp_launch<K, int64_t, RP_HASH_OUT, 8>(keys, vals, n, bits, shift, (int64_t*)kout,
-                                                   vout, bucket_offsets, r, st, who);
+                                                   vout, bucket_offsets, r, st, who, shape);
     return rp_launch<K, int64_t, RP_HASH_OUT, 12>(keys, vals, n, bits, shift, (int64_t*)kout,
-                                                  vout, bucket_offsets, r, st, who);
+                                                  vout, bucket_offsets, r, st, who, shape);
   }
   if (mode == RP_HASH_KEY) {
     if (bits <= 8)
       return rp_launch<K, K, RP_HASH_KEY, 8>(keys, vals, n, bits, shift, (K*)kout, vout,
-                                             bucket_offsets, r, st, who);
+                                             bucket_offsets, r, st, who, shape);
     return rp_launch<K, K, RP_HASH_KEY, 12>(keys, vals, n, bits, shift, (K*)kout, vout,
-                                            bucket_offsets, r, st, who);
+                                            bucket_offsets, r, st, who, shape);
   }
   if (bits <= 8)
     return rp_launch<K, K, RP_RAW, 8>(keys, vals, n, bits, shift, (K*)kout, vout,
-                                      bucket_offsets, r, st, who);
+                                      bucket_offsets, r, st, who, shape);
   return rp_launch<K, K, RP_RAW, 12>(keys, vals, n, bits, shift, (K*)kout, vout,
-                                     bucket_offsets, r, st, who);
+                                     bucket_offsets, r, st, who, shape);
 }
 
 }  // namespace
@@ -319,7 +469,7 @@ size_t radix_partition_ws(int64_t n, int bits) { return carve_rp(nullptr, n, bit
 int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
                     int shift, int mode, void* keys_out, int32_t* vals_out,
                     int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st,
-                    int who) {
+                    int who, int shape) {
   if (key_bytes != 8 && key_bytes != 4) return fail("radix_partition: keys are int64 or int32");
   if (bits < 0 || bits > 12) return fail("radix_partition: bits must be in [0, 12]");
   if (n < 0 || n >= (1LL << 32)) return fail("radix_partition: n out of range");
@@ -332,9 +482,9 @@ int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_
   if (ws_bytes < r.bytes) return fail("radix_partition: workspace too small");
   if (key_bytes == 8)
     return rp_dispatch<int64_t>((const int64_t*)keys, vals, n, bits, shift, mode, keys_out,
-                                vals_out, bucket_offsets, r, st, who);
+                                vals_out, bucket_offsets, r, st, who, shape);
   return rp_dispatch<int32_t>((const int32_t*)keys, vals, n, bits, shift, mode, keys_out,
-                              vals_out, bucket_offsets, r, st, who);
+                              vals_out, bucket_offsets, r, st, who, shape);
 }
 
 }  // namespace pf
@@ -347,14 +497,25 @@ size_t pf_radix_partition_workspace_bytes(int64_t n, int32_t bits) {
   return pf::radix_partition_ws(n, bits);
 }
 
+int pf_radix_partition_shape(const void* keys, int32_t key_bytes, const int32_t* vals,
+                             int64_t n, int32_t bits, int32_t shift, int32_t mode,
+                             void* keys_out, int32_t* vals_out, int64_t* bucket_offsets,
+                             void* workspace, size_t workspace_bytes, int32_t shape,
+                             void* stream) {
+  if (pf::radix_partition(keys, key_bytes, vals, n, bits, shift, mode, keys_out, vals_out,
+                          bucket_offsets, workspace, workspace_bytes, (hipStream_t)stream, 0,
+                          shape))
+    return 1;
+  return pf::ok();
+}
+
 int pf_radix_partition(const void* keys, int32_t key_bytes, const int32_t* vals, int64_t n,
                        int32_t bits, int32_t shift, int32_t mode, void* keys_out,
                        int32_t* vals_out, int64_t* bucket_offsets, void* workspace,
                        size_t workspace_bytes, void* stream) {
-  if (pf::radix_partition(keys, key_bytes, vals, n, bits, shift, mode, keys_out, vals_out,
-                          bucket_offsets, workspace, workspace_bytes, (hipStream_t)stream, 0))
-    return 1;
-  return pf::ok();
+  return pf_radix_partition_shape(keys, key_bytes, vals, n, bits, shift, mode, keys_out,
+                                  vals_out, bucket_offsets, workspace, workspace_bytes, -1,
+                                  stream);
 }
 
 }  // extern "C"

@@ -21,7 +21,7 @@ def radix_partition(keys: torch.Tensor, bits: int, vals: Optional[torch.Tensor] 
                     shift: int = 0, mode: int = HASH_KEY,
                     out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None,
                     bucket_offsets: Optional[torch.Tensor] = None,
-                    ws: Optional[torch.Tensor] = None):
+                    ws: Optional[torch.Tensor] = None, shape: int = -1):
     """(keys or hashes, row ids, bucket offsets[2^bits + 1]) of a stable
     partition of ``keys`` (int64 or int32, 1-D) by the digit ``mode``
     selects; ``vals`` are int32 row ids (None: the row index)."""
@@ -47,8 +47,8 @@ def radix_partition(keys: torch.Tensor, bits: int, vals: Optional[torch.Tensor] 
     if ws is None or ws.numel() < nbytes:
         ws = _lib.workspace(nbytes, dev)
     with torch.cuda.device(dev):
-        _lib.check(lib.pf_radix_partition(
+        _lib.check(lib.pf_radix_partition_shape(
             _lib.ptr(keys), keys.element_size(), _lib.ptr(vals), n, bits, shift, mode,
             _lib.ptr(out[0]), _lib.ptr(out[1]), _lib.ptr(bucket_offsets), _lib.ptr(ws),
-            ws.numel(), _lib.stream_of(dev)))
+            ws.numel(), shape, _lib.stream_of(dev)))
     return out[0], out[1], bucket_offsets

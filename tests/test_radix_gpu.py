@@ -74,3 +74,16 @@ def test_one_bucket_holds_every_row(bits):
 
 def test_large_partition_matches_oracle():
     _check(_keys(20_000_000, torch.int64, 11), 10, radix.HASH_KEY)
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("bits", [3, 8])
+def test_every_scatter_shape(shape, bits):
+    keys = _keys(700_001, torch.int64, 20 + shape)
+    vals = torch.randperm(keys.numel(), generator=torch.Generator().manual_seed(5)).to(torch.int32)
+    ko, vo, offs = radix.radix_partition(keys.cuda(), bits, vals.cuda(), mode=radix.HASH_OUT,
+                                         shape=shape)
+    wk, wv, counts = oracle.radix_partition(keys.numpy(), vals.numpy(), bits, 0, hashed=True)
+    assert np.array_equal(ko.cpu().numpy(), oracle.mix64(wk).view(np.int64))
+    assert np.array_equal(vo.cpu().numpy(), wv)
+    assert np.array_equal(offs.cpu().numpy(), np.concatenate(([0], np.cumsum(counts))))
