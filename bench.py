@@ -223,6 +223,18 @@ def main() -> None:
     assert int(torch.unique(op).numel()) == pairs
     log("checks passed: pair count, key equality, payload gather, unique probe rows")
 
+    # the achievable streaming rate on this box: a 5 GB device copy
+    src = probe[: min(np_, 640_000_000)]
+    dst = out_b[: src.numel()]
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    copy_gbps = 2 * src.numel() * 8 * 3 / (time.perf_counter() - tc) / 1e9
+    log(f"device copy: {copy_gbps:.0f} GB/s")
+
     kernels = {}
     for name, (tot_ms, launches) in prof.items():
         per_step = launches / args.steps
@@ -268,6 +280,7 @@ def main() -> None:
                 "GBps": e2e_bytes / (ms * 1e-3) / 1e9,
                 "frac": e2e_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS},
         "kernels": kernels,
+        "copy_GBps": copy_gbps,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -49,6 +49,7 @@ constexpr int SORT_CAP = 8192;             // multi: largest build bucket sorted
 constexpr int SORT_NT = 512;
 constexpr int PCH = 65536;                 // probe rows a work item
 constexpr int MAX_PAY = 2;
+constexpr int kOnePassBits = 8;            // wider partitions take two passes
 
 struct HjParams {
   const uint64_t* bh;     // build (hash, row) partitioned [nb]
@@ -264,13 +265,20 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
     for (int64_t r0 = it.ps; r0 < it.pe; r0 += HJ_ROUND) {
       int hidx[HJ_IT];
       int m[HJ_IT];
+      uint64_t hv[HJ_IT];
+      // every slot's load issued before any lookup waits on one
+#pragma unroll
+      for (int k = 0; k < HJ_IT; ++k) {
+        const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
+        hv[k] = r < it.pe ? p.ph[r] : 0;
+      }
 #pragma unroll
       for (int k = 0; k < HJ_IT; ++k) {
         const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
         hidx[k] = -1;
         m[k] = 0;
         if (r < it.pe) {
-          const uint64_t h = p.ph[r];
+          const uint64_t h = hv[k];
           uint32_t s = (uint32_t)h & (TS - 1);
           for (;;) {
             const uint32_t v = slot[s];
@@ -372,6 +380,27 @@ __global__ void hj_readback(const int64_t* __restrict__ a, const int64_t* __rest
   if (threadIdx.x == 2) store_host(&host[2], b ? b[0] : 0);
 }
 
+// Bucket b's first row in h sorted by its top `bits` bits (b = P: n).
+__global__ void hj_bucket_bounds(const uint64_t* __restrict__ h, int64_t n, int bits,
+                                 int64_t* __restrict__ boff) {
+  const int P = 1 << bits;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > P) return;
+  if (b == P) {
+    boff[b] = n;
+    return;
+  }
+  int64_t lo = 0, hi = n;  // first index with top bits >= b
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)(h[mid] >> (64 - bits)) < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  boff[b] = lo;
+}
+
 struct HjWs {
   uint64_t* bh;
   int32_t* brow;
@@ -382,6 +411,8 @@ struct HjWs {
   int64_t* item_off;
   int64_t* info;       // [0] items, [1] largest build bucket
   int64_t* total;
+  uint64_t* th;        // the first pass's output of a two-pass partition
+  int32_t* trow;
   int64_t* item_cnt;   // [max items] (counts, then scanned in place)
   int64_t* pay_part[MAX_PAY];  // build payloads in partition order
   void* part_ws;
@@ -403,6 +434,9 @@ HjWs carve_hj(void* ws, int64_t nb, int64_t np, int bits, bool multi) {
   w.prow = cv.take<int32_t>(np > 0 ? np : 1);
   w.poff = cv.take<int64_t>(P + 1);
   w.item_off = cv.take<int64_t>(P + 1);
+  const int64_t nt = bits > kOnePassBits ? (nb > np ? nb : np) : 1;
+  w.th = cv.take<uint64_t>(nt > 0 ? nt : 1);
+  w.trow = cv.take<int32_t>(nt > 0 ? nt : 1);
   w.info = cv.take<int64_t>(4);
   w.total = cv.take<int64_t>(2);
   const int64_t mi = max_items(np, bits);
@@ -413,6 +447,28 @@ HjWs carve_hj(void* ws, int64_t nb, int64_t np, int bits, bool multi) {
   w.scan = cv.take<char>(scan_i64_scratch(mi));
   w.bytes = cv.bytes();
   return w;
+}
+
+// One side as (mix64(key), row id) ordered by the top `bits` bits of the
+// hash, stable.  Up to kOnePassBits in one pass; above, two LSD passes of
+// about half the bits each (the lower digit first, then the upper one over
+// the hashes), so that every pass writes runs long enough to coalesce, then
+// the bucket bounds by binary search over the sorted hashes.
+int partition_side(const int64_t* keys, int64_t n, int bits, uint64_t* h, int32_t* row,
+                   int64_t* boff, uint64_t* th, int32_t* trow, void* pws, size_t pws_bytes,
+                   hipStream_t st, int who) {
+  if (bits <= kOnePassBits)
+    return radix_partition(keys, 8, nullptr, n, bits, 0, 1, h, row, boff, pws, pws_bytes, st,
+                           who, -1);
+  const int hi = bits / 2, lo = bits - hi;
+  if (radix_partition(keys, 8, nullptr, n, lo, hi, 1, th, trow, boff, pws, pws_bytes, st, who,
+                      -1))
+    return 1;
+  if (radix_partition(th, 8, trow, n, hi, 64 - hi, 2, h, row, boff, pws, pws_bytes, st, who,
+                      -1))
+    return 1;
+  hj_bucket_bounds<<<grid_for((1 << bits) + 1, 256), 256, 0, st>>>(h, n, bits, boff);
+  return launch_check("hash_join bounds");
 }
 
 HjParams params_of(const HjWs& w, int bits, int64_t n_items) {
@@ -458,11 +514,9 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
   const int cap = multi ? CAP_M : CAP_U;
   const size_t pws = radix_partition_ws(nb > np ? nb : np, bits);
   // both sides as (mix64(key), row) by the top `bits` bits of the hash
-  if (radix_partition(bkeys, 8, nullptr, nb, bits, 0, 1, w.bh, w.brow, w.boff, w.part_ws, pws,
-                      st, 1, -1))
+  if (partition_side(bkeys, nb, bits, w.bh, w.brow, w.boff, w.th, w.trow, w.part_ws, pws, st, 1))
     return 1;
-  if (radix_partition(pkeys, 8, nullptr, np, bits, 0, 1, w.ph, w.prow, w.poff, w.part_ws, pws,
-                      st, 2, -1))
+  if (partition_side(pkeys, np, bits, w.ph, w.prow, w.poff, w.th, w.trow, w.part_ws, pws, st, 2))
     return 1;
   hj_items<<<1, 1024, 0, st>>>(w.boff, w.poff, P, cap, w.item_off, w.info);
   hj_readback<<<1, 64, 0, st>>>(w.info, nullptr, (int64_t*)hdev);
