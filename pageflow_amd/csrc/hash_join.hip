@@ -41,13 +41,13 @@ constexpr int HJ_NT = 1024;
 constexpr int HJ_W = HJ_NT / kWave;
 constexpr int HJ_IT = 4;                   // probe rows a thread per round
 constexpr int HJ_ROUND = HJ_NT * HJ_IT;
-constexpr int CAP_U = 6144;                // unique: build rows an LDS chunk
+constexpr int CAP_U = 5120;                // unique: build rows an LDS chunk
 constexpr int TS_U = 8192;                 // unique: table slots
 constexpr int CAP_M = 4096;                // multi: build rows an LDS chunk
 constexpr int TS_M = 8192;
 constexpr int SORT_CAP = 8192;             // multi: largest build bucket sorted in LDS
 constexpr int SORT_NT = 512;
-constexpr int PCH = 65536;                 // probe rows a work item
+constexpr int PCH = 262144;                // probe rows a work item
 constexpr int MAX_PAY = 2;
 constexpr int kOnePassBits = 8;            // wider partitions take two passes
 
@@ -59,6 +59,7 @@ struct HjParams {
   const int32_t* prow;
   const int64_t* poff;    // [P + 1]
   const int64_t* item_off;  // [P + 1] each bucket's first work item
+  const int32_t* item_bucket;  // [items] each work item's bucket
   int P;
   int64_t n_items;
   int64_t* item_cnt;      // COUNT: pairs per item
@@ -77,7 +78,8 @@ struct HjParams {
 // into info[0..1].
 __global__ __launch_bounds__(1024) void hj_items(const int64_t* __restrict__ boff,
                                                  const int64_t* __restrict__ poff, int P,
-                                                 int cap, int64_t* __restrict__ item_off,
+                                                 int64_t max_items, int64_t* __restrict__ item_off,
+                                                 int32_t* __restrict__ item_bucket,
                                                  int64_t* __restrict__ info) {
   __shared__ int64_t tmp[1024 / kWave + 1];
   __shared__ int64_t mx[1024 / kWave];
@@ -99,7 +101,10 @@ __global__ __launch_bounds__(1024) void hj_items(const int64_t* __restrict__ bof
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int b = threadIdx.x * PER + q;
-    if (b < P) item_off[b] = ex;
+    if (b < P) {
+      item_off[b] = ex;
+      for (int64_t c = 0; c < v[q] && ex + c < max_items; ++c) item_bucket[ex + c] = b;
+    }
     ex += v[q];
   }
   // largest build bucket
@@ -174,17 +179,7 @@ struct Item {
 };
 
 __device__ __forceinline__ Item item_of(const HjParams& p, int64_t item) {
-  // bucket: last b with item_off[b] <= item
-  int lo = 0, hi = p.P;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (p.item_off[mid] <= item)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  // skip buckets without items (equal offsets): the last such index holds it
-  const int b = lo;
+  const int b = p.item_bucket[item];
   const int64_t c = item - p.item_off[b];
   Item it;
   it.bs = p.boff[b];
@@ -216,6 +211,9 @@ template <bool MULTI, int MODE>
 __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
   constexpr int CAP = Lds<MULTI>::CAP, TS = Lds<MULTI>::TS;
   constexpr bool W1 = MODE == 1;
+  // the count pass has no write-out bookkeeping: twice the rows in flight
+  constexpr int IT = W1 ? HJ_IT : 2 * HJ_IT;
+  constexpr int ROUND = HJ_NT * IT;
   __shared__ uint64_t keys[CAP];
   __shared__ uint32_t slot[TS];
   __shared__ int32_t rows[W1 ? CAP : 1];
@@ -262,18 +260,18 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
     }
     __syncthreads();
 
-    for (int64_t r0 = it.ps; r0 < it.pe; r0 += HJ_ROUND) {
-      int hidx[HJ_IT];
-      int m[HJ_IT];
-      uint64_t hv[HJ_IT];
+    for (int64_t r0 = it.ps; r0 < it.pe; r0 += ROUND) {
+      int hidx[IT];
+      int m[IT];
+      uint64_t hv[IT];
       // every slot's load issued before any lookup waits on one
 #pragma unroll
-      for (int k = 0; k < HJ_IT; ++k) {
+      for (int k = 0; k < IT; ++k) {
         const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
         hv[k] = r < it.pe ? p.ph[r] : 0;
       }
 #pragma unroll
-      for (int k = 0; k < HJ_IT; ++k) {
+      for (int k = 0; k < IT; ++k) {
         const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
         hidx[k] = -1;
         m[k] = 0;
@@ -303,7 +301,7 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
       }
       if (!W1) {
 #pragma unroll
-        for (int k = 0; k < HJ_IT; ++k) count += m[k];
+        for (int k = 0; k < IT; ++k) count += m[k];
         continue;
       }
       // output offsets in row order: within a wave by lane, across (round
@@ -409,6 +407,7 @@ struct HjWs {
   int32_t* prow;
   int64_t* poff;
   int64_t* item_off;
+  int32_t* item_bucket;
   int64_t* info;       // [0] items, [1] largest build bucket
   int64_t* total;
   uint64_t* th;        // the first pass's output of a two-pass partition
@@ -434,6 +433,7 @@ HjWs carve_hj(void* ws, int64_t nb, int64_t np, int bits, bool multi) {
   w.prow = cv.take<int32_t>(np > 0 ? np : 1);
   w.poff = cv.take<int64_t>(P + 1);
   w.item_off = cv.take<int64_t>(P + 1);
+  w.item_bucket = cv.take<int32_t>(max_items(np, bits));
   const int64_t nt = bits > kOnePassBits ? (nb > np ? nb : np) : 1;
   w.th = cv.take<uint64_t>(nt > 0 ? nt : 1);
   w.trow = cv.take<int32_t>(nt > 0 ? nt : 1);
@@ -480,6 +480,7 @@ HjParams params_of(const HjWs& w, int bits, int64_t n_items) {
   p.prow = w.prow;
   p.poff = w.poff;
   p.item_off = w.item_off;
+  p.item_bucket = w.item_bucket;
   p.P = 1 << bits;
   p.n_items = n_items;
   p.item_cnt = w.item_cnt;
@@ -518,7 +519,8 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
     return 1;
   if (partition_side(pkeys, np, bits, w.ph, w.prow, w.poff, w.th, w.trow, w.part_ws, pws, st, 2))
     return 1;
-  hj_items<<<1, 1024, 0, st>>>(w.boff, w.poff, P, cap, w.item_off, w.info);
+  hj_items<<<1, 1024, 0, st>>>(w.boff, w.poff, P, max_items(np, bits), w.item_off,
+                               w.item_bucket, w.info);
   hj_readback<<<1, 64, 0, st>>>(w.info, nullptr, (int64_t*)hdev);
   if (launch_check("hash_join items")) return 1;
   if (wait_stream(st, "hash_join")) return 1;
