@@ -186,6 +186,53 @@ int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int6
                        int64_t* out_build, int64_t* const* pay_out, void* workspace,
                        size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------ gather --
+ * Replaces the payload gather of operators/hashjoin.py:359-377
+ * (torch.index_select per output column) and conversion.py:39-50
+ * (index_with_null): out[i] = src[idx[i]] for rows of row_elems elements of
+ * elem_bytes (1, 2, 4, 8); where idx[i] is the null sentinel (-64), or src
+ * has no row, every element of the row is -64.  An index outside [0, nsrc)
+ * also writes the null row and sets *bad (optional, device).  No wait. */
+int pf_gather_rows(const void* src, int64_t nsrc, int32_t elem_bytes, int64_t row_elems,
+                   const int64_t* idx, int64_t n, void* out, int64_t* bad, void* stream);
+
+/* The join-type tails (operators/hashjoin.py:293-352, the bool scatter of
+ * matched rows and its nonzero): pf_mark_rows sets flags[idx[i]] = value;
+ * pf_select_rows writes the rows whose flag equals `want` (0 or nonzero),
+ * ascending, into out (int64[n]) and their count into host_out[0] (pinned);
+ * one wait. */
+int pf_mark_rows(const int64_t* idx, int64_t n, uint8_t* flags, int64_t nflags, int32_t value,
+                 void* stream);
+size_t pf_select_rows_workspace_bytes(int64_t n);
+int pf_select_rows(const uint8_t* flags, int64_t n, int32_t want, int64_t* out,
+                   int64_t* host_out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------- group-by --
+ * Replaces the SUM/COUNT evaluation of operators/operator_utils.py:342-399
+ * under KeyedAggregate.pass1 (operators/aggregate.py:1511-1563), in exact
+ * int64: sum[g] += val[i], cnt[g] += (val[i] != -64) for gid[i] = g in
+ * [0, G); other gids are skipped and counted into *bad (optional, device).
+ * accumulate = 0 zeroes sum/cnt first.  No wait. */
+int pf_group_sum_count(const int32_t* gid, const int64_t* val, int64_t n, int32_t G,
+                       int64_t* sum, int64_t* cnt, int32_t accumulate, int64_t* bad,
+                       void* stream);
+
+/* --------------------------------------------------------- serving --
+ * No counterpart in the reference (the serving tier): a resident dimension
+ * table, int32 key -> int32 group in [0, G), G <= 1024, as `slots` packed
+ * uint64 (pf_dim_table_slots: a power of two >= 2n); err (device int64[2])
+ * counts groups out of range and repeated keys.  pf_fact_join_groupby joins
+ * n fact rows (int32 foreign key, int64 amount) to the table and writes
+ * SUM(amount) and COUNT(amount) (values != -64) per group into sum/cnt
+ * (int64[G], zeroed first); fact rows without a match are dropped.  No
+ * wait. */
+int64_t pf_dim_table_slots(int64_t n_dim);
+int pf_dim_table_build(const int32_t* key, const int32_t* grp, int64_t n, int32_t G,
+                       uint64_t* table, int64_t slots, int64_t* err, void* stream);
+int pf_fact_join_groupby(const int32_t* fk, const int64_t* amt, int64_t n,
+                         const uint64_t* table, int64_t slots, int32_t G, int64_t* sum,
+                         int64_t* cnt, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

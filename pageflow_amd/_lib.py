@@ -47,6 +47,14 @@ _SIGNATURES = {
     "pf_radix_partition": (ctypes.c_int, [P, I32, P, I64, I32, I32, I32, P, P, P, P, SZ, P]),
     "pf_radix_partition_shape": (ctypes.c_int, [P, I32, P, I64, I32, I32, I32, P, P, P, P, SZ,
                                                 I32, P]),
+    "pf_gather_rows": (ctypes.c_int, [P, I64, I32, I64, P, I64, P, P, P]),
+    "pf_mark_rows": (ctypes.c_int, [P, I64, P, I64, I32, P]),
+    "pf_select_rows_workspace_bytes": (SZ, [I64]),
+    "pf_select_rows": (ctypes.c_int, [P, I64, I32, P, P, P, SZ, P]),
+    "pf_group_sum_count": (ctypes.c_int, [P, P, I64, I32, P, P, I32, P, P]),
+    "pf_dim_table_slots": (I64, [I64]),
+    "pf_dim_table_build": (ctypes.c_int, [P, P, I64, I32, P, I64, P, P]),
+    "pf_fact_join_groupby": (ctypes.c_int, [P, P, I64, P, I64, I32, P, P, P]),
     "pf_hash_join_workspace_bytes": (SZ, [I64, I64, I32, I32]),
     "pf_hash_join_prepare": (ctypes.c_int, [P, I64, P, I64, I32, I32, P, P, SZ, P]),
     "pf_hash_join_write": (ctypes.c_int, [I64, I64, I32, I32, I64, PP, I32, P, P, PP, P, SZ,

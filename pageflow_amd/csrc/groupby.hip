@@ -1,0 +1,272 @@
+// Exact int64 SUM and COUNT per group, and the serving tier's fused
+// fact->dimension join + group-by.
+//
+// The reference's group-by (operators/aggregate.py:1511-1563 pass 1 ->
+// operators/operator_utils.py:342-399) sums through float64 bincount below
+// 2^16 rows a group and a two-step int64 scatter_reduce above; its result
+// equals the exact int64 sum while every partial stays below 2^53.  Here the
+// sums are exact int64 throughout (integer atomics are order-free, so the
+// result is bit-identical whatever the schedule), and COUNT counts the
+// values that are not the null sentinel (-64).
+//
+// gfx950 design: each workgroup accumulates into LDS copies of the group
+// table (R replicas, lane % R picks one, so lanes of a wave that share a
+// group hit different banks and addresses), reduces the replicas and adds
+// its partials to the output with one global int64 atomic per group.
+//
+// Serving: a dimension table (int32 key -> int32 group in [0, G)) is an
+// open-addressing table of packed 64-bit slots (key << 32 | group, empty =
+// all ones, which no valid slot equals because a group is never -1) built
+// once and kept resident; each request streams its fact rows (int32 foreign
+// key, int64 amount) through one kernel that probes the table and
+// aggregates: 12 bytes a fact row, the algorithmic minimum.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "pf_common.hpp"
+#include "../../include/pageflow_join.h"
+
+namespace pf {
+namespace {
+
+constexpr int GB_NT = 512;
+constexpr int GB_LDS_GROUPS = 1024;   // groups accumulated in LDS (with replicas)
+constexpr uint64_t kEmptySlot = ~0ULL;
+
+// replicas of the LDS table for G groups: as many as fit 32 KiB of int64 pairs
+__host__ __device__ inline int replicas_for(int G) {
+  int r = 32;
+  while (r > 1 && (int64_t)r * G * 16 > 32 * 1024) r >>= 1;
+  return r;
+}
+
+__device__ __forceinline__ void lds_add(int64_t* s, int64_t v) {
+  atomicAdd((unsigned long long*)s, (unsigned long long)v);
+}
+
+__device__ __forceinline__ void global_add(int64_t* s, int64_t v) {
+  atomicAdd((unsigned long long*)s, (unsigned long long)v);
+}
+
+// SUM/COUNT of val by gid.  Rows with gid outside [0, G) are skipped (and
+// counted into *bad when bad is not null).
+template <bool LDS>
+__global__ __launch_bounds__(GB_NT) void k_group_sum_count(const int32_t* __restrict__ gid,
+                                                           const int64_t* __restrict__ val,
+                                                           int64_t n, int G, int R,
+                                                           int64_t* sum, int64_t* cnt,
+                                                           int64_t* bad) {
+  __shared__ int64_t acc[LDS ? 32 * 1024 / 8 : 1];  // [R][G][2]
+  const int tid = threadIdx.x;
+  if (LDS) {
+    for (int x = tid; x < R * G * 2; x += GB_NT) acc[x] = 0;
+    __syncthreads();
+  }
+  const int rep = (int)(lane_id() % R);
+  int64_t nbad = 0;
+  const int64_t stride = (int64_t)gridDim.x * GB_NT;
+  for (int64_t i = blockIdx.x * (int64_t)GB_NT + tid; i < n; i += stride) {
+    const int g = gid[i];
+    const int64_t v = val[i];
+    if (g < 0 || g >= G) {
+      ++nbad;
+      continue;
+    }
+    if (LDS) {
+      int64_t* a = acc + ((int64_t)rep * G + g) * 2;
+      lds_add(a, v);
+      if (v != kNull) lds_add(a + 1, 1);
+    } else {
+      global_add(&sum[g], v);
+      if (v != kNull) global_add(&cnt[g], 1);
+    }
+  }
+  if (bad) {
+    nbad = wave_sum(nbad);
+    if (lane_id() == 0 && nbad) global_add(bad, nbad);
+  }
+  if (LDS) {
+    __syncthreads();
+    for (int g = tid; g < G; g += GB_NT) {
+      int64_t s = 0, c = 0;
+      for (int r = 0; r < R; ++r) {
+        s += acc[((int64_t)r * G + g) * 2];
+        c += acc[((int64_t)r * G + g) * 2 + 1];
+      }
+      if (s) global_add(&sum[g], s);
+      if (c) global_add(&cnt[g], c);
+    }
+  }
+}
+
+// ------------------------------------------------------------ serving --
+
+__device__ __forceinline__ uint32_t dim_slot(int32_t key, uint32_t mask) {
+  return (uint32_t)mix64((uint64_t)(int64_t)key) & mask;
+}
+
+// Inserts each dimension row's (key, group).  A repeated key keeps its
+// first insertion (dimension keys are unique; a repeat sets *dup).
+__global__ void k_dim_build(const int32_t* __restrict__ key, const int32_t* __restrict__ grp,
+                            int64_t n, uint64_t* table, uint32_t mask, int G, int64_t* err) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t k = key[i], g = grp[i];
+  if (g < 0 || g >= G) {
+    atomicAdd((unsigned long long*)&err[0], 1ULL);
+    return;
+  }
+  const uint64_t v = ((uint64_t)(uint32_t)k << 32) | (uint32_t)g;
+  uint32_t s = dim_slot(k, mask);
+  for (;;) {
+    uint64_t prev = table[s];
+    if (prev == kEmptySlot)
+      prev = atomicCAS((unsigned long long*)&table[s], (unsigned long long)kEmptySlot,
+                       (unsigned long long)v);
+    if (prev == kEmptySlot) return;
+    if ((int32_t)(prev >> 32) == k) {
+      atomicAdd((unsigned long long*)&err[1], 1ULL);
+      return;
+    }
+    s = (s + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ int dim_lookup(const uint64_t* __restrict__ table, uint32_t mask,
+                                          int32_t k) {
+  uint32_t s = dim_slot(k, mask);
+  for (;;) {
+    const uint64_t t = table[s];
+    if (t == kEmptySlot) return -1;
+    if ((int32_t)(t >> 32) == k) return (int)(uint32_t)t;
+    s = (s + 1) & mask;
+  }
+}
+
+// SUM(amount), COUNT(amount) by the dimension group of each fact row's key
+// (inner join: rows without a dimension match are dropped).
+__global__ __launch_bounds__(GB_NT) void k_fact_join_groupby(
+    const int32_t* __restrict__ fk, const int64_t* __restrict__ amt, int64_t n,
+    const uint64_t* __restrict__ table, uint32_t mask, int G, int R, int64_t* sum,
+    int64_t* cnt) {
+  __shared__ int64_t acc[32 * 1024 / 8];
+  const int tid = threadIdx.x;
+  for (int x = tid; x < R * G * 2; x += GB_NT) acc[x] = 0;
+  __syncthreads();
+  const int rep = (int)(lane_id() % R);
+  constexpr int U = 4;  // rows in flight a thread
+  const int64_t stride = (int64_t)gridDim.x * GB_NT;
+  for (int64_t i0 = blockIdx.x * (int64_t)GB_NT + tid; i0 < n; i0 += stride * U) {
+    int32_t k[U];
+    int64_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + u * stride;
+      k[u] = i < n ? fk[i] : 0;
+      v[u] = i < n ? amt[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (i0 + u * stride >= n) break;
+      const int g = dim_lookup(table, mask, k[u]);
+      if (g < 0) continue;
+      int64_t* a = acc + ((int64_t)rep * G + g) * 2;
+      lds_add(a, v[u]);
+      if (v[u] != kNull) lds_add(a + 1, 1);
+    }
+  }
+  __syncthreads();
+  for (int g = tid; g < G; g += GB_NT) {
+    int64_t s = 0, c = 0;
+    for (int r = 0; r < R; ++r) {
+      s += acc[((int64_t)r * G + g) * 2];
+      c += acc[((int64_t)r * G + g) * 2 + 1];
+    }
+    if (s) global_add(&sum[g], s);
+    if (c) global_add(&cnt[g], c);
+  }
+}
+
+unsigned groupby_grid(int64_t n) {
+  int64_t g = (n + GB_NT * 8 - 1) / (GB_NT * 8);
+  if (g < 1) g = 1;
+  if (g > 2048) g = 2048;
+  return (unsigned)g;
+}
+
+}  // namespace
+}  // namespace pf
+
+extern "C" {
+
+int pf_group_sum_count(const int32_t* gid, const int64_t* val, int64_t n, int32_t G,
+                       int64_t* sum, int64_t* cnt, int32_t accumulate, int64_t* bad,
+                       void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (n < 0 || G < 1) return fail("group_sum_count: need n >= 0 and G >= 1");
+  if (!accumulate) {
+    if (fill_i64(sum, G, 0, st) || fill_i64(cnt, G, 0, st)) return 1;
+  }
+  if (n == 0) return ok();
+  const bool lds = G <= GB_LDS_GROUPS;
+  const int R = lds ? replicas_for(G) : 1;
+  {
+    KTimer _kt("group_sum_count", st);
+    if (lds)
+      k_group_sum_count<true><<<groupby_grid(n), GB_NT, 0, st>>>(gid, val, n, G, R, sum, cnt,
+                                                                 bad);
+    else
+      k_group_sum_count<false><<<groupby_grid(n), GB_NT, 0, st>>>(gid, val, n, G, R, sum, cnt,
+                                                                  bad);
+  }
+  if (launch_check("group_sum_count")) return 1;
+  return ok();
+}
+
+int64_t pf_dim_table_slots(int64_t n_dim) {
+  int64_t cap = 16;
+  while (cap < 2 * n_dim) cap <<= 1;
+  return cap;
+}
+
+int pf_dim_table_build(const int32_t* key, const int32_t* grp, int64_t n, int32_t G,
+                       uint64_t* table, int64_t slots, int64_t* err, void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (n < 0 || G < 1 || G > GB_LDS_GROUPS)
+    return fail("dim_table_build: need n >= 0 and 1 <= G <= 1024");
+  if (slots < 2 * n || (slots & (slots - 1)) || slots > (1LL << 32))
+    return fail("dim_table_build: slots must be a power of two >= 2n");
+  if (fill_i64((int64_t*)table, slots, -1, st)) return 1;
+  if (fill_i64(err, 2, 0, st)) return 1;
+  if (n > 0) {
+    KTimer _kt("dim_table_build", st);
+    k_dim_build<<<grid_for(n, 256), 256, 0, st>>>(key, grp, n, table, (uint32_t)(slots - 1), G,
+                                                  err);
+  }
+  if (launch_check("dim_table_build")) return 1;
+  return ok();
+}
+
+int pf_fact_join_groupby(const int32_t* fk, const int64_t* amt, int64_t n,
+                         const uint64_t* table, int64_t slots, int32_t G, int64_t* sum,
+                         int64_t* cnt, void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (n < 0 || G < 1 || G > GB_LDS_GROUPS)
+    return fail("fact_join_groupby: need n >= 0 and 1 <= G <= 1024");
+  if (slots < 16 || (slots & (slots - 1))) return fail("fact_join_groupby: bad table size");
+  if (fill_i64(sum, G, 0, st) || fill_i64(cnt, G, 0, st)) return 1;
+  if (n > 0) {
+    KTimer _kt("fact_join_groupby", st);
+    k_fact_join_groupby<<<groupby_grid(n), GB_NT, 0, st>>>(fk, amt, n, table,
+                                                           (uint32_t)(slots - 1), G,
+                                                           replicas_for(G), sum, cnt);
+  }
+  if (launch_check("fact_join_groupby")) return 1;
+  return ok();
+}
+
+}  // extern "C"

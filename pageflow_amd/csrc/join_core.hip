@@ -222,6 +222,41 @@ __global__ __launch_bounds__(CT_NT) void k_rows_compact(const uint8_t* __restric
     if (keep[j]) out[pos[j]] = base + j * CT_NT + threadIdx.x;
 }
 
+// rows whose flag equals `want`: per-tile counts, then compaction.
+__global__ __launch_bounds__(CT_NT) void k_flag_count(const uint8_t* __restrict__ flags,
+                                                      int64_t n, bool want,
+                                                      int32_t* __restrict__ tilecnt) {
+  __shared__ int red[CT_W];
+  const int64_t base = (int64_t)blockIdx.x * CT_TILE;
+  int c = 0;
+#pragma unroll
+  for (int j = 0; j < CT_IT; ++j) {
+    const int64_t i = base + j * CT_NT + threadIdx.x;
+    c += (i < n && ((flags[i] != 0) == want)) ? 1 : 0;
+  }
+  c = block_sum_ct(c, red);
+  if (threadIdx.x == 0) tilecnt[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(CT_NT) void k_flag_compact(const uint8_t* __restrict__ flags,
+                                                        int64_t n, bool want,
+                                                        const int64_t* __restrict__ offs,
+                                                        int64_t* __restrict__ out) {
+  __shared__ int cnt[kWave];
+  const int64_t base = (int64_t)blockIdx.x * CT_TILE;
+  bool keep[CT_IT];
+#pragma unroll
+  for (int j = 0; j < CT_IT; ++j) {
+    const int64_t i = base + j * CT_NT + threadIdx.x;
+    keep[j] = i < n && ((flags[i] != 0) == want);
+  }
+  int64_t pos[CT_IT];
+  tile_positions(keep, offs[blockIdx.x], pos, cnt);
+#pragma unroll
+  for (int j = 0; j < CT_IT; ++j)
+    if (keep[j]) out[pos[j]] = base + j * CT_NT + threadIdx.x;
+}
+
 __global__ void k_readback(const int64_t* __restrict__ dev, int n, int64_t* host) {
   if ((int)threadIdx.x < n) store_host(&host[threadIdx.x], dev[threadIdx.x]);
 }
@@ -541,6 +576,43 @@ int pf_semi_anti_mask(const int64_t* bk, int64_t nb, const int64_t* pk, int64_t 
     k_readback<<<1, 64, 0, st>>>(s.dev + 1, 1, bad_out);
     if (launch_check("semi_anti_mask")) return 1;
   }
+  return ok();
+}
+
+size_t pf_select_rows_workspace_bytes(int64_t n) {
+  pf::Carve cv(nullptr);
+  const int64_t tiles = (n + pf::CT_TILE - 1) / pf::CT_TILE;
+  cv.take<int32_t>(tiles > 0 ? tiles : 1);
+  cv.take<int64_t>(tiles > 0 ? tiles : 1);
+  cv.take<int64_t>(2);
+  cv.take<char>(pf::scan_i32_scratch(tiles));
+  return cv.bytes();
+}
+
+int pf_select_rows(const uint8_t* flags, int64_t n, int32_t want, int64_t* out,
+                   int64_t* host_out, void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (n < 0) return fail("select_rows: negative rows");
+  if (workspace_bytes < pf_select_rows_workspace_bytes(n))
+    return fail("select_rows: workspace too small");
+  void* hdev = nullptr;
+  if (host_alias(host_out, &hdev)) return 1;
+  Carve cv(workspace);
+  const int64_t tiles = (n + CT_TILE - 1) / CT_TILE;
+  int32_t* tilecnt = cv.take<int32_t>(tiles > 0 ? tiles : 1);
+  int64_t* offs = cv.take<int64_t>(tiles > 0 ? tiles : 1);
+  int64_t* total = cv.take<int64_t>(2);
+  void* scan = cv.take<char>(scan_i32_scratch(tiles));
+  if (n > 0) {
+    k_flag_count<<<(unsigned)tiles, CT_NT, 0, st>>>(flags, n, want != 0, tilecnt);
+    if (launch_check("select_rows")) return 1;
+  }
+  if (scan_i32_to_i64_excl(tilecnt, offs, tiles, total, scan, st)) return 1;
+  k_readback<<<1, 64, 0, st>>>(total, 1, (int64_t*)hdev);
+  if (n > 0) k_flag_compact<<<(unsigned)tiles, CT_NT, 0, st>>>(flags, n, want != 0, offs, out);
+  if (launch_check("select_rows")) return 1;
+  if (wait_stream(st, "select_rows")) return 1;
   return ok();
 }
 

@@ -1,0 +1,404 @@
+"""The join core's dispatch over the gfx950 kernels: drop-in for
+``join_kernel_new`` and its helpers in the reference's
+``operators/hashjoin.py`` (:36-503).
+
+A core call joins one tick's two sides (dicts of column id -> int64 tensor)
+on ``left_key`` = ``right_key``, with residual conditions evaluated on the
+pairs, and returns the output columns (``join_output_keys``, then the two
+global indexes when asked).  The path taken, in order:
+
+1. the unique-build hash core (``join_core.unique_hash_pairs``) for an inner
+   join whose selected equality keys a uniqueness proof covers, inside the
+   build/probe ratio and scratch guards (hashjoin.py:87-163);
+2. for semi and anti joins, the hash set with the optional min/max summary
+   (``join_core.semi_anti``, hashjoin.py:215-232);
+3. otherwise every pair: here the partitioned multi-match LDS-table join
+   (``pageflow_amd.join.hash_join``), which stands in for the reference's
+   sort/search core (hashjoin.py:380-441) and yields the same pair multiset
+   (the reference's own tests compare these pairs as multisets); when a
+   build bucket is too large for its LDS sort, the sort/search program
+   itself (a device sort, two searches and the expand kernel).
+
+The join-type tails (right-semi / right-anti / right-outer) and the payload
+gather (with the null row for right-outer) run on the library's kernels.
+The residual evaluator covers comparisons (=, <>, !=, <, <=, >, >=) of
+column references and integer literals under AND, OR and NOT; the
+reference's general expression evaluator is outside this path.
+"""
+from __future__ import annotations
+
+from typing import Literal, Optional
+
+import torch
+
+from .. import join as _join
+from .. import ops
+from ..layout import NULL
+from . import join_core
+
+LEFT, RIGHT = 0, 1
+
+# The reference's join-core gate (hashjoin.py:36-40): tests switch it.
+_JOIN_CORE_MODES = {"legacy", "unique_hash"}
+_join_core_mode = "unique_hash"
+# The unique-build core's guards (hashjoin.py:41-55).
+_UNIQUE_HASH_MAX_BUILD_PROBE_RATIO = 8
+_UNIQUE_HASH_SCRATCH_LIMIT_BYTES = 2 * 1024 ** 3
+# Pairs a residual evaluation takes (operators/operator_utils.py:19).
+INT_ROWS_MAX = (1 << 31) - 2
+
+JOIN_TYPES = {
+    'Inner': 'inner',
+    'LeftSemi': 'right-semi',
+    'LeftOuter': 'right-outer',
+    'LeftAnti': 'right-anti',
+}
+
+
+def set_join_core_mode(mode: str) -> None:
+    global _join_core_mode
+    if mode not in _JOIN_CORE_MODES:
+        raise ValueError(f"join core mode must be one of {sorted(_JOIN_CORE_MODES)}, got {mode!r}")
+    _join_core_mode = mode
+
+
+def _join_core_fallback(reason: str):
+    """None: the caller takes the pair path (``reason`` names the guard)."""
+    return None
+
+
+# ------------------------------------------------------------ expressions --
+
+def normalize_expr_tree(tree):
+    """Unwraps single-item list wrappers of a parsed condition."""
+    while isinstance(tree, list) and len(tree) == 1:
+        tree = tree[0]
+    return tree
+
+
+def column_ref(token: str) -> Optional[int]:
+    """The column id of ``name#id`` / ``#id``, else None."""
+    if not isinstance(token, str) or '#' not in token:
+        return None
+    try:
+        return int(token.rsplit('#', 1)[1])
+    except ValueError:
+        return None
+
+
+def _proof_is_covered(unique_key_sets, selected_ids) -> bool:
+    selected = set(selected_ids)
+    return any(set(k) <= selected for k in (unique_key_sets or []))
+
+
+def _valid_hash_key(t: torch.Tensor) -> bool:
+    return t.is_cuda and t.dtype == torch.int64 and t.dim() == 1 and t.is_contiguous()
+
+
+def _try_unique_hash_pairs(tensors_left, tensors_right, left_key, right_key, join_type,
+                           equality_keys, unique_key_sets, scratch_limit_bytes):
+    """(left_idx, right_idx, hashed_pairs) from the unique-build core, or
+    None for the pair path."""
+    if _join_core_mode == "legacy":
+        return _join_core_fallback("gate_legacy")
+    if join_type != "inner":
+        return _join_core_fallback("join_type")
+    if equality_keys is None or unique_key_sets is None:
+        return _join_core_fallback("missing_proof_metadata")
+    left_eq, right_eq = equality_keys
+    if len(left_eq) != len(right_eq) or not left_eq:
+        return _join_core_fallback("unaligned_equality_keys")
+    selected = []
+    for lid, rid in zip(left_eq, right_eq):
+        if lid in tensors_left and rid in tensors_right and \
+                _valid_hash_key(tensors_left[lid]) and _valid_hash_key(tensors_right[rid]):
+            selected.append((lid, rid))
+    if (left_key, right_key) not in selected:
+        return _join_core_fallback("primary_key_layout_or_dtype")
+    if len(selected) not in (1, 2):
+        return _join_core_fallback("unsupported_key_count")
+    left_ids = [p[0] for p in selected]
+    right_ids = [p[1] for p in selected]
+    can_left = _proof_is_covered(unique_key_sets[0], left_ids)
+    can_right = _proof_is_covered(unique_key_sets[1], right_ids)
+    if not can_left and not can_right:
+        return _join_core_fallback("selected_keys_not_proven_unique")
+    left_rows = tensors_left[left_ids[0]].numel()
+    right_rows = tensors_right[right_ids[0]].numel()
+    # both proven: build the smaller side, ties on the left
+    build_left = can_left and (not can_right or left_rows <= right_rows)
+    if build_left:
+        build_ids, probe_ids, bt, pt = left_ids, right_ids, tensors_left, tensors_right
+    else:
+        build_ids, probe_ids, bt, pt = right_ids, left_ids, tensors_right, tensors_left
+    n_build, n_probe = bt[build_ids[0]].numel(), pt[probe_ids[0]].numel()
+    if n_probe > 0 and n_build > _UNIQUE_HASH_MAX_BUILD_PROBE_RATIO * n_probe:
+        return _join_core_fallback("build_probe_ratio_guard")
+    if scratch_limit_bytes is not None and \
+            join_core.unique_hash_peak_bytes(n_build, n_probe) > scratch_limit_bytes:
+        return _join_core_fallback("scratch_guard")
+    build_idx, probe_idx = join_core.unique_hash_pairs([bt[k] for k in build_ids],
+                                                       [pt[k] for k in probe_ids])
+    hashed = frozenset(selected)
+    if build_left:
+        return build_idx, probe_idx, hashed
+    return probe_idx, build_idx, hashed
+
+
+_NEGATED_OP = {'=': '<>', '<': '>=', '<=': '>', '>': '<=', '>=': '<'}
+_FLIPPED_OP = {'<>': '<>', '<': '>', '<=': '>=', '>': '<', '>=': '<='}
+
+
+def _summary_column(token):
+    token = normalize_expr_tree(token)
+    if not isinstance(token, str) or ' ' in token:
+        return None
+    return column_ref(token)
+
+
+def _residual_summary(condition, tensors_left, tensors_right):
+    """(build column, probe column, op) when ``condition`` compares one build
+    (left) int64 column with one probe (right) int64 column by an operator
+    the per-key summary answers; else None (hashjoin.py:181-212)."""
+    tree = normalize_expr_tree(condition)
+    negated = False
+    while isinstance(tree, list) and len(tree) == 2 and tree[0] == 'NOT':
+        negated = not negated
+        tree = normalize_expr_tree(tree[1])
+    if not (isinstance(tree, list) and len(tree) == 3 and isinstance(tree[1], str)):
+        return None
+    first, op, second = _summary_column(tree[0]), tree[1], _summary_column(tree[2])
+    if negated:
+        op = _NEGATED_OP.get(op)
+    if any(c in tensors_left and c in tensors_right for c in (first, second)):
+        return None
+    if first in tensors_left and second in tensors_right:
+        build_id, probe_id = first, second
+    elif first in tensors_right and second in tensors_left:
+        build_id, probe_id, op = second, first, _FLIPPED_OP.get(op)
+    else:
+        return None
+    if op not in join_core.SUMMARY_OPS:
+        return None
+    build, probe = tensors_left[build_id], tensors_right[probe_id]
+    if not (_valid_hash_key(build) and _valid_hash_key(probe)):
+        return None
+    return build, probe, op
+
+
+def _try_hash_semi_anti(tensors_left, tensors_right, left_key, right_key, condition_list,
+                        anti):
+    build, probe = tensors_left[left_key], tensors_right[right_key]
+    if not (_valid_hash_key(build) and _valid_hash_key(probe)):
+        return None
+    summary = None
+    if condition_list:
+        if len(condition_list) != 1:
+            return None
+        summary = _residual_summary(condition_list[0], tensors_left, tensors_right)
+        if summary is None:
+            return None
+    return join_core.semi_anti(build, probe, anti, summary)
+
+
+_CMP = {'=': torch.eq, '==': torch.eq, '<>': torch.ne, '!=': torch.ne, '<': torch.lt,
+        '<=': torch.le, '>': torch.gt, '>=': torch.ge}
+
+
+def evaluate_join(tensors_left, tensors_right, tree, leftidx, rightidx) -> torch.Tensor:
+    """A residual condition on pairs: a column id the left side holds is
+    read on the left rows, otherwise on the right rows."""
+    tree = normalize_expr_tree(tree)
+
+    def value(tok):
+        tok = normalize_expr_tree(tok)
+        cid = column_ref(tok) if isinstance(tok, str) else None
+        if cid is not None:
+            if cid in tensors_left:
+                return ops.gather_rows(tensors_left[cid], leftidx)
+            return ops.gather_rows(tensors_right[cid], rightidx)
+        if isinstance(tok, str):
+            try:
+                return int(tok)
+            except ValueError:
+                pass
+        if isinstance(tok, int):
+            return tok
+        raise NotImplementedError(f"residual operand {tok!r} is outside the join path's "
+                                  "evaluator")
+
+    if isinstance(tree, list) and len(tree) == 2 and tree[0] == 'NOT':
+        return ~evaluate_join(tensors_left, tensors_right, tree[1], leftidx, rightidx)
+    if isinstance(tree, list) and len(tree) == 3 and tree[1] in ('AND', 'OR'):
+        a = evaluate_join(tensors_left, tensors_right, tree[0], leftidx, rightidx)
+        b = evaluate_join(tensors_left, tensors_right, tree[2], leftidx, rightidx)
+        return a & b if tree[1] == 'AND' else a | b
+    if isinstance(tree, list) and len(tree) == 3 and tree[1] in _CMP:
+        a, b = value(tree[0]), value(tree[2])
+        res = _CMP[tree[1]](torch.as_tensor(a, device=leftidx.device)
+                            if isinstance(a, int) else a, b)
+        if res.dim() == 0:
+            res = res.expand(leftidx.numel())
+        return res
+    raise NotImplementedError(f"residual condition {tree!r} is outside the join path's "
+                              "evaluator")
+
+
+def _condition_col_id(token):
+    return column_ref(token) if isinstance(token, str) else None
+
+
+def _is_hashed_equality_condition(condition, hashed_pairs) -> bool:
+    if not isinstance(condition, (list, tuple)) or len(condition) != 3 or condition[1] != '=':
+        return False
+    a, b = _condition_col_id(condition[0]), _condition_col_id(condition[2])
+    return (a, b) in hashed_pairs or (b, a) in hashed_pairs
+
+
+def _filter_pairs(tensors_left, tensors_right, conditions, left_m, right_m):
+    if not conditions or left_m.numel() == 0:
+        return left_m, right_m
+    keep = torch.ones(left_m.numel(), dtype=torch.bool, device=left_m.device)
+    for cond in conditions:
+        keep &= evaluate_join(tensors_left, tensors_right, cond, left_m, right_m)
+    return left_m[keep], right_m[keep]
+
+
+def _apply_fast_path_conditions(tensors_left, tensors_right, condition_list, left_matched,
+                                right_matched, device, hashed_pairs):
+    """Residuals after the unique-hash core; the hashed equality itself is
+    not re-evaluated (hashjoin.py:264-290)."""
+    residual = [c for c in condition_list if not _is_hashed_equality_condition(c, hashed_pairs)]
+    return _filter_pairs(tensors_left, tensors_right, residual, left_matched, right_matched)
+
+
+def _finish_join_result(tensors_left, tensors_right, join_output_keys, return_global_index,
+                        join_type, left_matched, right_matched, left_global_idx,
+                        right_global_idx, n_right, device):
+    """The join-type tail, then the gather (hashjoin.py:293-352)."""
+    empty = torch.empty(0, dtype=torch.int64, device=device)
+    if join_type == 'inner':
+        left_out, right_out = left_matched, right_matched
+    elif join_type == 'right-semi':
+        right_out = ops.matched_rows(right_matched, n_right, True)
+        left_out = empty
+    elif join_type == 'right-anti':
+        right_out = ops.matched_rows(right_matched, n_right, False)
+        left_out = empty
+    elif join_type == 'right-outer':
+        unmatched = ops.matched_rows(right_matched, n_right, False)
+        right_out = torch.cat([right_matched, unmatched])
+        left_out = torch.cat([left_matched,
+                              torch.full((unmatched.numel(),), NULL, dtype=torch.int64,
+                                         device=device)])
+    else:
+        raise NotImplementedError("unhandled join type")
+    return _gather_join_result(tensors_left, tensors_right, join_output_keys,
+                               return_global_index, left_out, right_out, left_global_idx,
+                               right_global_idx)
+
+
+def _gather_join_result(tensors_left, tensors_right, join_output_keys, return_global_index,
+                        left_out, right_out, left_global_idx, right_global_idx):
+    """Each output column at the output rows of its side, then the two
+    global indexes (hashjoin.py:359-377); a null left index gathers the null
+    row."""
+    result = []
+    for key in join_output_keys:
+        if key in tensors_left:
+            result.append(ops.gather_rows(tensors_left[key], left_out))
+        else:
+            result.append(ops.gather_rows(tensors_right[key], right_out))
+    if return_global_index:
+        result.extend([ops.gather_rows(left_global_idx, left_out),
+                       ops.gather_rows(right_global_idx, right_out)])
+    return result
+
+
+def _sort_search_pairs(tensors_left, tensors_right, left, right, condition_list,
+                       bucket_counts, device, name="join"):
+    """Every matching (left row, right row) pair, residuals applied.  The
+    multi-match LDS-table join; the sort/search program (device sort, two
+    searches, the expand kernel) when a bucket is too large for it."""
+    nL, nR = left.size(0), right.size(0)
+    if bucket_counts is not None:
+        lc, rc = bucket_counts
+        if len(lc) != len(rc) or sum(lc) != nL or sum(rc) != nR:
+            raise AssertionError("bucket_counts misaligned with the tick's tensors")
+    if nL == 0 or nR == 0:
+        e = torch.empty(0, dtype=torch.int64, device=device)
+        return e, e.clone()
+    res = _join.hash_join(left.contiguous(), right.contiguous(), unique=False)
+    if res is not None:
+        right_m, left_m, _ = res
+    else:
+        sk, perm = torch.sort(left)
+        lo = torch.searchsorted(sk, right)
+        length = torch.searchsorted(sk, right, right=True) - lo
+        left_m, right_m = join_core.expand(perm, lo, length)
+    if condition_list and left_m.numel() > INT_ROWS_MAX:
+        raise RuntimeError(f"{name}: a core call of {nL} left and {nR} right rows makes "
+                           f"{left_m.numel()} pairs, over the {INT_ROWS_MAX} its residual "
+                           f"conditions take")
+    return _filter_pairs(tensors_left, tensors_right, condition_list, left_m, right_m)
+
+
+def join_kernel_new(tensors_left, tensors_right, left_key, right_key, condition_list,
+                    join_output_keys, return_global_index,
+                    type: Literal['inner', 'right-semi', 'right-anti', 'right-outer'],
+                    left_global_idx, right_global_idx, bucket_counts=None, equality_keys=None,
+                    unique_key_sets=None, hash_scratch_limit_bytes=None, name="join"):
+    """One core call (hashjoin.py:444-494): the unique-hash core, else the
+    semi/anti hash set, else every pair; then the join-type tail and the
+    gather."""
+    assert type in {'inner', 'right-semi', 'right-anti', 'right-outer'}, "Invalid Join type."
+    left = tensors_left[left_key]
+    right = tensors_right[right_key]
+    device = left.device if left.is_cuda else right.device
+    n_right = right.size(0)
+    fast = _try_unique_hash_pairs(
+        tensors_left, tensors_right, left_key, right_key, type, equality_keys, unique_key_sets,
+        _UNIQUE_HASH_SCRATCH_LIMIT_BYTES if hash_scratch_limit_bytes is None
+        else hash_scratch_limit_bytes)
+    if fast is not None:
+        left_m, right_m, hashed = fast
+        left_m, right_m = _apply_fast_path_conditions(tensors_left, tensors_right,
+                                                      condition_list, left_m, right_m, device,
+                                                      hashed)
+        return _finish_join_result(tensors_left, tensors_right, join_output_keys,
+                                   return_global_index, type, left_m, right_m,
+                                   left_global_idx, right_global_idx, n_right, device)
+    if type in ('right-semi', 'right-anti'):
+        kept = _try_hash_semi_anti(tensors_left, tensors_right, left_key, right_key,
+                                   condition_list, anti=type == 'right-anti')
+        if kept is not None:
+            return _gather_join_result(tensors_left, tensors_right, join_output_keys,
+                                       return_global_index,
+                                       torch.empty(0, dtype=torch.int64, device=device), kept,
+                                       left_global_idx, right_global_idx)
+    left_m, right_m = _sort_search_pairs(tensors_left, tensors_right, left, right,
+                                         condition_list, bucket_counts, device, name)
+    return _finish_join_result(tensors_left, tensors_right, join_output_keys,
+                               return_global_index, type, left_m, right_m, left_global_idx,
+                               right_global_idx, n_right, device)
+
+
+def join_call_args(args, tensor_group):
+    """(left key, right key, condition list, join type) of a plan Join
+    (hashjoin.py:506-528): a two-key join partitions on its second key when
+    the first is a float; the other key's equality joins the conditions."""
+    left_keys, right_keys = args['left keys'], args['right keys']
+    condition_list = []
+    if 'join condition' in args.keys():
+        condition_list.append(args['join condition'])
+    lk, rk = left_keys[0], right_keys[0]
+    if len(left_keys) == 2:
+        if tensor_group[left_keys[0]].is_floating_point():
+            assert not tensor_group[left_keys[1]].is_floating_point(), \
+                "Do not support join on two float-type keys"
+            lk, rk = left_keys[1], right_keys[1]
+            condition_list.append([f'#{left_keys[0]}', '=', f'#{right_keys[0]}'])
+        else:
+            condition_list.append([f'#{left_keys[1]}', '=', f'#{right_keys[1]}'])
+    assert args['join type'] in JOIN_TYPES, f"Unsupport Join Type {args['join type']}"
+    return lk, rk, condition_list, JOIN_TYPES[args['join type']]

@@ -1,0 +1,143 @@
+"""Small device operations of the join's output side over the library:
+payload gather (with the null row), matched-row marking and selection, and
+the int64 group-by.  See include/pageflow_join.h for the kernels."""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+import torch
+
+from . import _lib
+from .layout import NULL
+
+_host: Optional[torch.Tensor] = None
+
+
+def _host_buffer() -> torch.Tensor:
+    global _host
+    if _host is None:
+        _host = torch.empty(2, dtype=torch.int64, pin_memory=True)
+    return _host
+
+
+def gather_rows(src: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
+    """``src``'s rows at ``idx`` (int64, 1-D), a null row (every element
+    -64) where ``idx`` is -64 or ``src`` has no row: index_select and
+    index_with_null (conversion.py:39-50) in one kernel."""
+    _lib.require_cuda(src, "gather source")
+    _lib.require_cuda(idx, "gather index")
+    if src.element_size() not in (1, 2, 4, 8) or src.dim() not in (1, 2):
+        raise ValueError("gather_rows: 1-D or 2-D rows of 1/2/4/8-byte elements")
+    if src.is_floating_point() or src.dtype == torch.bool:
+        raise ValueError("gather_rows: integer columns (the join path is integer)")
+    src = src.contiguous()
+    idx = idx.contiguous().to(torch.int64)
+    row_elems = src.shape[1] if src.dim() == 2 else 1
+    out = torch.empty((idx.numel(),) + tuple(src.shape[1:]), dtype=src.dtype, device=src.device)
+    lib = _lib.load()
+    with torch.cuda.device(src.device):
+        _lib.check(lib.pf_gather_rows(_lib.ptr(src), src.shape[0], src.element_size(), row_elems,
+                                      _lib.ptr(idx), idx.numel(), _lib.ptr(out), None,
+                                      _lib.stream_of(src.device)))
+    return out
+
+
+def matched_rows(idx: torch.Tensor, n: int, matched: bool) -> torch.Tensor:
+    """The rows of [0, n), ascending, that appear in ``idx`` (matched) or
+    do not (unmatched): the bool scatter + nonzero tails of
+    operators/hashjoin.py:301-348 as two kernels."""
+    dev = idx.device
+    flags = torch.zeros(max(n, 1), dtype=torch.uint8, device=dev)
+    lib = _lib.load()
+    st = _lib.stream_of(dev)
+    out = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+    host = _host_buffer()
+    ws_bytes = lib.pf_select_rows_workspace_bytes(n)
+    ws = _lib.workspace(ws_bytes, dev)
+    with torch.cuda.device(dev):
+        _lib.check(lib.pf_mark_rows(_lib.ptr(idx.contiguous()), idx.numel(), _lib.ptr(flags), n,
+                                    1, st))
+        _lib.check(lib.pf_select_rows(_lib.ptr(flags), n, int(matched), _lib.ptr(out),
+                                      ctypes.c_void_p(host.data_ptr()), _lib.ptr(ws), ws_bytes,
+                                      st))
+    return out[:int(host[0])]
+
+
+def group_sum_count(gid: torch.Tensor, val: torch.Tensor, G: int,
+                    out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None,
+                    accumulate: bool = False) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Exact int64 SUM(val) and COUNT(val != -64) per group id in [0, G)."""
+    _lib.require_cuda(gid, "group ids")
+    if gid.dtype != torch.int32 or val.dtype != torch.int64 or gid.numel() != val.numel():
+        raise ValueError("group_sum_count: int32 group ids and int64 values of one length")
+    dev = gid.device
+    if out is None:
+        out = (torch.empty(G, dtype=torch.int64, device=dev),
+               torch.empty(G, dtype=torch.int64, device=dev))
+    lib = _lib.load()
+    with torch.cuda.device(dev):
+        _lib.check(lib.pf_group_sum_count(_lib.ptr(gid.contiguous()), _lib.ptr(val.contiguous()),
+                                          gid.numel(), G, _lib.ptr(out[0]), _lib.ptr(out[1]),
+                                          int(accumulate), None, _lib.stream_of(dev)))
+    return out
+
+
+class DimTable:
+    """A resident dimension hash table: int32 key -> int32 group in [0, G).
+    Built once on the device and reused by every request that joins to it."""
+
+    def __init__(self, keys: torch.Tensor, groups: torch.Tensor, G: int,
+                 stream: Optional[torch.cuda.Stream] = None):
+        _lib.require_cuda(keys, "dimension keys")
+        if keys.dtype != torch.int32 or groups.dtype != torch.int32 or \
+                keys.numel() != groups.numel():
+            raise ValueError("DimTable: int32 keys and int32 groups of one length")
+        self.G = int(G)
+        self.n = keys.numel()
+        self.device = keys.device
+        lib = _lib.load()
+        self.slots = int(lib.pf_dim_table_slots(self.n))
+        self.table = torch.empty(self.slots, dtype=torch.int64, device=self.device)
+        self.err = torch.empty(2, dtype=torch.int64, device=self.device)
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        with torch.cuda.device(self.device):
+            _lib.check(lib.pf_dim_table_build(_lib.ptr(keys.contiguous()),
+                                              _lib.ptr(groups.contiguous()), self.n, self.G,
+                                              _lib.ptr(self.table), self.slots,
+                                              _lib.ptr(self.err),
+                                              ctypes.c_void_p(st.cuda_stream)))
+
+    @property
+    def nbytes(self) -> int:
+        return self.slots * 8
+
+    def check(self) -> None:
+        """Raises when a group was out of range or a key repeated (waits)."""
+        bad_group, dup = (int(v) for v in self.err.tolist())
+        if bad_group or dup:
+            raise ValueError(f"DimTable: {bad_group} groups out of [0, {self.G}), "
+                             f"{dup} repeated keys")
+
+    def join_groupby(self, fk: torch.Tensor, amt: torch.Tensor,
+                     out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None,
+                     stream: Optional[torch.cuda.Stream] = None
+                     ) -> Tuple[torch.Tensor, torch.Tensor]:
+        """SUM(amt), COUNT(amt) per group over the fact rows whose key is in
+        the table (inner join)."""
+        if fk.dtype != torch.int32 or amt.dtype != torch.int64 or fk.numel() != amt.numel():
+            raise ValueError("join_groupby: int32 keys and int64 amounts of one length")
+        if out is None:
+            out = (torch.empty(self.G, dtype=torch.int64, device=self.device),
+                   torch.empty(self.G, dtype=torch.int64, device=self.device))
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        lib = _lib.load()
+        with torch.cuda.device(self.device):
+            _lib.check(lib.pf_fact_join_groupby(_lib.ptr(fk), _lib.ptr(amt), fk.numel(),
+                                                _lib.ptr(self.table), self.slots, self.G,
+                                                _lib.ptr(out[0]), _lib.ptr(out[1]),
+                                                ctypes.c_void_p(st.cuda_stream)))
+        return out
+
+
+__all__ = ["gather_rows", "matched_rows", "group_sum_count", "DimTable", "NULL"]
