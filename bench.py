@@ -153,6 +153,79 @@ def bench_partition(args, dev) -> None:
     print(json.dumps(line), flush=True)
 
 
+def bench_batch(args, dev, emit: bool = True):
+    """256 uneven inner joins (int32 keys): the batched pipeline (autotuned,
+    replayed from a hipGraph) against the same kernels launched one join at
+    a time with one host wait a join.  tuples/s = (build + probe rows) /
+    time; algorithmic bytes = sum over joins of 4 nb + 4 np + 16 matches."""
+    import torch
+    from pageflow_amd import _lib
+    from pageflow_amd.batch import BatchJoin, autotune, make_batch
+
+    log("generating the 256-join batch")
+    nb, np_, bkeys, pkeys, expect = make_batch(device=dev)
+    tuples = float(nb.sum() + np_.sum())
+    tune = autotune(nb, np_, bkeys, pkeys, reps=2,
+                    cache=os.path.join(ROOT, "profiles", "batch_autotune.json"), device=dev)
+    log(f"autotuned per class: {tune}")
+    base = BatchJoin(nb, np_, tune, dev)
+    t_plain = []
+    for _ in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        base.run(bkeys, pkeys, per_join=True)
+        torch.cuda.synchronize()
+        t_plain.append(time.perf_counter() - t0)
+    t_plain = sorted(t_plain[args.warmup:])[len(t_plain[args.warmup:]) // 2]
+    assert (base.counts.cpu().numpy() == expect).all()
+    bj = BatchJoin(nb, np_, tune, dev)
+    st = bj.capture(bkeys, pkeys)
+    for _ in range(args.warmup):
+        bj.replay()
+    st.synchronize()
+    _lib.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bj.replay()
+    st.synchronize()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    _lib.profile(False)
+    assert (bj.counts.cpu().numpy() == expect).all()
+    # eager run of the same pipeline, profiled per kernel
+    _lib.profile(True)
+    bj.run(bkeys, pkeys)
+    torch.cuda.synchronize()
+    prof = _lib.profile_read()
+    _lib.profile(False)
+    pairs = int(expect.sum())
+    alg = float(4 * nb.sum() + 4 * np_.sum() + 16 * pairs)
+    res = {"joins": int(len(nb)), "build_rows": int(nb.sum()), "probe_rows": int(np_.sum()),
+           "pairs": pairs, "tune": [list(t) for t in tune],
+           "batched_ms": t_batch * 1e3, "per_join_ms": t_plain * 1e3,
+           "batched_tuples_per_s": tuples / t_batch, "per_join_tuples_per_s": tuples / t_plain,
+           "speedup": t_plain / t_batch,
+           "batched_GBps": alg / t_batch / 1e9, "frac": alg / t_batch / 1e9 / HBM_PEAK_GBPS,
+           "kernels_ms": {k: v[0] for k, v in prof.items()}}
+    log(f"batch: batched {t_batch * 1e3:.2f} ms, per-join {t_plain * 1e3:.2f} ms, "
+        f"speedup {t_plain / t_batch:.2f}x")
+    if emit:
+        dom = max(prof, key=lambda k: prof[k][0])
+        line = {"metric": "batched join tuples/s (256 uneven joins)", "value": tuples / t_batch,
+                "unit": "tuples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+                "config": {"workload": "256 inner joins, build log-uniform 1K-4M unique int32, "
+                                       "probe 8x with 15% absent"},
+                "roofline": {"bound": "hbm", "kernel": dom, "achieved": alg / t_batch / 1e9,
+                             "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                             "frac": alg / t_batch / 1e9 / HBM_PEAK_GBPS, "traffic": None},
+                "batch": res}
+        print(json.dumps(line), flush=True)
+    bj.close()
+    base.close()
+    return res
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,7 +236,7 @@ def main() -> None:
     ap.add_argument("--bits", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["join", "partition"], default="join")
+    ap.add_argument("--workload", choices=["join", "partition", "batch"], default="join")
     ap.add_argument("--part-bits", default="8,9,10,11,12")
     ap.add_argument("--part-shapes", default="-1")
     args = ap.parse_args()
@@ -181,6 +254,9 @@ def main() -> None:
     torch.cuda.set_device(dev)
     if args.workload == "partition":
         bench_partition(args, dev)
+        return
+    if args.workload == "batch":
+        bench_batch(args, dev)
         return
     nb, np_ = args.build_rows, args.probe_rows
     log(f"generating {nb} build / {np_} probe rows")

@@ -233,6 +233,34 @@ int pf_fact_join_groupby(const int32_t* fk, const int64_t* amt, int64_t n,
                          const uint64_t* table, int64_t slots, int32_t G, int64_t* sum,
                          int64_t* cnt, void* stream);
 
+/* ---------------------------------------------------------- batching --
+ * No counterpart in the reference (the batched tier).  A plan for njoins
+ * independent inner joins on int32 keys (unique builds): nb[j] build and
+ * np[j] probe rows, stored back to back in the key buffers.  tune: NULL or
+ * 9 int32, per size class of the build (< 32K rows, < 512K, larger) the
+ * table slots per build row (2 or 4), the probe rows per work item and the
+ * probe/compaction workgroup size (256 or 512).  The plan owns its device
+ * scratch (one allocation at creation).
+ *
+ * pf_batch_join writes join j's pairs (join-local int64 probe row, build
+ * row; probe-row order) at out_probe/out_build + (sum of np[i], i < j),
+ * their count at counts[j] (device).  mode 0: the whole batch as one
+ * pipeline of persistent kernels fed by device work queues (no host wait);
+ * mode 1: the same kernels one join at a time with one host wait a join
+ * (the plain baseline).  pf_batch_capture records mode 0 on `stream` (not
+ * the null stream) as a hipGraph bound to these buffers; pf_batch_replay
+ * launches it.  pf_batch_plan_info: 0 build rows, 1 probe rows, 2 table
+ * slots, 3 build items, 4 probe items, 5 scratch bytes. */
+void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
+                           const int32_t* tune);
+int pf_batch_plan_destroy(void* plan);
+int64_t pf_batch_plan_info(void* plan, int32_t what);
+int pf_batch_join(void* plan, const int32_t* bkeys, const int32_t* pkeys, int64_t* out_probe,
+                  int64_t* out_build, int64_t* counts, int32_t mode, void* stream);
+int pf_batch_capture(void* plan, const int32_t* bkeys, const int32_t* pkeys, int64_t* out_probe,
+                     int64_t* out_build, int64_t* counts, void* stream);
+int pf_batch_replay(void* plan, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,12 @@ _SIGNATURES = {
     "pf_dim_table_slots": (I64, [I64]),
     "pf_dim_table_build": (ctypes.c_int, [P, P, I64, I32, P, I64, P, P]),
     "pf_fact_join_groupby": (ctypes.c_int, [P, P, I64, P, I64, I32, P, P, P]),
+    "pf_batch_plan_create": (P, [I32, P, P, P]),
+    "pf_batch_plan_destroy": (ctypes.c_int, [P]),
+    "pf_batch_plan_info": (I64, [P, I32]),
+    "pf_batch_join": (ctypes.c_int, [P, P, P, P, P, P, I32, P]),
+    "pf_batch_capture": (ctypes.c_int, [P, P, P, P, P, P, P]),
+    "pf_batch_replay": (ctypes.c_int, [P, P]),
     "pf_hash_join_workspace_bytes": (SZ, [I64, I64, I32, I32]),
     "pf_hash_join_prepare": (ctypes.c_int, [P, I64, P, I64, I32, I32, P, P, SZ, P]),
     "pf_hash_join_write": (ctypes.c_int, [I64, I64, I32, I32, I64, PP, I32, P, P, PP, P, SZ,

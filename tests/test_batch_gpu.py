@@ -1,0 +1,94 @@
+"""Batched joins (pf_batch_*): every join's pairs equal the oracle's
+unique-hash pairs of that join (probe-row order), in the batched pipeline,
+in the per-join baseline, and on hipGraph replays (several, with eager work
+between them); every tunable shape gives the same answer."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from pageflow_amd.batch import BatchJoin, make_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_all(bj, nb, np_, bkeys, pkeys):
+    counts = bj.counts.cpu().numpy()
+    bo = np.concatenate(([0], np.cumsum(nb)))
+    po = np.concatenate(([0], np.cumsum(np_)))
+    bk = bkeys.cpu().numpy().astype(np.int64)
+    pk = pkeys.cpu().numpy().astype(np.int64)
+    op = bj.out_probe.cpu().numpy()
+    ob = bj.out_build.cpu().numpy()
+    for j in range(len(nb)):
+        rb, rp = oracle.unique_hash_pairs([bk[bo[j]:bo[j + 1]]], [pk[po[j]:po[j + 1]]])
+        assert counts[j] == rb.size, j
+        assert np.array_equal(op[po[j]:po[j] + counts[j]], rp), j
+        assert np.array_equal(ob[po[j]:po[j] + counts[j]], rb), j
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    return make_batch(n_joins=24, seed=9, lo=10, hi=6e5)
+
+
+@pytest.mark.parametrize("per_join", [False, True])
+def test_batch_matches_oracle(small_batch, per_join):
+    nb, np_, bkeys, pkeys, expect = small_batch
+    bj = BatchJoin(nb, np_)
+    bj.run(bkeys, pkeys, per_join=per_join)
+    torch.cuda.synchronize()
+    assert np.array_equal(bj.counts.cpu().numpy(), expect)
+    _check_all(bj, nb, np_, bkeys, pkeys)
+
+
+@pytest.mark.parametrize("tune", [[(4, 4096, 512)] * 3, [(2, 1024, 256), (4, 65536, 512),
+                                                         (2, 16384, 256)]])
+def test_every_tunable_gives_the_same_pairs(small_batch, tune):
+    nb, np_, bkeys, pkeys, expect = small_batch
+    bj = BatchJoin(nb, np_, tune)
+    bj.run(bkeys, pkeys)
+    torch.cuda.synchronize()
+    _check_all(bj, nb, np_, bkeys, pkeys)
+
+
+def test_graph_replays_equal_eager(small_batch):
+    nb, np_, bkeys, pkeys, expect = small_batch
+    eager = BatchJoin(nb, np_)
+    eager.run(bkeys, pkeys)
+    torch.cuda.synchronize()
+    want = (eager.out_probe.clone(), eager.out_build.clone(), eager.counts.clone())
+    bj = BatchJoin(nb, np_)
+    st = bj.capture(bkeys, pkeys)
+    for rep in range(4):
+        bj.out_probe.fill_(-7)
+        bj.counts.fill_(-7)
+        torch.cuda.synchronize()
+        bj.replay()
+        st.synchronize()
+        n = int(want[2].sum())
+        assert torch.equal(bj.counts, want[2]), rep
+        for j in range(len(nb)):
+            a, b = bj.pairs(j)
+            ea, eb = eager.pairs(j)
+            assert torch.equal(a, ea) and torch.equal(b, eb), (rep, j)
+        # eager work between replays
+        eager.run(bkeys, pkeys)
+        torch.cuda.synchronize()
+    assert n == int(expect.sum())
+
+
+def test_empty_and_tiny_joins():
+    nb = np.array([0, 1, 5, 0, 3])
+    np_ = np.array([4, 0, 9, 0, 24])
+    g = torch.Generator().manual_seed(1)
+    bkeys = torch.randint(-50, 50, (int(nb.sum()),), generator=g, dtype=torch.int64)
+    # unique within each join
+    bkeys = torch.tensor([7, -3, 4, 9, 11, 12, 5, 6, 1], dtype=torch.int32)[:int(nb.sum())]
+    pkeys = torch.randint(-4, 14, (int(np_.sum()),), generator=g).to(torch.int32)
+    bj = BatchJoin(nb, np_)
+    bj.run(bkeys.cuda(), pkeys.cuda())
+    torch.cuda.synchronize()
+    _check_all(bj, nb, np_, bkeys, pkeys)
