@@ -83,6 +83,17 @@ struct Plan {
   int persist = 1024;  // workgroups of a persistent launch
 };
 
+// A work item taken from the queue, as a wave-uniform (scalar) value.  The
+// persistent loops below exit on it and hold barriers; the compiler must see
+// the exit as uniform, otherwise it may nest the loop so that the barriers
+// and the read of the next item run in an inner loop that the lane fetching
+// items (thread 0) never re-enters, and the other lanes spin on a stale item.
+__device__ __forceinline__ int64_t uniform_item(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ uint32_t bslot(int32_t key, uint32_t mask) {
   return (uint32_t)mix64((uint64_t)(int64_t)key) & mask;
 }
@@ -107,7 +118,7 @@ __global__ __launch_bounds__(256) void bj_build(const int32_t* __restrict__ bkey
   for (;;) {
     if (threadIdx.x == 0) s_item = item_lo + atomicAdd(queue, 1);
     __syncthreads();
-    const int64_t item = s_item;
+    const int64_t item = uniform_item(s_item);
     __syncthreads();
     if (item >= n_items) return;
     const int j = item_join[item];
@@ -150,10 +161,17 @@ __global__ __launch_bounds__(NT) void bj_probe(const int32_t* __restrict__ pkeys
                                                int64_t* __restrict__ item_cnt, int32_t* queue) {
   __shared__ int64_t s_item;
   __shared__ int red[NT / kWave];
+  // one thread-0 region an iteration (the previous item's count, then the
+  // next item), as in the other persistent loops
+  int64_t prev = -1;
+  int prev_tot = 0;
   for (;;) {
-    if (threadIdx.x == 0) s_item = item_lo + atomicAdd(queue, 1);
+    if (threadIdx.x == 0) {
+      if (prev >= 0) item_cnt[prev] = prev_tot;
+      s_item = item_lo + atomicAdd(queue, 1);
+    }
     __syncthreads();
-    const int64_t item = s_item;
+    const int64_t item = uniform_item(s_item);
     __syncthreads();
     if (item >= item_lo + n_items) return;
     const int j = item_join[item];
@@ -192,11 +210,10 @@ __global__ __launch_bounds__(NT) void bj_probe(const int32_t* __restrict__ pkeys
     c = wave_sum(c);
     if (lane_id() == 0) red[threadIdx.x / kWave] = c;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      int tot = 0;
-      for (int w = 0; w < NT / kWave; ++w) tot += red[w];
-      item_cnt[item] = tot;
-    }
+    prev_tot = 0;
+#pragma unroll
+    for (int w = 0; w < NT / kWave; ++w) prev_tot += red[w];
+    prev = item;
   }
 }
 
@@ -218,7 +235,7 @@ __global__ __launch_bounds__(NT) void bj_compact(const int64_t* __restrict__ p_o
   for (;;) {
     if (threadIdx.x == 0) s_item = item_lo + atomicAdd(queue, 1);
     __syncthreads();
-    const int64_t item = s_item;
+    const int64_t item = uniform_item(s_item);
     __syncthreads();
     if (item >= item_lo + n_items) return;
     const int j = item_join[item];
