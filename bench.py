@@ -226,6 +226,83 @@ def bench_batch(args, dev, emit: bool = True):
     return res
 
 
+def bench_serve(args, dev, emit: bool = True):
+    """The serving workload: a Poisson stream of fact->dimension join +
+    group-by requests (fact rows log-uniform 100K-50M, 8 recurring
+    dimensions of 1K-2M rows, 25 groups) through the continuous-batching
+    engine.  Capacity is measured closed-loop (every request arrived at the
+    start); the open-loop run offers 0.7 of it and reports p50/p99 latency
+    (arrival -> result on the host).  Algorithmic bytes: 12 a fact row,
+    8 a dimension row built, 400 a result."""
+    import numpy as np
+    import torch
+    from pageflow_amd import _lib
+    from pageflow_amd import serving
+
+    log(f"generating the serving workload ({args.serve_requests} requests)")
+    wl = serving.make_workload(args.serve_requests, rate=0.0, seed=5, device=dev)
+    eng = serving.ServingEngine(dev, hbm_budget=args.serve_budget << 20, max_active=32,
+                                step_rows=args.serve_step_rows, rows_per_wg=1 << 16)
+    for d, (k, g) in wl.dims.items():
+        eng.register_dim(d, k, g, wl.G)
+    eng.run(wl.requests[:16], realtime=False)  # warm-up: builds the 8 tables
+    eng.check_tables()
+    fact_rows = sum(r.fk.numel() for r in wl.requests)
+    t0 = time.perf_counter()
+    closed = eng.run(wl.requests, realtime=False)
+    t_closed = time.perf_counter() - t0
+    capacity = len(wl.requests) / t_closed
+    rate = args.serve_load * capacity
+    log(f"serving capacity {capacity:.0f} requests/s ({fact_rows / t_closed / 1e9:.2f} G fact "
+        f"rows/s); offering {rate:.0f} requests/s")
+    reqs = serving.with_arrivals(wl.requests, rate, seed=5)
+    steps0 = eng.stats["steps"]
+    _lib.profile(True)
+    t0 = time.perf_counter()
+    opened = eng.run(reqs)
+    t_open = time.perf_counter() - t0
+    prof = _lib.profile_read()
+    _lib.profile(False)
+    steps = eng.stats["steps"] - steps0
+    # the mix does not change a result; every fact key has a dimension row
+    for a, b in zip(closed, opened):
+        assert np.array_equal(a.sum, b.sum) and np.array_equal(a.cnt, b.cnt), a.rid
+    for r, res in zip(wl.requests, opened):
+        assert int(res.cnt.sum()) == r.fk.numel(), r.rid
+    lat = serving.latency_summary(opened)
+    ss_ms, ss_n = prof.get("serve_step", (0.0, 0))
+    step_bytes = 12.0 * fact_rows / max(steps, 1)
+    achieved = step_bytes / (ss_ms / max(ss_n, 1) * 1e-3) / 1e9 if ss_n else None
+    res = {"requests": len(reqs), "fact_rows": int(fact_rows), "dims": len(wl.dims),
+           "dim_rows": wl.dim_rows, "offered_rate_rps": rate, "load": args.serve_load,
+           "capacity_rps": capacity, "closed_loop_s": t_closed, "open_loop_s": t_open,
+           "tuples_per_s": fact_rows / t_open, "closed_loop_tuples_per_s": fact_rows / t_closed,
+           **lat, "steps": steps, "max_items_per_step": eng.stats["max_items"],
+           "table_builds": eng.stats["builds"], "evictions": eng.stats["evictions"],
+           "hbm_budget_MiB": args.serve_budget, "peak_engine_MiB": eng.peak_used / 2**20,
+           "serve_step_avg_ms": ss_ms / max(ss_n, 1),
+           "serve_step_GBps": achieved,
+           "serve_step_frac": achieved / HBM_PEAK_GBPS if achieved else None}
+    log(f"serving: p50 {lat['p50_ms']:.2f} ms, p99 {lat['p99_ms']:.2f} ms, "
+        f"{fact_rows / t_open / 1e9:.2f} G fact rows/s over {len(reqs)} requests")
+    if emit:
+        line = {"metric": "serving join+group-by tuples/s (p99 latency beside it)",
+                "value": fact_rows / t_open, "unit": "tuples/s", "n_gpus": 1,
+                "steps": len(reqs), "warmup": 16, "ms_per_step": t_open * 1e3 / len(reqs),
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "int64", "data": "synthetic",
+                "config": {"workload": "Poisson stream of join+group-by requests: fact "
+                                       "log-uniform 100K-50M rows, 8 resident dimensions "
+                                       "1K-2M rows, 25 groups"},
+                "roofline": {"bound": "hbm", "kernel": "serve_step", "achieved": achieved,
+                             "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBPS if achieved else None,
+                             "traffic": None},
+                "serving": res}
+        print(json.dumps(line), flush=True)
+    return res
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,7 +313,15 @@ def main() -> None:
     ap.add_argument("--bits", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["join", "partition", "batch"], default="join")
+    ap.add_argument("--workload", choices=["join", "partition", "batch", "serve"],
+                    default="join")
+    ap.add_argument("--tiers", choices=["all", "none"], default="all",
+                    help="with the join workload: also run the batch and serving tiers and "
+                         "add their results to the line")
+    ap.add_argument("--serve-requests", type=int, default=2048)
+    ap.add_argument("--serve-load", type=float, default=0.7)
+    ap.add_argument("--serve-budget", type=int, default=8192, help="MiB")
+    ap.add_argument("--serve-step-rows", type=int, default=1 << 25)
     ap.add_argument("--part-bits", default="8,9,10,11,12")
     ap.add_argument("--part-shapes", default="-1")
     args = ap.parse_args()
@@ -257,6 +342,9 @@ def main() -> None:
         return
     if args.workload == "batch":
         bench_batch(args, dev)
+        return
+    if args.workload == "serve":
+        bench_serve(args, dev)
         return
     nb, np_ = args.build_rows, args.probe_rows
     log(f"generating {nb} build / {np_} probe rows")
@@ -332,6 +420,14 @@ def main() -> None:
         log("timing the CPU baseline")
         cpu = cpu_baseline(build.cpu(), probe[:args.cpu_sample].cpu(), args.cpu_sample)
 
+    tiers = {}
+    if args.tiers == "all":
+        del build, pays, probe, out_p, out_b, out_pay, src, dst, plan, op, ob
+        torch.cuda.empty_cache()
+        tiers["batch"] = bench_batch(args, dev, emit=False)
+        torch.cuda.empty_cache()
+        tiers["serving"] = bench_serve(args, dev, emit=False)
+
     line = {
         "metric": "join tuples/s + % HBM roofline, batched vs per-join launch; p99 ms (serving)",
         "value": tuples,
@@ -358,6 +454,7 @@ def main() -> None:
         "kernels": kernels,
         "copy_GBps": copy_gbps,
         "cpu_baseline": cpu,
+        **tiers,
     }
     print(json.dumps(line), flush=True)
 

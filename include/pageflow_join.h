@@ -233,6 +233,32 @@ int pf_fact_join_groupby(const int32_t* fk, const int64_t* amt, int64_t n,
                          const uint64_t* table, int64_t slots, int32_t G, int64_t* sum,
                          int64_t* cnt, void* stream);
 
+/* One (request, fact rows) item of a serving step.  No counterpart in the
+ * reference (the serving tier).  fk/amt: the request's fact rows to add in
+ * this step (n of them); table/slots/G: the request's resident dimension
+ * table (pf_dim_table_build); sum/cnt: the request's int64[G] accumulators,
+ * zeroed by the caller before its first step. */
+#define PF_SERVE_MAX_ITEMS 40
+typedef struct pf_serve_item {
+  const int32_t* fk;
+  const int64_t* amt;
+  int64_t n;
+  const uint64_t* table;
+  int64_t slots;
+  int64_t* sum;
+  int64_t* cnt;
+  int32_t G;
+  int32_t reserved;
+} pf_serve_item;
+
+/* One launch over up to PF_SERVE_MAX_ITEMS items (host array, passed to the
+ * kernel by value: nothing is copied or allocated), rows_per_wg fact rows a
+ * workgroup: SUM(amount), COUNT(amount != -64) per dimension group added to
+ * each item's accumulators.  Items of one request in different steps add
+ * up to the request's result.  No wait. */
+int pf_serve_step(const pf_serve_item* items, int32_t n_items, int32_t rows_per_wg,
+                  void* stream);
+
 /* ---------------------------------------------------------- batching --
  * No counterpart in the reference (the batched tier).  A plan for njoins
  * independent inner joins on int32 keys (unique builds): nb[j] build and

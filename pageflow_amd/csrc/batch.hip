@@ -127,21 +127,47 @@ __global__ __launch_bounds__(256) void bj_build(const int32_t* __restrict__ bkey
     const int64_t r1 = r0 + BUILD_ROWS < nrows ? r0 + BUILD_ROWS : nrows;
     uint64_t* t = table + t_off[j];
     const uint32_t m = mask[j];
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) {
-      const int32_t k = bkeys[b_off[j] + r];
-      const uint64_t v = ((uint64_t)(uint32_t)k << 32) | (uint32_t)r;
-      uint32_t s = bslot(k, m);
-      for (;;) {
-        uint64_t prev = t[s];
-        if (prev == kEmpty)
-          prev = atomicCAS((unsigned long long*)&t[s], (unsigned long long)kEmpty,
-                           (unsigned long long)v);
-        if (prev == kEmpty) break;
-        if ((uint32_t)(prev >> 32) == (uint32_t)k) {
-          atomicMin((unsigned long long*)&t[s], (unsigned long long)v);
-          break;
+    // U rows a thread with their CASes in flight together (one at a time is
+    // latency-bound).  No read before the CAS: in a fresh table of unique
+    // keys the first slot is usually empty and the CAS returns what a read
+    // would have.
+    constexpr int U = 4;
+    for (int64_t rr = r0 + threadIdx.x; rr < r1; rr += 256 * U) {
+      int32_t k[U];
+      uint32_t s[U];
+      bool todo[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t r = rr + u * 256;
+        todo[u] = r < r1;
+        k[u] = todo[u] ? bkeys[b_off[j] + r] : 0;
+        s[u] = bslot(k[u], m);
+      }
+      bool any = true;
+      while (any) {
+        uint64_t prev[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint64_t v = ((uint64_t)(uint32_t)k[u] << 32) | (uint32_t)(rr + u * 256);
+          prev[u] = todo[u] ? atomicCAS((unsigned long long*)&t[s[u]], (unsigned long long)kEmpty,
+                                        (unsigned long long)v)
+                            : kEmpty;
         }
-        s = (s + 1) & m;
+        any = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (!todo[u]) continue;
+          const uint64_t v = ((uint64_t)(uint32_t)k[u] << 32) | (uint32_t)(rr + u * 256);
+          if (prev[u] == kEmpty) {
+            todo[u] = false;
+          } else if ((uint32_t)(prev[u] >> 32) == (uint32_t)k[u]) {
+            atomicMin((unsigned long long*)&t[s[u]], (unsigned long long)v);
+            todo[u] = false;
+          } else {
+            s[u] = (s[u] + 1) & m;
+            any = true;
+          }
+        }
       }
     }
   }
@@ -180,29 +206,30 @@ __global__ __launch_bounds__(NT) void bj_probe(const int32_t* __restrict__ pkeys
     const uint64_t* t = table + t_off[j];
     const uint32_t m = mask[j];
     int c = 0;
-    constexpr int U = 4;
+    // U probe rows a thread with their first-slot reads in flight together;
+    // the rare longer probe sequences are resolved afterwards
+    constexpr int U = 8;
     for (int r0 = threadIdx.x; r0 < len; r0 += NT * U) {
       int32_t k[U];
+      uint32_t s[U];
+      uint64_t v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int r = r0 + u * NT;
         k[u] = r < len ? pkeys[base + r] : 0;
+        s[u] = bslot(k[u], m);
       }
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = r0 + u * NT < len ? t[s[u]] : kEmpty;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int r = r0 + u * NT;
         if (r >= len) break;
-        int32_t hit = -1;
-        uint32_t s = bslot(k[u], m);
-        for (;;) {
-          const uint64_t v = t[s];
-          if (v == kEmpty) break;
-          if ((uint32_t)(v >> 32) == (uint32_t)k[u]) {
-            hit = (int32_t)(uint32_t)v;
-            break;
-          }
-          s = (s + 1) & m;
+        while (v[u] != kEmpty && (uint32_t)(v[u] >> 32) != (uint32_t)k[u]) {
+          s[u] = (s[u] + 1) & m;
+          v[u] = t[s[u]];
         }
+        const int32_t hit = v[u] == kEmpty ? -1 : (int32_t)(uint32_t)v[u];
         partner[base + r] = hit;
         c += hit >= 0;
       }

@@ -133,14 +133,53 @@ __global__ void k_dim_build(const int32_t* __restrict__ key, const int32_t* __re
   }
 }
 
-__device__ __forceinline__ int dim_lookup(const uint64_t* __restrict__ table, uint32_t mask,
-                                          int32_t k) {
-  uint32_t s = dim_slot(k, mask);
-  for (;;) {
-    const uint64_t t = table[s];
-    if (t == kEmptySlot) return -1;
-    if ((int32_t)(t >> 32) == k) return (int)(uint32_t)t;
-    s = (s + 1) & mask;
+// Accumulates fact rows [i0, i1) (stride `step` from i0) into the LDS
+// group table: U rows a thread with their first table reads in flight
+// together (the rare longer probe sequences resolve afterwards).
+__device__ __forceinline__ void fact_rows_into(const int32_t* __restrict__ fk,
+                                               const int64_t* __restrict__ amt, int64_t i0,
+                                               int64_t i1, int64_t step,
+                                               const uint64_t* __restrict__ table, uint32_t mask,
+                                               int64_t* acc_rep) {
+  constexpr int U = 8;
+  for (int64_t b = i0; b < i1; b += step * U) {
+    int32_t k[U];
+    int64_t v[U];
+    uint32_t s[U];
+    uint64_t t[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = b + u * step;
+      k[u] = i < i1 ? fk[i] : 0;
+      v[u] = i < i1 ? amt[i] : 0;
+      s[u] = dim_slot(k[u], mask);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) t[u] = b + u * step < i1 ? table[s[u]] : kEmptySlot;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      while (t[u] != kEmptySlot && (int32_t)(t[u] >> 32) != k[u]) {
+        s[u] = (s[u] + 1) & mask;
+        t[u] = table[s[u]];
+      }
+      if (t[u] == kEmptySlot) continue;  // no dimension row (or past the end)
+      int64_t* a = acc_rep + (int64_t)(uint32_t)t[u] * 2;
+      lds_add(a, v[u]);
+      if (v[u] != kNull) lds_add(a + 1, 1);
+    }
+  }
+}
+
+__device__ __forceinline__ void flush_groups(const int64_t* acc, int G, int R, int64_t* sum,
+                                             int64_t* cnt, int nt) {
+  for (int g = threadIdx.x; g < G; g += nt) {
+    int64_t s = 0, c = 0;
+    for (int r = 0; r < R; ++r) {
+      s += acc[((int64_t)r * G + g) * 2];
+      c += acc[((int64_t)r * G + g) * 2 + 1];
+    }
+    if (s) global_add(&sum[g], s);
+    if (c) global_add(&cnt[g], c);
   }
 }
 
@@ -151,41 +190,45 @@ __global__ __launch_bounds__(GB_NT) void k_fact_join_groupby(
     const uint64_t* __restrict__ table, uint32_t mask, int G, int R, int64_t* sum,
     int64_t* cnt) {
   __shared__ int64_t acc[32 * 1024 / 8];
-  const int tid = threadIdx.x;
-  for (int x = tid; x < R * G * 2; x += GB_NT) acc[x] = 0;
+  for (int x = threadIdx.x; x < R * G * 2; x += GB_NT) acc[x] = 0;
   __syncthreads();
   const int rep = (int)(lane_id() % R);
-  constexpr int U = 4;  // rows in flight a thread
   const int64_t stride = (int64_t)gridDim.x * GB_NT;
-  for (int64_t i0 = blockIdx.x * (int64_t)GB_NT + tid; i0 < n; i0 += stride * U) {
-    int32_t k[U];
-    int64_t v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t i = i0 + u * stride;
-      k[u] = i < n ? fk[i] : 0;
-      v[u] = i < n ? amt[i] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (i0 + u * stride >= n) break;
-      const int g = dim_lookup(table, mask, k[u]);
-      if (g < 0) continue;
-      int64_t* a = acc + ((int64_t)rep * G + g) * 2;
-      lds_add(a, v[u]);
-      if (v[u] != kNull) lds_add(a + 1, 1);
-    }
-  }
+  fact_rows_into(fk, amt, blockIdx.x * (int64_t)GB_NT + threadIdx.x, n, stride, table, mask,
+                 acc + (int64_t)rep * G * 2);
   __syncthreads();
-  for (int g = tid; g < G; g += GB_NT) {
-    int64_t s = 0, c = 0;
-    for (int r = 0; r < R; ++r) {
-      s += acc[((int64_t)r * G + g) * 2];
-      c += acc[((int64_t)r * G + g) * 2 + 1];
-    }
-    if (s) global_add(&sum[g], s);
-    if (c) global_add(&cnt[g], c);
-  }
+  flush_groups(acc, G, R, sum, cnt, GB_NT);
+}
+
+// One step of the serving engine: up to kServeItems (request, row range)
+// items passed by value.  Workgroups [wg_begin[i], wg_begin[i+1]) take item
+// i's rows, `rows_per_wg` each, and add their partial sums to the item's
+// request accumulators (integer atomics: the sums are exact and do not
+// depend on how a request's rows are split over steps and workgroups).
+struct ServeArgs {
+  int32_t n_items;
+  int32_t rows_per_wg;
+  pf_serve_item it[PF_SERVE_MAX_ITEMS];
+  int32_t wg_begin[PF_SERVE_MAX_ITEMS + 1];
+};
+
+__global__ __launch_bounds__(GB_NT) void k_serve_step(const ServeArgs a) {
+  __shared__ int64_t acc[32 * 1024 / 8];
+  const int wg = blockIdx.x;
+  int i = 0;
+  while (i + 1 < a.n_items && a.wg_begin[i + 1] <= wg) ++i;
+  const pf_serve_item& it = a.it[i];
+  const int G = it.G;
+  const int R = replicas_for(G);
+  for (int x = threadIdx.x; x < R * G * 2; x += GB_NT) acc[x] = 0;
+  __syncthreads();
+  const int64_t r0 = (int64_t)(wg - a.wg_begin[i]) * a.rows_per_wg;
+  const int64_t r1 = r0 + a.rows_per_wg < it.n ? r0 + a.rows_per_wg : it.n;
+  const int rep = (int)(lane_id() % R);
+  fact_rows_into(it.fk, it.amt, r0 + threadIdx.x, r1, GB_NT, it.table, (uint32_t)(it.slots - 1),
+                 acc + (int64_t)rep * G * 2);
+  __syncthreads();
+  flush_groups(acc, G, R, it.sum, it.cnt, GB_NT);
 }
 
 unsigned groupby_grid(int64_t n) {
@@ -266,6 +309,41 @@ int pf_fact_join_groupby(const int32_t* fk, const int64_t* amt, int64_t n,
                                                            replicas_for(G), sum, cnt);
   }
   if (launch_check("fact_join_groupby")) return 1;
+  return ok();
+}
+
+int pf_serve_step(const pf_serve_item* items, int32_t n_items, int32_t rows_per_wg,
+                  void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (n_items < 0 || n_items > PF_SERVE_MAX_ITEMS)
+    return fail("serve_step: 0 to PF_SERVE_MAX_ITEMS items");
+  if (rows_per_wg < GB_NT || rows_per_wg > (1 << 24))
+    return fail("serve_step: rows_per_wg in [512, 2^24]");
+  ServeArgs a{};
+  a.n_items = n_items;
+  a.rows_per_wg = rows_per_wg;
+  int64_t wgs = 0;
+  for (int i = 0; i < n_items; ++i) {
+    const pf_serve_item& it = items[i];
+    if (it.n < 0 || it.G < 1 || it.G > GB_LDS_GROUPS || it.slots < 16 ||
+        (it.slots & (it.slots - 1)) || it.slots > (1LL << 32))
+      return fail("serve_step: item " + std::to_string(i) +
+                  ": need n >= 0, 1 <= G <= 1024 and a power-of-two table");
+    if (it.n > 0 && (!it.fk || !it.amt || !it.table || !it.sum || !it.cnt))
+      return fail("serve_step: item " + std::to_string(i) + " has a null pointer");
+    a.it[i] = it;
+    a.wg_begin[i] = (int32_t)wgs;
+    wgs += (it.n + rows_per_wg - 1) / rows_per_wg;
+    if (wgs > (1LL << 30)) return fail("serve_step: too many workgroups");
+  }
+  a.wg_begin[n_items] = (int32_t)wgs;
+  if (wgs == 0) return ok();
+  {
+    KTimer _kt("serve_step", st);
+    k_serve_step<<<(unsigned)wgs, GB_NT, 0, st>>>(a);
+  }
+  if (launch_check("serve_step")) return 1;
   return ok();
 }
 

@@ -28,3 +28,13 @@ def test_library_exports_every_declared_symbol():
 
 def test_every_declared_symbol_has_a_binding():
     assert set(_declared()) == set(_lib.declared_symbols())
+
+
+def test_serve_item_mirrors_the_header():
+    from pageflow_amd import serving
+    text = open(os.path.join(ROOT, "include", "pageflow_join.h")).read()
+    assert int(re.search(r"#define PF_SERVE_MAX_ITEMS (\d+)", text).group(1)) == serving.MAX_ITEMS
+    body = re.search(r"typedef struct pf_serve_item \{(.*?)\} pf_serve_item;", text, re.S).group(1)
+    names = re.findall(r"(\w+);", body)
+    assert names == [f for f, _ in serving.ServeItem._fields_]
+    assert ctypes.sizeof(serving.ServeItem) == 64
