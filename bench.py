@@ -54,17 +54,20 @@ def make_c2(nb: int, np_: int, seed: int, device):
 def kernel_bytes(name: str, nb: int, np_: int, pairs: int) -> float:
     """Algorithmic bytes of one step's launches of a kernel: what the kernel
     must read once and write once for its function (inputs, its own outputs;
-    no table or re-read traffic)."""
+    no table or re-read traffic).  The partition kernels run twice a step
+    (two 6-bit passes): the first reads the int64 key and writes (hash 8 B,
+    row id 4 B), the second reads and writes (hash, row id)."""
     table = {
-        # key read (8 B) per row
-        "hj_probe_hist": np_ * 8, "hj_build_hist": nb * 8,
-        # key read 8 B, (hash 8 B, row id 4 B) written
-        "hj_probe_scatter": np_ * 20, "hj_build_scatter": nb * 20,
+        # pass 1 reads the key, pass 2 the hash: 8 B a row each
+        "hj_probe_hist": np_ * 16, "hj_build_hist": nb * 16,
+        # pass 1: key 8 B in, 12 B out; pass 2: 12 B in, 12 B out
+        "hj_probe_scatter": np_ * 44, "hj_build_scatter": nb * 44,
         # partitioned probe hashes read, build hashes read
         "hj_count": np_ * 8 + nb * 8,
-        # hashes + row ids read, pairs (2 x 8 B) + payload (8 B) written,
-        # payload read (8 B) per pair
-        "hj_write": np_ * 12 + nb * 8 + pairs * 32,
+        # probe (hash, row id) read; build (hash, row id) and the build
+        # payload in partition order read once (staged in LDS per bucket);
+        # pairs (2 x 8 B) and the gathered payload (8 B) written
+        "hj_write": np_ * 12 + nb * 20 + pairs * 24,
     }
     return float(table.get(name, 0))
 
@@ -420,6 +423,7 @@ def main() -> None:
         log("timing the CPU baseline")
         cpu = cpu_baseline(build.cpu(), probe[:args.cpu_sample].cpu(), args.cpu_sample)
 
+    bits = plan.bits
     tiers = {}
     if args.tiers == "all":
         del build, pays, probe, out_p, out_b, out_pay, src, dst, plan, op, ob
@@ -444,7 +448,7 @@ def main() -> None:
         "config": {"workload": "large plain join: build 16M unique int64 keys + 2 int64 "
                                "payloads, probe 640M int64 keys (10% absent); pairs + 1 "
                                "gathered payload",
-                   "build_rows": nb, "probe_rows": np_, "pairs": pairs, "bits": plan.bits},
+                   "build_rows": nb, "probe_rows": np_, "pairs": pairs, "bits": bits},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic},

@@ -89,6 +89,7 @@ class _Active:
     admitted: float
     pos: int = 0
     nbytes: int = 0
+    sent: bool = False       # every row is in a launched step
 
     @property
     def remaining(self) -> int:
@@ -198,7 +199,7 @@ class ServingEngine:
     def _launch_step(self, active: List[_Active], start: int) -> Tuple[List[_Active], int]:
         """One step over the active requests from index `start` on; returns
         the requests whose last rows it adds and the next start."""
-        live = [a for a in active if a.remaining > 0]
+        live = [a for a in active if not a.sent]
         if not live:
             return [], start
         n = len(live)
@@ -222,7 +223,8 @@ class ServingEngine:
             it.cnt = a.acc[1].data_ptr()
             it.G = d.G
             a.pos += take
-            if a.remaining == 0:
+            if a.remaining == 0:  # (an empty request rides along as a 0-row item)
+                a.sent = True
                 finished.append(a)
         for dim_id in waits:
             d = self.dims[dim_id]
@@ -273,7 +275,7 @@ class ServingEngine:
                 pending.popleft()
                 active.append(st)
             launched = False
-            if len(inflight) < self.depth and any(a.remaining > 0 for a in active):
+            if len(inflight) < self.depth and any(not a.sent for a in active):
                 finished, start = self._launch_step(active, start)
                 ev = torch.cuda.Event()
                 ev.record(self.stream)
@@ -290,7 +292,7 @@ class ServingEngine:
                     active.remove(a)
                     self._release(a)
             if not launched and inflight:
-                if len(inflight) >= self.depth or not any(a.remaining > 0 for a in active):
+                if len(inflight) >= self.depth or all(a.sent for a in active):
                     time.sleep(0)  # the device is busy: poll again
             elif not launched and not inflight and nxt < len(arrivals) and not pending:
                 wait = arrivals[nxt].arrival - (time.perf_counter() - t0)
