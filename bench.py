@@ -35,22 +35,6 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_c2(nb: int, np_: int, seed: int, device):
-    import torch
-    g = torch.Generator(device=device).manual_seed(seed)
-    build = torch.randperm(nb, generator=g, device=device) + 1
-    pay0 = torch.randint(-(1 << 62), 1 << 62, (nb,), generator=g, device=device)
-    pay1 = torch.randint(-(1 << 62), 1 << 62, (nb,), generator=g, device=device)
-    idx = torch.randint(0, nb, (np_,), generator=g, device=device)
-    probe = build[idx]
-    del idx
-    absent = torch.rand(np_, generator=g, device=device) < 0.1
-    n_abs = int(absent.sum())
-    probe[absent] = torch.randint(nb + 1, 1 << 62, (n_abs,), generator=g, device=device)
-    del absent
-    return build, (pay0, pay1), probe, np_ - n_abs
-
-
 def kernel_bytes(name: str, nb: int, np_: int, pairs: int) -> float:
     """Algorithmic bytes of one step's launches of a kernel: what the kernel
     must read once and write once for its function (inputs, its own outputs;
@@ -100,10 +84,11 @@ def bench_partition(args, dev) -> None:
     row)."""
     import torch
     from pageflow_amd import _lib, radix
+    from pageflow_amd.workloads import make_large_join
 
     nb, np_ = args.build_rows, args.probe_rows
     log(f"generating {np_} probe keys")
-    _, _, probe, _ = make_c2(nb, np_, 2, dev)
+    _, _, probe, _ = make_large_join(nb, np_, 2, dev)
     rows = torch.arange(np_, dtype=torch.int32, device=dev)
     kout = torch.empty_like(probe)
     vout = torch.empty_like(rows)
@@ -229,6 +214,91 @@ def bench_batch(args, dev, emit: bool = True):
     return res
 
 
+def bench_skew(args, dev, emit: bool = True):
+    """The skewed multi-match join: build 10M rows over 2.5M distinct int64
+    keys (Geometric(0.25) duplicates), probe 400M rows with Zipf(1.1) key
+    ranks and 20% absent keys.  Outputs: the inner-join pairs (partitioned
+    LDS join, count pass then write pass), the semi-join row mask and the
+    anti-join row mask.  Algorithmic bytes: inner 8 nb + 8 np + 16 pairs;
+    each mask 8 nb + 8 np + np."""
+    import torch
+    from pageflow_amd import _lib
+    from pageflow_amd.join import HashJoin
+    from pageflow_amd.operators.join_core import semi_anti_mask
+    from pageflow_amd.workloads import make_skewed_join
+
+    nb, nd, np_ = args.skew_build, args.skew_distinct, args.skew_probe
+    log(f"generating the skewed join: {nb} build rows over {nd} keys, {np_} probe rows")
+    build, probe, want = make_skewed_join(nb, nd, np_, 3, dev)
+    plan = HashJoin(nb, np_, unique=False, device=dev)
+    out_p = out_b = None
+
+    def inner():
+        nonlocal out_p, out_b
+        n = plan.prepare(build, probe)
+        if n < 0:
+            raise RuntimeError("skewed join: a build bucket is too large for the LDS path")
+        if out_p is None or out_p.numel() < n:
+            out_p = torch.empty(n, dtype=torch.int64, device=dev)
+            out_b = torch.empty(n, dtype=torch.int64, device=dev)
+        plan.write(out_p, out_b)
+        return n
+
+    def timed(fn, reps):
+        for _ in range(args.warmup):
+            r = fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, r
+
+    _lib.profile(True)
+    t_inner, pairs = timed(inner, args.steps)
+    t_semi, semi = timed(lambda: semi_anti_mask(build, probe, anti=False), args.steps)
+    t_anti, anti = timed(lambda: semi_anti_mask(build, probe, anti=True), args.steps)
+    prof = _lib.profile_read()
+    _lib.profile(False)
+    # full-size checks: exact pair count, every pair's keys equal, each
+    # probe row's pairs distinct, masks equal to key presence
+    assert pairs == want["pairs"], (pairs, want)
+    op, ob = out_p[:pairs], out_b[:pairs]
+    for a in range(0, pairs, 1 << 27):
+        assert bool((build[ob[a:a + (1 << 27)]] == probe[op[a:a + (1 << 27)]]).all())
+    assert int(torch.unique(op * nb + ob).numel()) == pairs
+    present = probe <= nd
+    assert int(semi.sum()) == want["present"] and bool((semi == present).all())
+    assert bool((anti == ~present).all())
+    log(f"skew checks passed: {pairs} pairs, {want['present']} semi rows")
+    b_inner = 8.0 * nb + 8.0 * np_ + 16.0 * pairs
+    b_mask = 8.0 * nb + 9.0 * np_
+    res = {"build_rows": nb, "distinct_keys": nd, "probe_rows": np_, "pairs": pairs,
+           "semi_rows": want["present"], "bits": plan.bits,
+           "inner_ms": t_inner * 1e3, "semi_ms": t_semi * 1e3, "anti_ms": t_anti * 1e3,
+           "inner_tuples_per_s": (nb + np_) / t_inner,
+           "inner_GBps": b_inner / t_inner / 1e9, "inner_frac": b_inner / t_inner / 8e12,
+           "semi_GBps": b_mask / t_semi / 1e9, "anti_GBps": b_mask / t_anti / 1e9,
+           "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in prof.items()}}
+    log(f"skew: inner {t_inner * 1e3:.2f} ms ({pairs} pairs), semi {t_semi * 1e3:.2f} ms, "
+        f"anti {t_anti * 1e3:.2f} ms")
+    if emit:
+        line = {"metric": "skewed multi-match join tuples/s", "value": (nb + np_) / t_inner,
+                "unit": "tuples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": t_inner * 1e3, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "int64", "data": "synthetic",
+                "config": {"workload": "skewed multi-match join: build 10M rows over 2.5M "
+                                       "int64 keys (geometric duplicates), probe 400M Zipf(1.1) "
+                                       "keys, 20% absent; pairs + semi and anti row masks"},
+                "roofline": {"bound": "hbm", "kernel": "inner join (all kernels)",
+                             "achieved": b_inner / t_inner / 1e9, "peak": HBM_PEAK_GBPS,
+                             "unit": "GB/s", "frac": b_inner / t_inner / 8e12, "traffic": None},
+                "skew": res}
+        print(json.dumps(line), flush=True)
+    del out_p, out_b, plan
+    return res
+
+
 def bench_serve(args, dev, emit: bool = True):
     """The serving workload: a Poisson stream of fact->dimension join +
     group-by requests (fact rows log-uniform 100K-50M, 8 recurring
@@ -316,8 +386,11 @@ def main() -> None:
     ap.add_argument("--bits", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["join", "partition", "batch", "serve"],
+    ap.add_argument("--workload", choices=["join", "partition", "batch", "serve", "skew"],
                     default="join")
+    ap.add_argument("--skew-build", type=int, default=10_000_000)
+    ap.add_argument("--skew-distinct", type=int, default=2_500_000)
+    ap.add_argument("--skew-probe", type=int, default=400_000_000)
     ap.add_argument("--tiers", choices=["all", "none"], default="all",
                     help="with the join workload: also run the batch and serving tiers and "
                          "add their results to the line")
@@ -337,6 +410,7 @@ def main() -> None:
     import torch
     from pageflow_amd import _lib
     from pageflow_amd.join import HashJoin
+    from pageflow_amd.workloads import make_large_join
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -349,9 +423,12 @@ def main() -> None:
     if args.workload == "serve":
         bench_serve(args, dev)
         return
+    if args.workload == "skew":
+        bench_skew(args, dev)
+        return
     nb, np_ = args.build_rows, args.probe_rows
     log(f"generating {nb} build / {np_} probe rows")
-    build, pays, probe, expect = make_c2(nb, np_, 2, dev)
+    build, pays, probe, expect = make_large_join(nb, np_, 2, dev)
     torch.cuda.synchronize()
     plan = HashJoin(nb, np_, unique=True, bits=args.bits, device=dev)
     out_p = torch.empty(np_, dtype=torch.int64, device=dev)
@@ -427,6 +504,8 @@ def main() -> None:
     tiers = {}
     if args.tiers == "all":
         del build, pays, probe, out_p, out_b, out_pay, src, dst, plan, op, ob
+        torch.cuda.empty_cache()
+        tiers["skew"] = bench_skew(args, dev, emit=False)
         torch.cuda.empty_cache()
         tiers["batch"] = bench_batch(args, dev, emit=False)
         torch.cuda.empty_cache()

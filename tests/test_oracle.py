@@ -241,3 +241,12 @@ def test_gather_with_null():
     src = np.array([10, 11, 12], np.int64)
     assert oracle.gather_with_null(src, [2, -64, 0]).tolist() == [12, -64, 10]
     assert oracle.gather_with_null(np.array([], np.int64), [-64, -64]).tolist() == [-64, -64]
+
+
+def test_geometric_duplicates_reach_the_exact_row_count():
+    from pageflow_amd.workloads import geometric_duplicates
+    for nd, nb, seed in ((2500, 10_000, 3), (10_000, 10_000, 1), (1000, 9000, 7), (3, 50, 2)):
+        c = geometric_duplicates(nd, nb, 0.25, np.random.default_rng(seed))
+        assert c.sum() == nb and c.min() >= 1 and c.size == nd
+        again = geometric_duplicates(nd, nb, 0.25, np.random.default_rng(seed))
+        assert np.array_equal(c, again)
