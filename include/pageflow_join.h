@@ -153,8 +153,11 @@ int pf_radix_partition(const void* keys, int32_t key_bytes, const int32_t* vals,
                        int32_t* vals_out, int64_t* bucket_offsets, void* workspace,
                        size_t workspace_bytes, void* stream);
 /* The same with the scatter's tile shape chosen by the caller (the
- * autotuner): for bits <= 8, shape 0..3 = (256 threads x 8 rows, 256 x 16,
- * 512 x 8, 512 x 16) a tile; -1 the default.  Ignored above 8 bits. */
+ * autotuner): shape 0..3 = (256 threads x 8 rows, 256 x 16, 512 x 8,
+ * 512 x 16) a tile for digits of up to 8 bits; -1 the default.  Above 8
+ * bits the partition runs as two stable passes (the low half of the digit,
+ * then the high half, each with that tile shape) and a bucket-bounds
+ * kernel, unless shape is 4: one pass of the wide-digit scatter. */
 int pf_radix_partition_shape(const void* keys, int32_t key_bytes, const int32_t* vals,
                              int64_t n, int32_t bits, int32_t shift, int32_t mode,
                              void* keys_out, int32_t* vals_out, int64_t* bucket_offsets,

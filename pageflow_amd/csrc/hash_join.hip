@@ -51,6 +51,9 @@ constexpr int PCH = 262144;                // probe rows a work item
 constexpr int MAX_PAY = 2;
 constexpr int kOnePassBits = 8;            // wider partitions take two passes
 
+// the widest digit one partition pass of partition_side takes
+int pass_bits(int bits) { return bits <= kOnePassBits ? bits : bits - bits / 2; }
+
 struct HjParams {
   const uint64_t* bh;     // build (hash, row) partitioned [nb]
   const int32_t* brow;
@@ -442,7 +445,7 @@ HjWs carve_hj(void* ws, int64_t nb, int64_t np, int bits, bool multi) {
   const int64_t mi = max_items(np, bits);
   w.item_cnt = cv.take<int64_t>(mi);
   for (int c = 0; c < MAX_PAY; ++c) w.pay_part[c] = cv.take<int64_t>(nb > 0 ? nb : 1);
-  const size_t pws = radix_partition_ws(nb > np ? nb : np, bits);
+  const size_t pws = radix_partition_ws(nb > np ? nb : np, pass_bits(bits));
   w.part_ws = cv.take<char>(pws);
   w.scan = cv.take<char>(scan_i64_scratch(mi));
   w.bytes = cv.bytes();
@@ -513,7 +516,7 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
   if (host_alias(host_out, &hdev)) return 1;
   const int P = 1 << bits;
   const int cap = multi ? CAP_M : CAP_U;
-  const size_t pws = radix_partition_ws(nb > np ? nb : np, bits);
+  const size_t pws = radix_partition_ws(nb > np ? nb : np, pass_bits(bits));
   // both sides as (mix64(key), row) by the top `bits` bits of the hash
   if (partition_side(bkeys, nb, bits, w.bh, w.brow, w.boff, w.th, w.trow, w.part_ws, pws, st, 1))
     return 1;

@@ -29,6 +29,12 @@
 #include "../../include/pageflow_join.h"
 
 namespace pf {
+
+int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
+                    int shift, int mode, void* keys_out, int32_t* vals_out,
+                    int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st,
+                    int who, int shape);
+
 namespace {
 
 enum RpMode : int { RP_HASH_KEY = 0, RP_HASH_OUT = 1, RP_RAW = 2 };
@@ -461,10 +467,104 @@ int rp_dispatch(const K* keys, const int32_t* vals, int64_t n, int bits, int shi
                                      bucket_offsets, r, st, who, shape);
 }
 
+// Bucket bounds of a partitioned output by binary search on the full
+// digit: bucket b's first row (b = 2^bits: n).  KO is the output key type;
+// HASH_OUT outputs hold mix64 already.
+template <typename KO, int MODE>
+__global__ void rp_bounds(const KO* __restrict__ kout, int64_t n, int bits, int shift,
+                          int64_t* __restrict__ offs) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > (1 << bits)) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    uint32_t d;
+    if (MODE == RP_HASH_OUT) {
+      const uint64_t h = (uint64_t)kout[mid];
+      d = bits ? (uint32_t)((h << shift) >> (64 - bits)) : 0u;
+    } else {
+      KO unused;
+      d = rp_digit<KO, KO, MODE>(kout[mid], bits, shift, unused);
+    }
+    if ((int)d < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  offs[b] = lo;
+}
+
+constexpr int kOnePassMax = 8;   // wider digits: two stable passes by default
+constexpr int kShapeOnePass = 4; // shape: one pass whatever the width
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t two_pass_ws(int64_t n, int bits) {
+  const int hi = bits / 2, lo = bits - hi;
+  size_t a = carve_rp(nullptr, n, lo).bytes, b = carve_rp(nullptr, n, hi).bytes;
+  return align256(a > b ? a : b) + align256((size_t)n * 8) + align256((size_t)n * 4);
+}
+
+template <typename KO, int MODE>
+int launch_bounds_of(const void* kout, int64_t n, int bits, int shift, int64_t* offs,
+                     hipStream_t st) {
+  rp_bounds<KO, MODE><<<grid_for((1 << bits) + 1, 256), 256, 0, st>>>((const KO*)kout, n, bits,
+                                                                        shift, offs);
+  return launch_check("radix_partition bounds");
+}
+
+// Two stable LSD passes, the low half of the digit first: the output is
+// ordered by the whole digit with each bucket's rows in input order, as
+// one pass would leave it, at two well-coalesced scatters instead of one
+// scatter over 2^bits runs.
+int two_pass(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
+             int shift, int mode, void* keys_out, int32_t* vals_out, int64_t* offs, void* ws,
+             hipStream_t st, int who, int shape) {
+  const int hi = bits / 2, lo = bits - hi;
+  const size_t wsb = two_pass_ws(n, bits);
+  char* base = (char*)ws;
+  const size_t hb = wsb - align256((size_t)n * 8) - align256((size_t)n * 4);
+  void* tk = base + hb;
+  int32_t* tv = (int32_t*)(base + hb + align256((size_t)n * 8));
+  const int sh = shape >= 0 && shape < kShapeOnePass ? shape : -1;
+  if (mode == RP_RAW) {
+    if (radix_partition(keys, key_bytes, vals, n, lo, shift, RP_RAW, tk, tv, offs, ws, hb, st,
+                        who, sh) ||
+        radix_partition(tk, key_bytes, tv, n, hi, shift + lo, RP_RAW, keys_out, vals_out, offs,
+                        ws, hb, st, who, sh))
+      return 1;
+    return key_bytes == 8
+               ? launch_bounds_of<int64_t, RP_RAW>(keys_out, n, bits, shift, offs, st)
+               : launch_bounds_of<int32_t, RP_RAW>(keys_out, n, bits, shift, offs, st);
+  }
+  if (mode == RP_HASH_KEY) {
+    if (radix_partition(keys, key_bytes, vals, n, lo, shift + hi, RP_HASH_KEY, tk, tv, offs, ws,
+                        hb, st, who, sh) ||
+        radix_partition(tk, key_bytes, tv, n, hi, shift, RP_HASH_KEY, keys_out, vals_out, offs,
+                        ws, hb, st, who, sh))
+      return 1;
+    return key_bytes == 8
+               ? launch_bounds_of<int64_t, RP_HASH_KEY>(keys_out, n, bits, shift, offs, st)
+               : launch_bounds_of<int32_t, RP_HASH_KEY>(keys_out, n, bits, shift, offs, st);
+  }
+  // HASH_OUT: the first pass writes the hashes, the second reads their bits
+  if (radix_partition(keys, key_bytes, vals, n, lo, shift + hi, RP_HASH_OUT, tk, tv, offs, ws,
+                      hb, st, who, sh) ||
+      radix_partition(tk, 8, tv, n, hi, 64 - shift - hi, RP_RAW, keys_out, vals_out, offs, ws,
+                      hb, st, who, sh))
+    return 1;
+  return launch_bounds_of<int64_t, RP_HASH_OUT>(keys_out, n, bits, shift, offs, st);
+}
+
 }  // namespace
 
 // Host-side use by the other tiers of the library.
-size_t radix_partition_ws(int64_t n, int bits) { return carve_rp(nullptr, n, bits).bytes; }
+size_t radix_partition_ws(int64_t n, int bits) {
+  const size_t one = carve_rp(nullptr, n, bits).bytes;
+  if (bits <= kOnePassMax) return one;
+  const size_t two = two_pass_ws(n, bits);
+  return one > two ? one : two;
+}
 
 int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
                     int shift, int mode, void* keys_out, int32_t* vals_out,
@@ -478,6 +578,11 @@ int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_
   if (shift < 0 || shift + bits > kbits) return fail("radix_partition: shift + bits too large");
   if (mode == RP_HASH_OUT && key_bytes != 8 && keys_out == keys)
     return fail("radix_partition: a hashed output cannot overwrite int32 keys");
+  if (bits > kOnePassMax && shape != kShapeOnePass) {
+    if (ws_bytes < two_pass_ws(n, bits)) return fail("radix_partition: workspace too small");
+    return two_pass(keys, key_bytes, vals, n, bits, shift, mode, keys_out, vals_out,
+                    bucket_offsets, ws, st, who, shape);
+  }
   RpWs r = carve_rp(ws, n, bits);
   if (ws_bytes < r.bytes) return fail("radix_partition: workspace too small");
   if (key_bytes == 8)

@@ -21,13 +21,13 @@ def _keys(n, dtype, seed, skew=False):
     return torch.randint(-hi, hi, (n,), dtype=dtype, generator=g)
 
 
-def _check(keys, bits, mode, shift=0, with_vals=True):
+def _check(keys, bits, mode, shift=0, with_vals=True, shape=-1):
     n = keys.numel()
     vals = torch.randperm(n, generator=torch.Generator().manual_seed(n)).to(torch.int32) \
         if with_vals else None
     ko, vo, offs = radix.radix_partition(keys.cuda(), bits,
                                          vals.cuda() if vals is not None else None,
-                                         shift=shift, mode=mode)
+                                         shift=shift, mode=mode, shape=shape)
     v_np = vals.numpy() if vals is not None else np.arange(n, dtype=np.int32)
     wk, wv, counts = oracle.radix_partition(keys.numpy(), v_np, bits, shift,
                                             hashed=mode != radix.RAW)
@@ -87,3 +87,18 @@ def test_every_scatter_shape(shape, bits):
     assert np.array_equal(ko.cpu().numpy(), oracle.mix64(wk).view(np.int64))
     assert np.array_equal(vo.cpu().numpy(), wv)
     assert np.array_equal(offs.cpu().numpy(), np.concatenate(([0], np.cumsum(counts))))
+
+
+@pytest.mark.parametrize("shape", [-1, 4, 1])
+@pytest.mark.parametrize("mode,shift,bits", [(0, 0, 9), (0, 3, 11), (1, 0, 12), (1, 5, 10),
+                                             (2, 0, 9), (2, 7, 12)])
+@pytest.mark.parametrize("dtype", [torch.int64, torch.int32])
+def test_wide_digits_two_passes_and_one(shape, mode, shift, bits, dtype):
+    # above 8 bits: two stable passes (default or a tile shape) or one (shape 4)
+    _check(_keys(400_003, dtype, bits + shift), bits, mode, shift=shift, shape=shape)
+
+
+def test_wide_digit_skew_and_tiny():
+    _check(_keys(100_000, torch.int64, 1, skew=True), 12, radix.HASH_OUT)
+    for n in (0, 1, 5):
+        _check(_keys(n, torch.int32, 2), 10, radix.HASH_KEY)
