@@ -22,7 +22,7 @@ from typing import Optional, Sequence, Tuple
 
 import torch
 
-from .. import _lib
+from .. import _lib, tqpkernels
 
 # Rows a call takes on a side (operators/join_core.py:30).
 MAX_ROWS = (1 << 31) - (1 << 16)
@@ -56,32 +56,7 @@ def expand(sorted_ids: torch.Tensor, start: torch.Tensor,
     order; ``start[i]``, ``length[i]``: probe row i's run in it (0: no
     match).  Returns (build rows, probe rows) in probe-row order and, within
     a probe row, run order."""
-    for t, what in ((sorted_ids, "sorted_ids"), (start, "start"), (length, "len")):
-        _lib.require_cuda(t, what)
-        if t.dtype != torch.int64 or t.dim() != 1 or not t.is_contiguous():
-            raise RuntimeError(f"join_core: {what} must be a contiguous 1-D int64 tensor")
-    if start.numel() != length.numel():
-        raise RuntimeError("join_core: start and len differ in length")
-    device = sorted_ids.device
-    m = length.numel()
-    if m == 0:
-        return _empty(device), _empty(device)
-    lib = _lib.load()
-    host = _host_buffer()
-    ws_bytes = lib.pf_join_expand_workspace_bytes(m)
-    ws = _lib.workspace(ws_bytes, device)
-    st = _lib.stream_of(device)
-    with torch.cuda.device(device):
-        _lib.check(lib.pf_join_expand_count(_lib.ptr(length), m, _hp(host), _lib.ptr(ws),
-                                            ws_bytes, st))
-        n_flat = int(host[0])
-        cs = torch.empty(n_flat, dtype=torch.int64, device=device)
-        cr = torch.empty(n_flat, dtype=torch.int64, device=device)
-        if n_flat > 0:
-            _lib.check(lib.pf_join_expand_write(_lib.ptr(sorted_ids), _lib.ptr(start), m,
-                                                n_flat, _lib.ptr(cs), _lib.ptr(cr),
-                                                _lib.ptr(ws), st))
-    return cs, cr
+    return tuple(tqpkernels.join_expand(sorted_ids, start, length, _host_buffer()))
 
 
 def unique_hash_peak_bytes(n_build: int, n_probe: int) -> int:
@@ -123,20 +98,8 @@ def unique_hash_pairs(build_keys: Sequence[torch.Tensor],
     nb, np_ = build_keys[0].numel(), probe_keys[0].numel()
     if nb == 0 or np_ == 0:
         return _empty(device), _empty(device)
-    two = len(build_keys) == 2
-    lib = _lib.load()
     host = _host_buffer()
-    ws_bytes = lib.pf_join_unique_hash_workspace_bytes(nb, np_, 2 if two else 1)
-    ws = _lib.workspace(ws_bytes, device)
-    build_idx = torch.empty(np_, dtype=torch.int64, device=device)
-    probe_idx = torch.empty(np_, dtype=torch.int64, device=device)
-    b1 = build_keys[1] if two else build_keys[0]
-    p1 = probe_keys[1] if two else probe_keys[0]
-    with torch.cuda.device(device):
-        _lib.check(lib.pf_join_unique_hash(
-            _lib.ptr(build_keys[0]), _lib.ptr(b1), nb, _lib.ptr(probe_keys[0]), _lib.ptr(p1),
-            np_, 2 if two else 1, _lib.ptr(build_idx), _lib.ptr(probe_idx), _hp(host),
-            _lib.ptr(ws), ws_bytes, _lib.stream_of(device)))
+    build_idx, probe_idx = tqpkernels.join_unique_hash(list(build_keys), list(probe_keys), host)
     n = int(host[0])
     return build_idx[:n], probe_idx[:n]
 
@@ -177,16 +140,8 @@ def semi_anti(build_key: torch.Tensor, probe_key: torch.Tensor, anti: bool,
         _check_column(probe_val, "the probe value", build_key)
         if build_val.numel() != n_build or probe_val.numel() != n_probe:
             raise RuntimeError("join_core: a value column differs in length from its key")
-    lib = _lib.load()
     host = _host_buffer()
-    ws_bytes = lib.pf_join_semi_anti_workspace_bytes(n_build, n_probe, op)
-    ws = _lib.workspace(ws_bytes, device)
-    out = torch.empty(n_probe, dtype=torch.int64, device=device)
-    with torch.cuda.device(device):
-        _lib.check(lib.pf_join_semi_anti(
-            _lib.ptr(build_key), n_build, _lib.ptr(probe_key), n_probe, _lib.ptr(build_val),
-            _lib.ptr(probe_val), op, int(bool(anti)), _lib.ptr(out), _hp(host),
-            _lib.ptr(ws), ws_bytes, _lib.stream_of(device)))
+    out = tqpkernels.join_semi_anti(build_key, probe_key, build_val, probe_val, op, anti, host)
     n, bad = host.tolist()
     if bad:
         return None

@@ -17,13 +17,12 @@ tiles walk the pre-mask rows with the mask as a predicate.
 """
 from __future__ import annotations
 
-import ctypes
 from typing import List, Optional
 
 import torch
 
-from .. import _lib
-from ..layout import PartitionedTensor, row_bytes, rows_per_unit
+from .. import _lib, tqpkernels
+from ..layout import PartitionedTensor, row_bytes
 from .partitions import JOIN_B_MAX as _B_MAX
 
 # Kept rows per mask row below which select-first runs (every column
@@ -102,30 +101,9 @@ def partition(cols: List[torch.Tensor], key_pos: int, mask: Optional[torch.Tenso
         _lib.require_cuda(t, "a partition column")
     if mask is not None:
         _lib.require_cuda(mask, "the mask")
-    device = key.device
-    lib = _lib.load()
-    cols = [t.contiguous() for t in cols]
-    outs = []
-    for t in cols:
-        shape = list(t.shape)
-        shape[0] = n_valid + B * (rows_per_unit(row_bytes(t)) - 1)
-        outs.append(torch.empty(shape, dtype=t.dtype, device=device))
-    gidx = torch.empty(n_valid + B * (rows_per_unit(8) - 1), dtype=torch.int64, device=device)
     host = _host_buffer()
-    ws_bytes = lib.pf_partition_sort_workspace_bytes(n_pre, n_valid, B, int(mask is not None),
-                                                     int(select_first), int(any(comp)))
-    ws = _lib.workspace(ws_bytes, device)
-    rbs = _lib.i32_array([row_bytes(t) for t in cols])
-    cps = _lib.i32_array(comp)
-    with torch.cuda.device(device):
-        _lib.check(lib.pf_partition_sort(
-            _lib.void_array([t.data_ptr() for t in cols]),
-            _lib.void_array([o.data_ptr() for o in outs]),
-            ctypes.cast(rbs, ctypes.c_void_p), ctypes.cast(cps, ctypes.c_void_p),
-            len(cols), key_pos, _lib.ptr(mask), n_pre, n_valid, goff, B,
-            int(select_first), _lib.ptr(gidx), ctypes.c_void_p(host.data_ptr()),
-            _lib.ptr(ws), ws_bytes, _lib.stream_of(device)))
-    outs.append(gidx)
+    outs = tqpkernels.partition_sort([t.contiguous() for t in cols], comp, key_pos, mask,
+                                     n_valid, goff, B, select_first, host)
     counts = host[:B].tolist()
     totals = host[B:B + len(widths)].tolist()
     kept = sum(counts)

@@ -38,3 +38,16 @@ def test_serve_item_mirrors_the_header():
     names = re.findall(r"(\w+);", body)
     assert names == [f for f, _ in serving.ServeItem._fields_]
     assert ctypes.sizeof(serving.ServeItem) == 64
+
+
+def test_extension_module_has_the_reference_functions():
+    import inspect
+    from pageflow_amd import tqpkernels
+    want = {"partition_sort": ["cols", "comp", "key_idx", "mask", "n_valid", "goff", "B",
+                               "select_first", "host_out"],
+            "join_expand": ["sorted_ids", "start", "length", "host_out"],
+            "join_unique_hash": ["build_keys", "probe_keys", "host_out"],
+            "join_semi_anti": ["build_key", "probe_key", "build_val", "probe_val", "op", "anti",
+                               "host_out"]}
+    for name, params in want.items():
+        assert list(inspect.signature(getattr(tqpkernels, name)).parameters) == params
