@@ -330,18 +330,51 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
         cnt[lane] = wave_incl_scan(v) - v;
       }
       __syncthreads();
+      if (MULTI) {
+        // A wave writes its (round slot)'s pairs as one contiguous run: lane
+        // j of each store takes output j of the run, found by a binary
+        // search of the lanes' offsets staged in LDS, so the stores are
+        // coalesced whatever the run lengths (skewed keys fan out).
+        __shared__ int32_t s_ex[MULTI ? HJ_NT : 1], s_hi[MULTI ? HJ_NT : 1],
+            s_pr[MULTI ? HJ_NT : 1];
+        const int wb = w * kWave;
 #pragma unroll
-      for (int k = 0; k < HJ_IT; ++k) {
-        if (m[k] == 0) continue;
-        const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
-        const int64_t prow = p.prow[r];
-        int64_t o = pos + cnt[k * HJ_W + w] + inw[k];
-        for (int q = 0; q < m[k]; ++q, ++o) {
-          const int li = hidx[k] + q;
-          p.out_probe[o] = prow;
-          p.out_build[o] = rows[li];
-          if (has_pay) p.pay_out[0][o] = pay0[li];
-          if (p.npay > 1) p.pay_out[1][o] = p.pay_src[1][cb + li];
+        for (int k = 0; k < HJ_IT; ++k) {
+          const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
+          s_ex[tid] = (int32_t)inw[k];
+          s_hi[tid] = hidx[k];
+          s_pr[tid] = m[k] ? p.prow[r] : 0;
+          const int32_t T = (int32_t)__shfl(inw[k] + m[k], kWave - 1, kWave);
+          __syncthreads();
+          const int64_t base = pos + cnt[k * HJ_W + w];
+          for (int j = lane; j < T; j += kWave) {
+            int lo = 0, hi = kWave;  // the last lane whose run starts at or before j
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (s_ex[wb + mid] <= j)
+                lo = mid;
+              else
+                hi = mid;
+            }
+            const int li = s_hi[wb + lo] + (j - s_ex[wb + lo]);
+            const int64_t o = base + j;
+            p.out_probe[o] = s_pr[wb + lo];
+            p.out_build[o] = rows[li];
+            if (has_pay) p.pay_out[0][o] = pay0[li];
+            if (p.npay > 1) p.pay_out[1][o] = p.pay_src[1][cb + li];
+          }
+          __syncthreads();  // s_* are rewritten for the next slot
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < HJ_IT; ++k) {
+          if (m[k] == 0) continue;
+          const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
+          const int64_t o = pos + cnt[k * HJ_W + w] + inw[k];
+          p.out_probe[o] = p.prow[r];
+          p.out_build[o] = rows[hidx[k]];
+          if (has_pay) p.pay_out[0][o] = pay0[hidx[k]];
+          if (p.npay > 1) p.pay_out[1][o] = p.pay_src[1][cb + hidx[k]];
         }
       }
       // the round's last (slot, wave) total, from the wave that holds it
