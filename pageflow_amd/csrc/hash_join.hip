@@ -203,8 +203,22 @@ struct Lds<true> {
   static constexpr int CAP = CAP_M, TS = TS_M;
 };
 
+// The LDS table's entry of hash h: its first row in the chunk, or -1.
+// Slots hold row + 1 (0 = empty), linear probing from the hash's low bits.
+template <int TS>
+__device__ __forceinline__ int lds_find(const uint64_t* keys, const uint32_t* slot, uint64_t h) {
+  uint32_t s = (uint32_t)h & (TS - 1);
+  uint32_t v = slot[s];
+  while (v != 0u) {
+    if (keys[v - 1] == h) return (int)v - 1;
+    s = (s + 1) & (TS - 1);
+    v = slot[s];
+  }
+  return -1;
+}
+
 // MODE 0: count pairs per item; MODE 1: write them.  Probe rows are taken
-// in rounds of HJ_ROUND, row r0 + k * HJ_NT + tid in slot k of a thread, so a
+// in rounds of HJ_ROUND, item row i0 + k * HJ_NT + tid in slot k of a thread, so a
 // wave's lanes touch consecutive rows: loads coalesce and, in the write
 // pass, consecutive matches land on consecutive output rows.  A match's
 // build row and first payload come from LDS (staged with the chunk).
@@ -263,44 +277,32 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
     }
     __syncthreads();
 
-    for (int64_t r0 = it.ps; r0 < it.pe; r0 += ROUND) {
+    // item-local 32-bit row indices (an item has at most PCH rows); the
+    // loads clamp to the last row so no lane branches around them
+    const uint64_t* __restrict__ ph = p.ph + it.ps;
+    const int32_t* __restrict__ prw = p.prow + it.ps;
+    const int np_item = (int)(it.pe - it.ps);
+    for (int i0 = 0; i0 < np_item; i0 += ROUND) {
       int hidx[IT];
       int m[IT];
       uint64_t hv[IT];
-      // every slot's load issued before any lookup waits on one
 #pragma unroll
       for (int k = 0; k < IT; ++k) {
-        const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
-        hv[k] = r < it.pe ? p.ph[r] : 0;
+        const int i = i0 + k * HJ_NT + tid;
+        hv[k] = ph[i < np_item ? i : np_item - 1];
       }
 #pragma unroll
       for (int k = 0; k < IT; ++k) {
-        const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
-        hidx[k] = -1;
-        m[k] = 0;
-        if (r < it.pe) {
-          const uint64_t h = hv[k];
-          uint32_t s = (uint32_t)h & (TS - 1);
-          for (;;) {
-            const uint32_t v = slot[s];
-            if (v == 0) break;
-            if (keys[v - 1] == h) {
-              hidx[k] = (int)v - 1;
-              break;
-            }
-            s = (s + 1) & (TS - 1);
-          }
-          if (hidx[k] >= 0) {
-            if (MULTI) {
-              int len = glen[hidx[k]];
-              if (len == 65535)  // a run longer than glen's range, counted to its end
-                while (hidx[k] + len < nloc && keys[hidx[k] + len] == h) ++len;
-              m[k] = len;
-            } else {
-              m[k] = 1;
-            }
-          }
+        const int i = i0 + k * HJ_NT + tid;
+        const int f = lds_find<TS>(keys, slot, hv[k]);
+        hidx[k] = i < np_item ? f : -1;
+        int len = hidx[k] >= 0 ? 1 : 0;
+        if (MULTI && hidx[k] >= 0) {
+          len = glen[hidx[k]];
+          if (len == 65535)  // a run longer than glen's range, counted to its end
+            while (hidx[k] + len < nloc && keys[hidx[k] + len] == hv[k]) ++len;
         }
+        m[k] = len;
       }
       if (!W1) {
 #pragma unroll
@@ -340,10 +342,10 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
         const int wb = w * kWave;
 #pragma unroll
         for (int k = 0; k < HJ_IT; ++k) {
-          const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
+          const int i = i0 + k * HJ_NT + tid;
           s_ex[tid] = (int32_t)inw[k];
           s_hi[tid] = hidx[k];
-          s_pr[tid] = m[k] ? p.prow[r] : 0;
+          s_pr[tid] = m[k] ? prw[i] : 0;
           const int32_t T = (int32_t)__shfl(inw[k] + m[k], kWave - 1, kWave);
           __syncthreads();
           const int64_t base = pos + cnt[k * HJ_W + w];
@@ -369,9 +371,9 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
 #pragma unroll
         for (int k = 0; k < HJ_IT; ++k) {
           if (m[k] == 0) continue;
-          const int64_t r = r0 + (int64_t)k * HJ_NT + tid;
+          const int i = i0 + k * HJ_NT + tid;
           const int64_t o = pos + cnt[k * HJ_W + w] + inw[k];
-          p.out_probe[o] = p.prow[r];
+          p.out_probe[o] = prw[i];
           p.out_build[o] = rows[hidx[k]];
           if (has_pay) p.pay_out[0][o] = pay0[hidx[k]];
           if (p.npay > 1) p.pay_out[1][o] = p.pay_src[1][cb + hidx[k]];
