@@ -39,7 +39,7 @@ namespace {
 
 constexpr int HJ_NT = 1024;
 constexpr int HJ_W = HJ_NT / kWave;
-constexpr int HJ_IT = 4;                   // probe rows a thread per round
+constexpr int HJ_IT = 8;                   // write pass: probe rows a thread per round
 constexpr int HJ_ROUND = HJ_NT * HJ_IT;
 constexpr int CAP_U = 5120;                // unique: build rows an LDS chunk
 constexpr int TS_U = 8192;                 // unique: table slots
@@ -222,14 +222,15 @@ __device__ __forceinline__ int lds_find(const uint64_t* keys, const uint32_t* sl
 // wave's lanes touch consecutive rows: loads coalesce and, in the write
 // pass, consecutive matches land on consecutive output rows.  A match's
 // build row and first payload come from LDS (staged with the chunk).
-static_assert(HJ_IT * HJ_W == kWave, "one wave scans a round's (slot, wave) totals");
+static_assert(HJ_IT * HJ_W % kWave == 0, "one wave scans a round's (slot, wave) totals");
 
 template <bool MULTI, int MODE>
 __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
   constexpr int CAP = Lds<MULTI>::CAP, TS = Lds<MULTI>::TS;
   constexpr bool W1 = MODE == 1;
-  // the count pass has no write-out bookkeeping: twice the rows in flight
-  constexpr int IT = W1 ? HJ_IT : 2 * HJ_IT;
+  // rows a thread per round (the count pass stays within 64 VGPRs: two
+  // 1024-thread workgroups a CU)
+  constexpr int IT = W1 ? HJ_IT : 8;
   constexpr int ROUND = HJ_NT * IT;
   __shared__ uint64_t keys[CAP];
   __shared__ uint32_t slot[TS];
@@ -327,9 +328,20 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
         if (lane == 0) cnt[k * HJ_W + w] = tot;
       }
       __syncthreads();
-      if (w == 0) {
-        const int64_t v = cnt[lane];
-        cnt[lane] = wave_incl_scan(v) - v;
+      if (w == 0) {  // PER consecutive (slot, wave) totals a lane
+        constexpr int PER = HJ_IT * HJ_W / kWave;
+        int64_t v[PER], sum = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          v[q] = cnt[lane * PER + q];
+          sum += v[q];
+        }
+        int64_t ex = wave_incl_scan(sum) - sum;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          cnt[lane * PER + q] = ex;
+          ex += v[q];
+        }
       }
       __syncthreads();
       if (MULTI) {
