@@ -39,7 +39,7 @@ namespace {
 
 constexpr int HJ_NT = 1024;
 constexpr int HJ_W = HJ_NT / kWave;
-constexpr int HJ_IT = 8;                   // write pass: probe rows a thread per round
+constexpr int HJ_IT = 12;                  // write pass: probe rows a thread per round
 constexpr int HJ_ROUND = HJ_NT * HJ_IT;
 constexpr int CAP_U = 5120;                // unique: build rows an LDS chunk
 constexpr int TS_U = 8192;                 // unique: table slots
