@@ -143,9 +143,13 @@ def bench_partition(args, dev) -> None:
 
 def bench_batch(args, dev, emit: bool = True):
     """256 uneven inner joins (int32 keys): the batched pipeline (autotuned,
-    replayed from a hipGraph) against the same kernels launched one join at
-    a time with one host wait a join.  tuples/s = (build + probe rows) /
-    time; algorithmic bytes = sum over joins of 4 nb + 4 np + 16 matches."""
+    replayed from a hipGraph) against two per-join baselines: the plain
+    per-join launch of the reference's path (one unique-hash join call a
+    join, int64 keys, one host wait a call) and, as an ablation, the batch
+    kernels run one join at a time with one host wait a join.  tuples/s =
+    (build + probe rows) / time; algorithmic bytes = sum over joins of
+    4 nb + 4 np + 16 matches."""
+    import numpy as np
     import torch
     from pageflow_amd import _lib
     from pageflow_amd.batch import BatchJoin, autotune, make_batch
@@ -166,8 +170,34 @@ def bench_batch(args, dev, emit: bool = True):
         t_plain.append(time.perf_counter() - t0)
     t_plain = sorted(t_plain[args.warmup:])[len(t_plain[args.warmup:]) // 2]
     assert (base.counts.cpu().numpy() == expect).all()
+    # the plain per-join launch of the reference's path: one unique-hash
+    # join call (join_unique_hash, int64 keys, one host wait) a join; the
+    # keys are widened to int64 before the timed region
+    from pageflow_amd.operators.join_core import unique_hash_pairs
+    bo = np.concatenate(([0], np.cumsum(nb)))
+    po = np.concatenate(([0], np.cumsum(np_)))
+    b64, p64 = bkeys.to(torch.int64), pkeys.to(torch.int64)
+    sides = [(b64[bo[j]:bo[j + 1]], p64[po[j]:po[j + 1]]) for j in range(len(nb))]
+    t_uh = []
+    for _ in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for b_j, p_j in sides:
+            unique_hash_pairs([b_j], [p_j])
+        torch.cuda.synchronize()
+        t_uh.append(time.perf_counter() - t0)
+    t_uh = sorted(t_uh[args.warmup:])[len(t_uh[args.warmup:]) // 2]
+    checked = [int(np.argmin(nb)), int(np.argsort(nb)[len(nb) // 2]), int(np.argmax(nb))]
+    plain_pairs = {j: unique_hash_pairs([sides[j][0]], [sides[j][1]]) for j in checked}
+    del b64, p64, sides
     bj = BatchJoin(nb, np_, tune, dev)
     st = bj.capture(bkeys, pkeys)
+    bj.replay()
+    st.synchronize()
+    for j, (bi, pi) in plain_pairs.items():  # the same pairs, in probe-row order
+        op, ob = bj.pairs(j)
+        assert torch.equal(op, pi) and torch.equal(ob, bi), j
+    del plain_pairs
     for _ in range(args.warmup):
         bj.replay()
     st.synchronize()
@@ -190,12 +220,17 @@ def bench_batch(args, dev, emit: bool = True):
     res = {"joins": int(len(nb)), "build_rows": int(nb.sum()), "probe_rows": int(np_.sum()),
            "pairs": pairs, "tune": [list(t) for t in tune],
            "batched_ms": t_batch * 1e3, "per_join_ms": t_plain * 1e3,
+           "plain_unique_hash_ms": t_uh * 1e3,
            "batched_tuples_per_s": tuples / t_batch, "per_join_tuples_per_s": tuples / t_plain,
-           "speedup": t_plain / t_batch,
+           "plain_unique_hash_tuples_per_s": tuples / t_uh,
+           "speedup_vs_plain_unique_hash": t_uh / t_batch,
+           "speedup_vs_same_kernels_per_join": t_plain / t_batch,
+           "speedup": t_uh / t_batch,
            "batched_GBps": alg / t_batch / 1e9, "frac": alg / t_batch / 1e9 / HBM_PEAK_GBPS,
            "kernels_ms": {k: v[0] for k, v in prof.items()}}
-    log(f"batch: batched {t_batch * 1e3:.2f} ms, per-join {t_plain * 1e3:.2f} ms, "
-        f"speedup {t_plain / t_batch:.2f}x")
+    log(f"batch: batched {t_batch * 1e3:.2f} ms; plain unique-hash call a join {t_uh * 1e3:.2f} "
+        f"ms ({t_uh / t_batch:.2f}x); the batch kernels a join at a time {t_plain * 1e3:.2f} ms "
+        f"({t_plain / t_batch:.2f}x)")
     if emit:
         dom = max(prof, key=lambda k: prof[k][0])
         line = {"metric": "batched join tuples/s (256 uneven joins)", "value": tuples / t_batch,
