@@ -266,20 +266,24 @@ int pf_serve_step(const pf_serve_item* items, int32_t n_items, int32_t rows_per_
  * No counterpart in the reference (the batched tier).  A plan for njoins
  * independent inner joins on int32 keys (unique builds): nb[j] build and
  * np[j] probe rows, stored back to back in the key buffers.  tune: NULL or
- * 9 int32, per size class of the build (< 32K rows, < 512K, larger) the
- * table slots per build row (2 or 4), the probe rows per work item and the
- * probe/compaction workgroup size (256 or 512).  The plan owns its device
- * scratch (one allocation at creation).
+ * 12 int32, per size class of the build (< 32K rows, < 512K, larger) the
+ * table slots per build row (2 or 4), the probe rows per work item, the
+ * probe/compaction workgroup size (256 or 512) and the probe slice size in
+ * KiB (0, or 64 to 2^20): a join whose table is larger has its probe rows
+ * partitioned by the top bits of their table slot (at most 8 bits, one
+ * pass) and probed slice by slice, each XCD on its own slices.  The plan
+ * owns its device scratch (one allocation at creation).
  *
  * pf_batch_join writes join j's pairs (join-local int64 probe row, build
- * row; probe-row order) at out_probe/out_build + (sum of np[i], i < j),
+ * row; probe-row order, within each slice for a sliced join, slices
+ * ascending) at out_probe/out_build + (sum of np[i], i < j),
  * their count at counts[j] (device).  mode 0: the whole batch as one
  * pipeline of persistent kernels fed by device work queues (no host wait);
  * mode 1: the same kernels one join at a time with one host wait a join
  * (the plain baseline).  pf_batch_capture records mode 0 on `stream` (not
  * the null stream) as a hipGraph bound to these buffers; pf_batch_replay
  * launches it.  pf_batch_plan_info: 0 build rows, 1 probe rows, 2 table
- * slots, 3 build items, 4 probe items, 5 scratch bytes. */
+ * slots, 3 build items, 4 probe items, 5 scratch bytes, 6 sliced joins. */
 void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
                            const int32_t* tune);
 int pf_batch_plan_destroy(void* plan);

@@ -23,12 +23,16 @@ import torch
 from . import _lib
 
 CLASSES = ("small", "medium", "large")        # build rows < 32K, < 512K, larger
-DEFAULT_TUNE = [(2, 4096, 256), (2, 16384, 256), (2, 16384, 256)]
-# candidates per class: (table slots per build row, probe rows an item,
-# workgroup size); the search also tries the large class on the partitioned
-# LDS-table join (radix bits 8 or 12, one or two passes) instead
-CANDIDATES = [(2, 4096, 256), (4, 4096, 256), (2, 16384, 256), (2, 16384, 512),
-              (4, 16384, 512), (2, 65536, 512)]
+# per class: (table slots per build row, probe rows an item, workgroup size,
+# probe slice KiB).  A slice size > 0 partitions the join's probe rows by
+# the top bits of their table slot (radix bits = log2(table / slice), one
+# pass) so that each XCD probes an L2-sized part of the table; 0 probes the
+# whole table at random.
+DEFAULT_TUNE = [(2, 4096, 256, 0), (2, 16384, 256, 0), (2, 16384, 256, 2048)]
+CANDIDATES = [(2, 4096, 256, 0), (4, 4096, 256, 0), (2, 16384, 256, 0), (2, 16384, 512, 0),
+              (4, 16384, 512, 0), (2, 65536, 512, 0),
+              (2, 16384, 512, 1024), (2, 16384, 512, 2048), (2, 16384, 512, 4096),
+              (4, 16384, 512, 2048)]
 
 
 def class_of(nb: int) -> int:
@@ -62,12 +66,14 @@ class BatchJoin:
     device scratch)."""
 
     def __init__(self, nb: Sequence[int], np_: Sequence[int],
-                 tune: Optional[Sequence[Tuple[int, int, int]]] = None, device=None):
+                 tune: Optional[Sequence[Tuple[int, int, int, int]]] = None, device=None):
         self.nb = np.asarray(nb, dtype=np.int64)
         self.np = np.asarray(np_, dtype=np.int64)
         self.n = len(self.nb)
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self.tune = [tuple(t) for t in (tune or DEFAULT_TUNE)]
+        if len(self.tune) != len(CLASSES) or any(len(t) != 4 for t in self.tune):
+            raise ValueError("BatchJoin: tune is (load, chunk, workgroup, slice KiB) per class")
         t = np.asarray(self.tune, dtype=np.int32).reshape(-1)
         lib = _lib.load()
         with torch.cuda.device(self.device):
@@ -98,7 +104,7 @@ class BatchJoin:
     def info(self) -> Dict[str, int]:
         lib = _lib.load()
         names = ["build_rows", "probe_rows", "table_slots", "build_items", "probe_items",
-                 "scratch_bytes"]
+                 "scratch_bytes", "sliced_joins"]
         return {n: int(lib.pf_batch_plan_info(self.plan, i)) for i, n in enumerate(names)}
 
     def run(self, bkeys: torch.Tensor, pkeys: torch.Tensor, per_join: bool = False,
@@ -133,7 +139,9 @@ class BatchJoin:
                                                    ctypes.c_void_p(self.stream.cuda_stream)))
 
     def pairs(self, j: int) -> Tuple[torch.Tensor, torch.Tensor]:
-        """Join j's (probe rows, build rows) after a run (reads its count)."""
+        """Join j's (probe rows, build rows) after a run (reads its count):
+        in probe-row order, or, for a join probed by slices, in probe-row
+        order within each slice (slices ascending)."""
         n = int(self.counts[j])
         a = int(self.p_off[j])
         return self.out_probe[a:a + n], self.out_build[a:a + n]
@@ -150,7 +158,7 @@ def _time(fn, reps: int, device) -> float:
 
 
 def autotune(nb, np_, bkeys, pkeys, reps: int = 3, cache: Optional[str] = None,
-             device=None) -> List[Tuple[int, int, int]]:
+             device=None) -> List[Tuple[int, int, int, int]]:
     """Per size class, the candidate tunables with the shortest batched run
     over this batch (coordinate search: one class at a time, the others
     fixed at their best so far).  Results are cached in ``cache`` (JSON) by

@@ -37,6 +37,7 @@
 #include "../../include/pageflow_join.h"
 
 namespace pf {
+
 namespace {
 
 constexpr uint64_t kEmpty = ~0ULL;
@@ -44,12 +45,20 @@ constexpr int BUILD_ROWS = 4096;
 constexpr int NCLASS = 3;
 
 struct Tune {
-  int load;   // slots per build row (2 or 4)
-  int chunk;  // probe rows an item
-  int nt;     // probe / compact workgroup size (256 or 512)
+  int load;      // slots per build row (2 or 4)
+  int chunk;     // probe rows an item
+  int nt;        // probe / compact workgroup size (256 or 512)
+  int slice_kb;  // 0, or the table bytes (KiB) of a probe slice (sliced probe)
 };
 
 int class_of(int64_t nb) { return nb < 32768 ? 0 : (nb < 524288 ? 1 : 2); }
+
+constexpr int SLICE_ROWS = 8192;  // probe rows a sliced work item
+constexpr int NXQ = 8;            // queues (XCDs)
+
+struct SliceQueues {
+  int32_t lo[NXQ], hi[NXQ];   // each queue's (join, slice) pairs in the pair lists
+};
 
 struct Plan {
   int nj = 0;
@@ -60,6 +69,7 @@ struct Plan {
   int64_t n_bitems = 0;
   int64_t n_pitems[2] = {0, 0};      // [0]: nt 256, [1]: nt 512
   int64_t pitem_base[2] = {0, 0};    // first item of each group in the item arrays
+  int64_t n_pitems_plain[2] = {0, 0};  // each group's items of joins probed whole (first)
   int64_t n_pitems_all = 0;
   int64_t total_slots = 0, total_b = 0, total_p = 0;
   std::vector<int64_t> first_item, last_item;  // per join, in the probe item arrays
@@ -81,6 +91,29 @@ struct Plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int persist = 1024;  // workgroups of a persistent launch
+  // sliced probe: slice bits per join (0: probed whole), the (join, slice)
+  // pairs of each queue, each join's pairs in each queue (per-join runs)
+  std::vector<int> sbits, log2cap;
+  std::vector<int64_t> soff_base;
+  int64_t soff_total = 0;
+  bool any_sliced = false;
+  std::vector<int32_t> pair_join, pair_slice;
+  SliceQueues qall{};
+  std::vector<SliceQueues> qjoin;
+  int32_t *d_sliced = nullptr, *d_chunk = nullptr, *d_pair_join = nullptr,
+          *d_pair_slice = nullptr, *d_xstart = nullptr, *d_pk_part = nullptr,
+          *d_pv_part = nullptr;
+  int64_t *d_soff_base = nullptr, *d_soff = nullptr;
+  // the sliced joins' probe keys as segments of one partition: all of them
+  // (batched runs) and each alone (per-join runs)
+  std::vector<RpSeg> segs, segs1;
+  std::vector<int32_t> seg_join;
+  std::vector<int> seg_of;  // per join: its segment, or -1
+  int64_t seg_blocks = 0, seg_entries = 0;
+  RpSeg *d_segs = nullptr, *d_segs1 = nullptr;
+  int32_t *d_seg_join = nullptr, *d_shist = nullptr;
+  int64_t* d_soffs = nullptr;
+  void* d_sscan = nullptr;
 };
 
 // A work item taken from the queue, as a wave-uniform (scalar) value.  The
@@ -183,6 +216,7 @@ __global__ __launch_bounds__(NT) void bj_probe(const int32_t* __restrict__ pkeys
                                                const int32_t* __restrict__ item_len,
                                                int64_t n_items, int64_t item_lo,
                                                const uint64_t* __restrict__ table,
+                                               const int32_t* __restrict__ sliced,
                                                int32_t* __restrict__ partner,
                                                int64_t* __restrict__ item_cnt, int32_t* queue) {
   __shared__ int64_t s_item;
@@ -201,6 +235,10 @@ __global__ __launch_bounds__(NT) void bj_probe(const int32_t* __restrict__ pkeys
     __syncthreads();
     if (item >= item_lo + n_items) return;
     const int j = item_join[item];
+    if (sliced[j]) {  // probed by bj_probe_sliced, which also counts the item
+      prev = -1;
+      continue;
+    }
     const int64_t base = p_off[j] + item_start[item];
     const int len = item_len[item];
     const uint64_t* t = table + t_off[j];
@@ -252,6 +290,8 @@ __global__ __launch_bounds__(NT) void bj_compact(const int64_t* __restrict__ p_o
                                                  int64_t n_items, int64_t item_lo,
                                                  const int64_t* __restrict__ first,
                                                  const int32_t* __restrict__ partner,
+                                                 const int32_t* __restrict__ sliced,
+                                                 const int32_t* __restrict__ rows_part,
                                                  const int64_t* __restrict__ item_pos,
                                                  int64_t* __restrict__ out_probe,
                                                  int64_t* __restrict__ out_build, int32_t* queue) {
@@ -269,6 +309,9 @@ __global__ __launch_bounds__(NT) void bj_compact(const int64_t* __restrict__ p_o
     const int64_t r00 = item_start[item];
     const int64_t base = p_off[j] + r00;
     const int len = item_len[item];
+    // a sliced join's partner is in partition order: its probe rows come
+    // from the partition's row ids
+    const int32_t* rmap = sliced[j] ? rows_part + base : nullptr;
     int64_t o = p_off[j] + item_pos[item] - item_pos[first[j]];
     for (int r0 = 0; r0 < len; r0 += NT) {
       const int r = r0 + threadIdx.x;
@@ -285,13 +328,150 @@ __global__ __launch_bounds__(NT) void bj_compact(const int64_t* __restrict__ p_o
       }
       if (pr >= 0) {
         const int64_t pos = o + pre + mbcnt(bal);
-        out_probe[pos] = r00 + r;
+        out_probe[pos] = rmap ? (int64_t)rmap[r] : r00 + r;
         out_build[pos] = pr;
       }
       o += tot;
       __syncthreads();
     }
     (void)s_base;
+  }
+}
+
+// ------------------------------------------------------ sliced probe --
+//
+// A large join's table spans far more than an XCD's L2, so its random
+// probes are bound by 64-byte memory requests.  Its probe rows are first
+// partitioned (stably) by the top bits of their table slot into slices of
+// about an L2's worth of table; work items (join, slice, chunk) go to eight
+// queues, slice s of join j to queue (s + j) % 8, and workgroup b starts on
+// queue b % 8: workgroups are dealt to the eight XCDs round robin, so each
+// XCD's L2 holds the slices it probes.  A workgroup whose queue runs dry
+// takes items from the next queues, so every item is probed once whatever
+// the placement.
+
+// Block x: each pair's chunks and their exclusive scan over the queue's
+// pairs [lo, hi) into xstart[lo..hi] (xstart[hi] = the queue's items).
+__global__ __launch_bounds__(1024) void bj_slice_scan(const int32_t* __restrict__ pair_join,
+                                                      const int32_t* __restrict__ pair_slice,
+                                                      const int64_t* __restrict__ soff_base,
+                                                      const int64_t* __restrict__ soff,
+                                                      const SliceQueues q,
+                                                      int32_t* __restrict__ xstart) {
+  __shared__ int32_t wsum[1024 / kWave];
+  __shared__ int32_t carry;
+  const int x = blockIdx.x;
+  const int lo = q.lo[x], hi = q.hi[x];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int p0 = lo; p0 < hi; p0 += 1024) {
+    const int p = p0 + (int)threadIdx.x;
+    int nch = 0;
+    if (p < hi) {
+      const int64_t* so = soff + soff_base[pair_join[p]];
+      const int sl = pair_slice[p];
+      nch = (int)((so[sl + 1] - so[sl] + SLICE_ROWS - 1) / SLICE_ROWS);
+    }
+    const int incl = wave_incl_scan(nch);
+    if (lane_id() == kWave - 1) wsum[threadIdx.x / kWave] = incl;
+    __syncthreads();
+    int before = carry;
+    for (int w = 0; w < (int)threadIdx.x / kWave; ++w) before += wsum[w];
+    if (p < hi) xstart[p] = before + incl - nch;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = before + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) xstart[hi] = carry;
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void bj_probe_sliced(
+    const int32_t* __restrict__ pk_part, const int64_t* __restrict__ p_off,
+    const int64_t* __restrict__ t_off, const uint32_t* __restrict__ mask,
+    const int32_t* __restrict__ pair_join, const int32_t* __restrict__ pair_slice,
+    const int64_t* __restrict__ soff_base, const int64_t* __restrict__ soff,
+    const int32_t* __restrict__ xstart, const SliceQueues q,
+    const int64_t* __restrict__ first, const int32_t* __restrict__ chunk,
+    const uint64_t* __restrict__ table, int32_t* __restrict__ partner,
+    int64_t* __restrict__ item_cnt, int32_t* xqueue) {
+  __shared__ int64_t s_w[4];  // item, pair (or -1: this queue is done), items in the queue
+  int qi = 0;
+  int x = (int)(blockIdx.x % NXQ);
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const int lo = q.lo[x], hi = q.hi[x];
+      const int n = xstart[hi];
+      const int it = atomicAdd(&xqueue[x], 1);
+      int p = -1;
+      if (it < n) {  // the pair holding item `it`: the last with xstart[p] <= it
+        int a = lo, b = hi;
+        while (b - a > 1) {
+          const int mid = (a + b) >> 1;
+          if (xstart[mid] <= it)
+            a = mid;
+          else
+            b = mid;
+        }
+        p = a;
+      }
+      s_w[0] = it;
+      s_w[1] = p;
+    }
+    __syncthreads();
+    const int64_t it = uniform_item(s_w[0]);
+    const int64_t p = uniform_item(s_w[1]);
+    __syncthreads();
+    if (p < 0) {  // this queue is done: the next one, until all eight are
+      if (++qi == NXQ) return;
+      x = (x + 1) % NXQ;
+      continue;
+    }
+    const int j = pair_join[p];
+    const int sl = pair_slice[p];
+    const int64_t* so = soff + soff_base[j];
+    const int64_t r0 = so[sl] + (it - xstart[p]) * (int64_t)SLICE_ROWS;
+    const int64_t r1 = r0 + SLICE_ROWS < so[sl + 1] ? r0 + SLICE_ROWS : so[sl + 1];
+    const int64_t pb = p_off[j];
+    const uint64_t* t = table + t_off[j];
+    const uint32_t m = mask[j];
+    const int64_t ci0 = first[j];
+    const int ch = chunk[j];
+    constexpr int U = 8;
+    for (int64_t b0 = r0 + threadIdx.x; b0 < r1; b0 += (int64_t)NT * U) {
+      int32_t k[U];
+      uint32_t sv[U];
+      uint64_t v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t r = b0 + (int64_t)u * NT;
+        k[u] = r < r1 ? pk_part[pb + r] : 0;
+        sv[u] = bslot(k[u], m);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = b0 + (int64_t)u * NT < r1 ? t[sv[u]] : kEmpty;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t r = b0 + (int64_t)u * NT;
+        while (v[u] != kEmpty && (uint32_t)(v[u] >> 32) != (uint32_t)k[u]) {
+          sv[u] = (sv[u] + 1) & m;
+          v[u] = t[sv[u]];
+        }
+        const bool hit = v[u] != kEmpty;
+        if (r < r1) partner[pb + r] = hit ? (int32_t)(uint32_t)v[u] : -1;
+        // the hit counts of the compaction items (chunks of `ch` rows of the
+        // partitioned order) this wave's rows fall in: at most two
+        const int64_t rr = r < r1 ? r : r0;
+        const int64_t cw = __shfl(rr, 0, kWave) / ch;  // the first lane's item
+        const int64_t ci = rr / ch;
+        const uint64_t b_lo = ballot(hit && r < r1 && ci == cw);
+        const uint64_t b_hi = ballot(hit && r < r1 && ci != cw);
+        if (lane_id() == 0) {
+          if (b_lo) atomicAdd((unsigned long long*)&item_cnt[ci0 + cw], (unsigned long long)__popcll(b_lo));
+          if (b_hi) atomicAdd((unsigned long long*)&item_cnt[ci0 + cw + 1], (unsigned long long)__popcll(b_hi));
+        }
+      }
+    }
   }
 }
 
@@ -310,16 +490,19 @@ int64_t pow2_at_least(int64_t x) {
 int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int32_t* tune) {
   P.nj = nj;
   for (int c = 0; c < NCLASS; ++c) {
-    P.tune[c] = Tune{2, c == 0 ? 4096 : 16384, 256};
+    P.tune[c] = Tune{2, c == 0 ? 4096 : 16384, 256, c == 2 ? 2048 : 0};
     if (tune) {
-      P.tune[c].load = tune[c * 3 + 0];
-      P.tune[c].chunk = tune[c * 3 + 1];
-      P.tune[c].nt = tune[c * 3 + 2];
+      P.tune[c].load = tune[c * 4 + 0];
+      P.tune[c].chunk = tune[c * 4 + 1];
+      P.tune[c].nt = tune[c * 4 + 2];
+      P.tune[c].slice_kb = tune[c * 4 + 3];
     }
     const Tune& t = P.tune[c];
     if ((t.load != 2 && t.load != 4) || t.chunk < 256 || t.chunk > (1 << 20) ||
-        (t.nt != 256 && t.nt != 512))
-      return fail("batch plan: tunables are load 2|4, chunk in [256, 2^20], nt 256|512");
+        (t.nt != 256 && t.nt != 512) ||
+        (t.slice_kb != 0 && (t.slice_kb < 64 || t.slice_kb > (1 << 20))))
+      return fail("batch plan: tunables are load 2|4, chunk in [256, 2^20], nt 256|512, "
+                  "slice KiB 0 or in [64, 2^20]");
   }
   P.nb.assign(nb, nb + nj);
   P.np.assign(np, np + nj);
@@ -340,6 +523,69 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
   P.total_b = P.b_off[nj];
   P.total_p = P.p_off[nj];
   P.total_slots = P.t_off[nj];
+  // slices: 2^sbits of a table, each about slice_kb KiB of slots
+  P.sbits.assign(nj, 0);
+  P.log2cap.assign(nj, 0);
+  P.soff_base.assign(nj, 0);
+  for (int j = 0; j < nj; ++j) {
+    int l = 0;
+    while ((1LL << l) < P.cap[j]) ++l;
+    P.log2cap[j] = l;
+    const int64_t kb = P.tune[P.cls[j]].slice_kb;
+    int sb = 0;
+    if (kb > 0 && np[j] > 0 && nb[j] > 0)
+      while (sb < 8 && sb < l && (P.cap[j] * 8 >> sb) > kb * 1024) ++sb;
+    P.sbits[j] = sb;
+    P.soff_base[j] = P.soff_total;
+    if (sb > 0) {
+      P.soff_total += (1 << sb) + 1;
+      P.any_sliced = true;
+    }
+  }
+  // segments: the sliced joins in join order
+  P.seg_of.assign(nj, -1);
+  int64_t cum = 0;
+  for (int j = 0; j < nj; ++j) {
+    if (P.sbits[j] == 0) continue;
+    RpSeg g{};
+    g.in_off = P.p_off[j];
+    g.n = np[j];
+    g.nchunks = (np[j] + kSegChunkRows - 1) / kSegChunkRows;
+    g.bits = P.sbits[j];
+    g.shift = 64 - P.log2cap[j];
+    RpSeg one = g;
+    g.blk_start = P.seg_blocks;
+    g.hist_off = P.seg_entries;
+    g.delta = P.p_off[j] - cum;
+    one.blk_start = 0;
+    one.hist_off = 0;
+    one.delta = P.p_off[j];
+    P.seg_of[j] = (int)P.segs.size();
+    P.segs.push_back(g);
+    P.segs1.push_back(one);
+    P.seg_join.push_back(j);
+    P.seg_blocks += g.nchunks;
+    P.seg_entries += g.nchunks << g.bits;
+    cum += np[j];
+  }
+  // the (join, slice) pairs of each queue, join-major, a one-entry gap after
+  // each queue (where the queue's item total is written)
+  P.qjoin.assign(nj, SliceQueues{});
+  for (int x = 0; x < NXQ; ++x) {
+    P.qall.lo[x] = (int32_t)P.pair_join.size();
+    for (int j = 0; j < nj; ++j) {
+      P.qjoin[j].lo[x] = (int32_t)P.pair_join.size();
+      for (int sl = 0; sl < (1 << P.sbits[j]) && P.sbits[j] > 0; ++sl)
+        if ((sl + j) % NXQ == x) {
+          P.pair_join.push_back(j);
+          P.pair_slice.push_back(sl);
+        }
+      P.qjoin[j].hi[x] = (int32_t)P.pair_join.size();
+    }
+    P.qall.hi[x] = (int32_t)P.pair_join.size();
+    P.pair_join.push_back(-1);  // the gap
+    P.pair_slice.push_back(-1);
+  }
   return 0;
 }
 
@@ -368,16 +614,20 @@ void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
       bj.push_back(j);
       bs.push_back(r);
     }
-  for (int j = 0; j < njoins; ++j) {
-    const Tune& t = P->tune[P->cls[j]];
-    const int g = t.nt == 512 ? 1 : 0;
-    if (P->nb[j] == 0) continue;  // no match possible
-    for (int64_t r = 0; r < P->np[j]; r += t.chunk) {
-      pj[g].push_back(j);
-      ps[g].push_back(r);
-      pl[g].push_back((int32_t)std::min<int64_t>(t.chunk, P->np[j] - r));
+  // in each workgroup-size group the sliced joins' items come last: the
+  // probe kernel takes the others, the sliced probe counts those
+  for (int pass = 0; pass < 2; ++pass)
+    for (int j = 0; j < njoins; ++j) {
+      const Tune& t = P->tune[P->cls[j]];
+      const int g = t.nt == 512 ? 1 : 0;
+      if (P->nb[j] == 0 || (P->sbits[j] > 0) != (pass == 1)) continue;  // no match possible
+      for (int64_t r = 0; r < P->np[j]; r += t.chunk) {
+        pj[g].push_back(j);
+        ps[g].push_back(r);
+        pl[g].push_back((int32_t)std::min<int64_t>(t.chunk, P->np[j] - r));
+      }
+      if (pass == 0) P->n_pitems_plain[g] = (int64_t)pj[g].size();
     }
-  }
   // probe items: group 0 then group 1; per-join first/last item
   std::vector<int32_t> ij, il;
   std::vector<int64_t> is;
@@ -416,8 +666,26 @@ void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
     P->d_last = c.take<int64_t>(njoins);
     P->d_partner = c.take<int32_t>(std::max<int64_t>(P->total_p, 1));
     P->d_table = c.take<uint64_t>(std::max<int64_t>(P->total_slots, 1));
-    P->d_queue = c.take<int32_t>(8);
+    P->d_queue = c.take<int32_t>(16);  // [0..4] the kernels' queues, [8..15] the slice queues
     P->d_scan = c.take<char>(scan_i64_scratch(npi + 1));
+    const int64_t npairs = (int64_t)P->pair_join.size();
+    P->d_sliced = c.take<int32_t>(njoins);
+    P->d_chunk = c.take<int32_t>(njoins);
+    P->d_soff_base = c.take<int64_t>(njoins);
+    P->d_pair_join = c.take<int32_t>(npairs);
+    P->d_pair_slice = c.take<int32_t>(npairs);
+    P->d_xstart = c.take<int32_t>(npairs);
+    if (P->any_sliced) {
+      P->d_soff = c.take<int64_t>(P->soff_total);
+      P->d_pk_part = c.take<int32_t>(std::max<int64_t>(P->total_p, 1));
+      P->d_pv_part = c.take<int32_t>(std::max<int64_t>(P->total_p, 1));
+      P->d_segs = c.take<RpSeg>(P->segs.size());
+      P->d_segs1 = c.take<RpSeg>(P->segs1.size());
+      P->d_seg_join = c.take<int32_t>(P->seg_join.size());
+      P->d_shist = c.take<int32_t>(P->seg_entries);
+      P->d_soffs = c.take<int64_t>(P->seg_entries);
+      P->d_sscan = c.take<char>(scan_i32_scratch(P->seg_entries));
+    }
   };
   layout(cv);
   P->dbytes = cv.bytes();
@@ -444,6 +712,20 @@ void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
              up(P->d_pitem_start, is.data(), is.size() * 8) &&
              up(P->d_first, P->first_item.data(), njoins * 8) &&
              up(P->d_last, P->last_item.data(), njoins * 8);
+  std::vector<int32_t> sliced(njoins), chunk(njoins);
+  for (int j = 0; j < njoins; ++j) {
+    sliced[j] = P->sbits[j];
+    chunk[j] = P->tune[P->cls[j]].chunk;
+  }
+  okc = okc && up(P->d_sliced, sliced.data(), njoins * 4) &&
+        up(P->d_chunk, chunk.data(), njoins * 4) &&
+        up(P->d_soff_base, P->soff_base.data(), njoins * 8) &&
+        up(P->d_pair_join, P->pair_join.data(), P->pair_join.size() * 4) &&
+        up(P->d_pair_slice, P->pair_slice.data(), P->pair_slice.size() * 4);
+  if (P->any_sliced)
+    okc = okc && up(P->d_segs, P->segs.data(), P->segs.size() * sizeof(RpSeg)) &&
+          up(P->d_segs1, P->segs1.data(), P->segs1.size() * sizeof(RpSeg)) &&
+          up(P->d_seg_join, P->seg_join.data(), P->seg_join.size() * 4);
   if (!okc) {
     fail("batch plan: upload failed");
     (void)hipFree(P->dmem);
@@ -479,6 +761,11 @@ int64_t pf_batch_plan_info(void* plan, int32_t what) {
     case 3: return P->n_bitems;
     case 4: return P->n_pitems_all;
     case 5: return (int64_t)P->dbytes;
+    case 6: {
+      int64_t n = 0;
+      for (int b : P->sbits) n += b > 0;
+      return n;
+    }
     default: return -1;
   }
 }
@@ -488,11 +775,58 @@ int64_t pf_batch_plan_info(void* plan, int32_t what) {
 namespace pf {
 namespace {
 
+// Block g: segment g's slice offsets, join-local, into soff (2^bits + 1).
+__global__ void bj_slice_bounds(const RpSeg* __restrict__ segs, const int32_t* __restrict__ seg_join,
+                                const int64_t* __restrict__ offs, const int64_t* __restrict__ p_off,
+                                const int64_t* __restrict__ soff_base, int64_t* __restrict__ soff) {
+  const RpSeg& g = segs[blockIdx.x];
+  const int j = seg_join[blockIdx.x];
+  const int B = 1 << g.bits;
+  for (int b = threadIdx.x; b <= B; b += blockDim.x)
+    soff[soff_base[j] + b] =
+        b < B ? offs[g.hist_off + (int64_t)b * g.nchunks] + g.delta - p_off[j] : g.n;
+}
+
+// The sliced joins' probe rows partitioned by slice (all of them, or join
+// `only`), their items counted from zero, and the queues' chunk scans.
+int slice_stage(Plan& P, const int32_t* pkeys, const SliceQueues& q, int only, hipStream_t st) {
+  if (only < 0) {
+    if (fill_i64(P.d_item_cnt, P.n_pitems_all, 0, st)) return 1;
+  } else if (P.first_item[only] >= 0) {
+    if (fill_i64(P.d_item_cnt + P.first_item[only],
+                 P.last_item[only] - P.first_item[only] + 1, 0, st))
+      return 1;
+  }
+  const int nseg = only < 0 ? (int)P.segs.size() : 1;
+  const int k = only < 0 ? 0 : P.seg_of[only];
+  const RpSeg* segs = only < 0 ? P.d_segs : P.d_segs1 + k;
+  const int64_t blocks = only < 0 ? P.seg_blocks : P.segs1[k].nchunks;
+  const int64_t entries = only < 0 ? P.seg_entries : P.segs1[k].nchunks << P.segs1[k].bits;
+  if (radix_partition_segments(pkeys, segs, nseg, blocks, entries, P.d_shist, P.d_soffs,
+                               P.d_sscan, P.d_pk_part, P.d_pv_part, st, "bj_slice_hist",
+                               "bj_slice_scatter"))
+    return 1;
+  bj_slice_bounds<<<nseg, 256, 0, st>>>(segs, P.d_seg_join + k, P.d_soffs, P.d_p_off,
+                                        P.d_soff_base, P.d_soff);
+  bj_slice_scan<<<NXQ, 1024, 0, st>>>(P.d_pair_join, P.d_pair_slice, P.d_soff_base, P.d_soff, q,
+                                      P.d_xstart);
+  return launch_check("batch slices");
+}
+
+int launch_sliced_probe(Plan& P, const SliceQueues& q, hipStream_t st) {
+  KTimer _kt("bj_probe_sliced", st);
+  bj_probe_sliced<512><<<P.persist, 512, 0, st>>>(
+      P.d_pk_part, P.d_p_off, P.d_t_off, P.d_mask, P.d_pair_join, P.d_pair_slice,
+      P.d_soff_base, P.d_soff, P.d_xstart, q, P.d_first, P.d_chunk, P.d_table, P.d_partner,
+      P.d_item_cnt, P.d_queue + 8);
+  return launch_check("batch sliced probe");
+}
+
 int run_batched(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* out_probe,
                 int64_t* out_build, int64_t* counts, hipStream_t st) {
   {
     KTimer _kt("bj_fill", st);
-    bj_fill_tables<<<2048, 256, 0, st>>>(P.d_table, P.total_slots, P.d_queue, 8);
+    bj_fill_tables<<<2048, 256, 0, st>>>(P.d_table, P.total_slots, P.d_queue, 16);
   }
   if (P.n_bitems > 0) {
     KTimer _kt("bj_build", st);
@@ -500,22 +834,24 @@ int run_batched(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* ou
         bkeys, P.d_b_off, P.d_t_off, P.d_mask, P.d_bitem_join, P.d_bitem_start, P.n_bitems, 0,
         P.d_table, P.d_queue + 0);
   }
+  if (P.any_sliced && slice_stage(P, pkeys, P.qall, -1, st)) return 1;
   for (int g = 0; g < 2; ++g) {
-    if (P.n_pitems[g] == 0) continue;
+    if (P.n_pitems_plain[g] == 0) continue;
     KTimer _kt("bj_probe", st);
-    const unsigned grid = (unsigned)std::min<int64_t>(P.persist, P.n_pitems[g]);
+    const unsigned grid = (unsigned)std::min<int64_t>(P.persist, P.n_pitems_plain[g]);
     if (g == 0)
       bj_probe<256><<<grid, 256, 0, st>>>(pkeys, P.d_p_off, P.d_t_off, P.d_mask, P.d_pitem_join,
-                                          P.d_pitem_start, P.d_pitem_len, P.n_pitems[g],
-                                          P.pitem_base[g], P.d_table, P.d_partner,
+                                          P.d_pitem_start, P.d_pitem_len, P.n_pitems_plain[g],
+                                          P.pitem_base[g], P.d_table, P.d_sliced, P.d_partner,
                                           P.d_item_cnt, P.d_queue + 1);
     else
       bj_probe<512><<<grid, 512, 0, st>>>(pkeys, P.d_p_off, P.d_t_off, P.d_mask, P.d_pitem_join,
-                                          P.d_pitem_start, P.d_pitem_len, P.n_pitems[g],
-                                          P.pitem_base[g], P.d_table, P.d_partner,
+                                          P.d_pitem_start, P.d_pitem_len, P.n_pitems_plain[g],
+                                          P.pitem_base[g], P.d_table, P.d_sliced, P.d_partner,
                                           P.d_item_cnt, P.d_queue + 2);
   }
   if (launch_check("batch join")) return 1;
+  if (P.any_sliced && launch_sliced_probe(P, P.qall, st)) return 1;
   if (scan_i64_excl_from(P.d_item_cnt, P.d_item_pos, P.n_pitems_all, P.d_item_pos +
                          std::max<int64_t>(P.n_pitems_all, 0), P.d_scan, st))
     return 1;
@@ -526,12 +862,14 @@ int run_batched(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* ou
     if (g == 0)
       bj_compact<256><<<grid, 256, 0, st>>>(P.d_p_off, P.d_pitem_join, P.d_pitem_start,
                                             P.d_pitem_len, P.n_pitems[g], P.pitem_base[g],
-                                            P.d_first, P.d_partner, P.d_item_pos, out_probe,
+                                            P.d_first, P.d_partner, P.d_sliced, P.d_pv_part,
+                                            P.d_item_pos, out_probe,
                                             out_build, P.d_queue + 3);
     else
       bj_compact<512><<<grid, 512, 0, st>>>(P.d_p_off, P.d_pitem_join, P.d_pitem_start,
                                             P.d_pitem_len, P.n_pitems[g], P.pitem_base[g],
-                                            P.d_first, P.d_partner, P.d_item_pos, out_probe,
+                                            P.d_first, P.d_partner, P.d_sliced, P.d_pv_part,
+                                            P.d_item_pos, out_probe,
                                             out_build, P.d_queue + 4);
   }
   bj_counts<<<grid_for(P.nj, 256), 256, 0, st>>>(P.d_item_pos, P.d_first, P.d_last, P.nj,
@@ -549,7 +887,7 @@ int run_per_join(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* o
     bfirst[j + 1] = bfirst[j] + (P.nb[j] + BUILD_ROWS - 1) / BUILD_ROWS;
   for (int j = 0; j < P.nj; ++j) {
     bj_fill_tables<<<grid_for(P.cap[j], 256) < 2048 ? grid_for(P.cap[j], 256) : 2048, 256, 0,
-                     st>>>(P.d_table + P.t_off[j], P.cap[j], P.d_queue, 8);
+                     st>>>(P.d_table + P.t_off[j], P.cap[j], P.d_queue, 16);
     const int64_t nbi = bfirst[j + 1] - bfirst[j];
     if (nbi > 0)
       bj_build<<<(unsigned)std::min<int64_t>(P.persist, nbi), 256, 0, st>>>(
@@ -560,15 +898,18 @@ int run_per_join(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* o
       const int64_t n = l - f + 1;
       const unsigned grid = (unsigned)std::min<int64_t>(P.persist, n);
       const bool big = P.tune[P.cls[j]].nt == 512;
-      if (!big)
+      if (P.sbits[j] > 0) {
+        if (slice_stage(P, pkeys, P.qjoin[j], j, st) || launch_sliced_probe(P, P.qjoin[j], st))
+          return 1;
+      } else if (!big)
         bj_probe<256><<<grid, 256, 0, st>>>(pkeys, P.d_p_off, P.d_t_off, P.d_mask,
                                             P.d_pitem_join, P.d_pitem_start, P.d_pitem_len, n, f,
-                                            P.d_table, P.d_partner, P.d_item_cnt,
+                                            P.d_table, P.d_sliced, P.d_partner, P.d_item_cnt,
                                             P.d_queue + 1);
       else
         bj_probe<512><<<grid, 512, 0, st>>>(pkeys, P.d_p_off, P.d_t_off, P.d_mask,
                                             P.d_pitem_join, P.d_pitem_start, P.d_pitem_len, n, f,
-                                            P.d_table, P.d_partner, P.d_item_cnt,
+                                            P.d_table, P.d_sliced, P.d_partner, P.d_item_cnt,
                                             P.d_queue + 1);
       // this join's items scanned on their own; positions relative to f
       if (scan_i64_excl_from(P.d_item_cnt + f, P.d_item_pos + f, n, P.d_item_pos + f + n,
@@ -577,12 +918,14 @@ int run_per_join(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* o
       if (!big)
         bj_compact<256><<<grid, 256, 0, st>>>(P.d_p_off, P.d_pitem_join, P.d_pitem_start,
                                               P.d_pitem_len, n, f, P.d_first, P.d_partner,
-                                              P.d_item_pos, out_probe, out_build,
+                                              P.d_sliced, P.d_pv_part, P.d_item_pos, out_probe,
+                                              out_build,
                                               P.d_queue + 3);
       else
         bj_compact<512><<<grid, 512, 0, st>>>(P.d_p_off, P.d_pitem_join, P.d_pitem_start,
                                               P.d_pitem_len, n, f, P.d_first, P.d_partner,
-                                              P.d_item_pos, out_probe, out_build,
+                                              P.d_sliced, P.d_pv_part, P.d_item_pos, out_probe,
+                                              out_build,
                                               P.d_queue + 3);
     }
     bj_counts<<<1, 64, 0, st>>>(P.d_item_pos, P.d_first + j, P.d_last + j, 1, counts + j);

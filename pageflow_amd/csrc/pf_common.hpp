@@ -176,6 +176,26 @@ size_t scan_i32_scratch(int64_t n);
 int scan_i32_to_i64_excl(const int32_t* in, int64_t* out, int64_t n, int64_t* total,
                          void* scratch, hipStream_t st);
 
+// One segment of a segmented radix partition (many arrays partitioned in
+// one launch each of hist, scan and scatter): its blocks start at
+// blk_start; its keys at in_off of the key buffer (n of them, int32); its
+// histogram (bucket-major, nchunks a bucket) at hist_off; its outputs at
+// the scanned position + delta; its digit is `bits` bits of mix64 below the
+// top `shift` bits.  Row ids are segment-local.
+struct RpSeg {
+  int64_t blk_start, in_off, n, nchunks, hist_off, delta;
+  int32_t bits, shift;
+};
+constexpr int64_t kSegChunkRows = 32768;  // rows a chunk, every segment
+
+// Stable segmented partition: one hist, one scan and one scatter launch
+// over the nseg segments in d_segs (nblocks chunks in all, hist_entries
+// histogram entries); hist/offs/scan_ws sized for hist_entries.
+int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
+                             int64_t nblocks, int64_t hist_entries, int32_t* hist, int64_t* offs,
+                             void* scan_ws, int32_t* kout, int32_t* vout, hipStream_t st,
+                             const char* hist_name, const char* scatter_name);
+
 // Fills n values of a device array (a kernel, never hipMemsetAsync, so that
 // graph capture records no memset node).
 int fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);

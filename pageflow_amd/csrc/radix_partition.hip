@@ -78,23 +78,53 @@ struct RpShape<12> {
 };
 
 template <typename K, typename KO, int MODE, int LOGB>
-__global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_hist(const K* __restrict__ keys,
-                                                            int64_t n, int64_t crows, int bits,
-                                                            int shift, int64_t nchunks,
-                                                            int32_t* __restrict__ hist) {
+__device__ __forceinline__ void rp_hist_body(const K* __restrict__ keys, int64_t n,
+                                             int64_t crows, int bits, int shift, int64_t nchunks,
+                                             int32_t* __restrict__ hist, int64_t chunk) {
   constexpr int NT = RpShape<LOGB>::NT;
   __shared__ int32_t h[1 << LOGB];
   const int B = 1 << bits;
   for (int b = threadIdx.x; b < B; b += NT) h[b] = 0;
   __syncthreads();
-  const int64_t c0 = (int64_t)blockIdx.x * crows;
+  const int64_t c0 = chunk * crows;
   const int64_t c1 = c0 + crows < n ? c0 + crows : n;
   for (int64_t i = c0 + threadIdx.x; i < c1; i += NT) {
     KO o;
     atomicAdd(&h[rp_digit<K, KO, MODE>(keys[i], bits, shift, o)], 1);
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < B; b += NT) hist[(int64_t)b * nchunks + blockIdx.x] = h[b];
+  for (int b = threadIdx.x; b < B; b += NT) hist[(int64_t)b * nchunks + chunk] = h[b];
+}
+
+template <typename K, typename KO, int MODE, int LOGB>
+__global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_hist(const K* __restrict__ keys,
+                                                            int64_t n, int64_t crows, int bits,
+                                                            int shift, int64_t nchunks,
+                                                            int32_t* __restrict__ hist) {
+  rp_hist_body<K, KO, MODE, LOGB>(keys, n, crows, bits, shift, nchunks, hist, blockIdx.x);
+}
+
+// The segment a block works for: the last whose first block is <= it.
+__device__ __forceinline__ int seg_of_block(const RpSeg* __restrict__ segs, int nseg) {
+  int a = 0, b = nseg;
+  while (b - a > 1) {
+    const int mid = (a + b) >> 1;
+    if (segs[mid].blk_start <= (int64_t)blockIdx.x)
+      a = mid;
+    else
+      b = mid;
+  }
+  return a;
+}
+
+__global__ __launch_bounds__(RpShape<8>::NT) void rp_hist_seg(const int32_t* __restrict__ keys,
+                                                             const RpSeg* __restrict__ segs,
+                                                             int nseg,
+                                                             int32_t* __restrict__ hist) {
+  const RpSeg& g = segs[seg_of_block(segs, nseg)];
+  rp_hist_body<int32_t, int32_t, RP_HASH_KEY, 8>(keys + g.in_off, g.n, kSegChunkRows, g.bits,
+                                                 g.shift, g.nchunks, hist + g.hist_off,
+                                                 (int64_t)blockIdx.x - g.blk_start);
 }
 
 // Lanes of the wave holding the same digit (d == ~0u: not a row).
@@ -233,10 +263,10 @@ __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_scatter(
 // bucket owned by one thread across the per-tile bookkeeping (wave prefix,
 // tile scan, global base, cursor), and the tile's bucket ids staged as bytes.
 template <typename K, typename KO, int MODE, int NT, int IT>
-__global__ __launch_bounds__(NT) void rp_scatter8(
+__device__ __forceinline__ void rp_scatter8_body(
     const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t n, int64_t crows,
     int bits, int shift, int64_t nchunks, const int64_t* __restrict__ offs,
-    KO* __restrict__ kout, int32_t* __restrict__ vout) {
+    KO* __restrict__ kout, int32_t* __restrict__ vout, int64_t chunk, int64_t delta) {
   constexpr int W = NT / kWave, TILE = NT * IT, BMAX = 256, WROWS = kWave * IT;
   static_assert(BMAX <= NT, "a thread owns each bucket");
   __shared__ uint16_t wh[W * BMAX];
@@ -249,9 +279,9 @@ __global__ __launch_bounds__(NT) void rp_scatter8(
 
   const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   const int B = 1 << bits;
-  const int64_t c0 = (int64_t)blockIdx.x * crows;
+  const int64_t c0 = chunk * crows;
   const int64_t c1 = c0 + crows < n ? c0 + crows : n;
-  uint32_t cursor = tid < B ? (uint32_t)offs[(int64_t)tid * nchunks + blockIdx.x] : 0u;
+  uint32_t cursor = tid < B ? (uint32_t)(offs[(int64_t)tid * nchunks + chunk] + delta) : 0u;
   for (int x = tid; x < W * B; x += NT) wh[x] = 0;
 
   K kn[IT];
@@ -345,6 +375,27 @@ __global__ __launch_bounds__(NT) void rp_scatter8(
     for (int x = tid; x < W * B; x += NT) wh[x] = 0;
     __syncthreads();
   }
+}
+
+template <typename K, typename KO, int MODE, int NT, int IT>
+__global__ __launch_bounds__(NT) void rp_scatter8(
+    const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t n, int64_t crows,
+    int bits, int shift, int64_t nchunks, const int64_t* __restrict__ offs,
+    KO* __restrict__ kout, int32_t* __restrict__ vout) {
+  rp_scatter8_body<K, KO, MODE, NT, IT>(keys, vals, n, crows, bits, shift, nchunks, offs, kout,
+                                        vout, blockIdx.x, 0);
+}
+
+template <int NT, int IT>
+__global__ __launch_bounds__(NT) void rp_scatter8_seg(const int32_t* __restrict__ keys,
+                                                      const RpSeg* __restrict__ segs, int nseg,
+                                                      const int64_t* __restrict__ offs,
+                                                      int32_t* __restrict__ kout,
+                                                      int32_t* __restrict__ vout) {
+  const RpSeg& g = segs[seg_of_block(segs, nseg)];
+  rp_scatter8_body<int32_t, int32_t, RP_HASH_KEY, NT, IT>(
+      keys + g.in_off, nullptr, g.n, kSegChunkRows, g.bits, g.shift, g.nchunks,
+      offs + g.hist_off, kout, vout, (int64_t)blockIdx.x - g.blk_start, g.delta);
 }
 
 __global__ void rp_offsets(const int64_t* __restrict__ offs, int64_t nchunks, int B, int64_t n,
@@ -559,6 +610,26 @@ int two_pass(const void* keys, int key_bytes, const int32_t* vals, int64_t n, in
 }  // namespace
 
 // Host-side use by the other tiers of the library.
+int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
+                             int64_t nblocks, int64_t hist_entries, int32_t* hist, int64_t* offs,
+                             void* scan_ws, int32_t* kout, int32_t* vout, hipStream_t st,
+                             const char* hist_name, const char* scatter_name) {
+  if (nseg <= 0 || nblocks <= 0) return 0;
+  static_assert(kSegChunkRows % (512 * 8) == 0, "a segment chunk is whole scatter tiles");
+  {
+    KTimer _kt(hist_name, st);
+    rp_hist_seg<<<(unsigned)nblocks, RpShape<8>::NT, 0, st>>>(keys, d_segs, nseg, hist);
+  }
+  if (launch_check("segmented partition hist")) return 1;
+  if (scan_i32_to_i64_excl(hist, offs, hist_entries, nullptr, scan_ws, st)) return 1;
+  {
+    KTimer _kt(scatter_name, st);
+    rp_scatter8_seg<512, 8><<<(unsigned)nblocks, 512, 0, st>>>(keys, d_segs, nseg, offs, kout,
+                                                               vout);
+  }
+  return launch_check("segmented partition scatter");
+}
+
 size_t radix_partition_ws(int64_t n, int bits) {
   const size_t one = carve_rp(nullptr, n, bits).bytes;
   if (bits <= kOnePassMax) return one;

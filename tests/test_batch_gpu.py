@@ -1,7 +1,8 @@
 """Batched joins (pf_batch_*): every join's pairs equal the oracle's
-unique-hash pairs of that join (probe-row order), in the batched pipeline,
-in the per-join baseline, and on hipGraph replays (several, with eager work
-between them); every tunable shape gives the same answer."""
+unique-hash pairs of that join, in the batched pipeline, in the per-join
+baseline, and on hipGraph replays (several, with eager work between them);
+every tunable shape gives the same answer; a join probed by slices writes
+its pairs in (slice, probe row) order."""
 from __future__ import annotations
 
 import numpy as np
@@ -15,6 +16,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _check_all(bj, nb, np_, bkeys, pkeys):
+    """Each join's pairs against the oracle's (probe-row order); a sliced
+    join's pairs come in slice order, so they are compared sorted by probe
+    row (a probe row has at most one pair)."""
     counts = bj.counts.cpu().numpy()
     bo = np.concatenate(([0], np.cumsum(nb)))
     po = np.concatenate(([0], np.cumsum(np_)))
@@ -25,8 +29,10 @@ def _check_all(bj, nb, np_, bkeys, pkeys):
     for j in range(len(nb)):
         rb, rp = oracle.unique_hash_pairs([bk[bo[j]:bo[j + 1]]], [pk[po[j]:po[j + 1]]])
         assert counts[j] == rb.size, j
-        assert np.array_equal(op[po[j]:po[j] + counts[j]], rp), j
-        assert np.array_equal(ob[po[j]:po[j] + counts[j]], rb), j
+        gp, gb = op[po[j]:po[j] + counts[j]], ob[po[j]:po[j] + counts[j]]
+        order = np.argsort(gp, kind="stable")
+        assert np.array_equal(gp[order], rp), j
+        assert np.array_equal(gb[order], rb), j
 
 
 @pytest.fixture(scope="module")
@@ -44,8 +50,10 @@ def test_batch_matches_oracle(small_batch, per_join):
     _check_all(bj, nb, np_, bkeys, pkeys)
 
 
-@pytest.mark.parametrize("tune", [[(4, 4096, 512)] * 3, [(2, 1024, 256), (4, 65536, 512),
-                                                         (2, 16384, 256)]])
+@pytest.mark.parametrize("tune", [[(4, 4096, 512, 0)] * 3,
+                                  [(2, 1024, 256, 0), (4, 65536, 512, 0), (2, 16384, 256, 0)],
+                                  [(2, 4096, 256, 0), (2, 4096, 512, 128), (4, 16384, 512, 256)],
+                                  [(2, 4096, 256, 64), (2, 1024, 256, 64), (2, 16384, 256, 1024)]])
 def test_every_tunable_gives_the_same_pairs(small_batch, tune):
     nb, np_, bkeys, pkeys, expect = small_batch
     bj = BatchJoin(nb, np_, tune)
@@ -92,3 +100,26 @@ def test_empty_and_tiny_joins():
     bj.run(bkeys.cuda(), pkeys.cuda())
     torch.cuda.synchronize()
     _check_all(bj, nb, np_, bkeys, pkeys)
+
+
+@pytest.mark.parametrize("per_join", [False, True])
+def test_sliced_join_order(per_join):
+    # one 40K-row join, slices of 64 KiB of its 1 MiB table: 16 slices
+    nb, np_ = np.array([40_000]), np.array([320_000])
+    g = torch.Generator().manual_seed(3)
+    bkeys = (torch.randperm(40_000, generator=g) * 3 - 5).to(torch.int32)
+    pkeys = bkeys[torch.randint(0, 40_000, (320_000,), generator=g)].clone()
+    pkeys[::7] = 2  # absent (not 1 mod 3)
+    tune = [(2, 4096, 256, 0), (2, 4096, 512, 64), (2, 16384, 512, 0)]
+    bj = BatchJoin(nb, np_, tune)
+    assert bj.info()["sliced_joins"] == 1
+    bj.run(bkeys.cuda(), pkeys.cuda(), per_join=per_join)
+    torch.cuda.synchronize()
+    _check_all(bj, nb, np_, bkeys, pkeys)
+    p, _ = bj.pairs(0)
+    p = p.cpu().numpy()
+    cap, sbits = 1 << 17, 4  # pow2 >= 2 * 40000 slots; 1 MiB >> 4 = 64 KiB
+    h = oracle.mix64(pkeys.numpy().astype(np.int64)[p])
+    sl = (h & np.uint64(cap - 1)) >> np.uint64(17 - sbits)
+    key = sl.astype(np.int64) * (1 << 32) + p
+    assert np.all(np.diff(key) > 0)  # slices ascending, probe rows ascending in a slice
