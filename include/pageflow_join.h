@@ -222,25 +222,28 @@ int pf_group_sum_count(const int32_t* gid, const int64_t* val, int64_t n, int32_
 
 /* --------------------------------------------------------- serving --
  * No counterpart in the reference (the serving tier): a resident dimension
- * table, int32 key -> int32 group in [0, G), G <= 1024, as `slots` packed
- * uint64 (pf_dim_table_slots: a power of two >= 2n); err (device int64[2])
- * counts groups out of range and repeated keys.  pf_fact_join_groupby joins
- * n fact rows (int32 foreign key, int64 amount) to the table and writes
- * SUM(amount) and COUNT(amount) (values != -64) per group into sum/cnt
- * (int64[G], zeroed first); fact rows without a match are dropped.  No
- * wait. */
+ * table, int32 key -> int32 group in [0, G), G <= 1024, in `slots` uint64
+ * of memory (pf_dim_table_slots: a power of two >= 2n).  info (device
+ * int64[4]) receives [0] the groups out of range, [1] the repeated keys,
+ * [2] the smallest key and [3] the layout: keys spanning at most 4n values
+ * with G <= 255 are stored dense (a byte a key value; [3] = the key range),
+ * others in an open-addressing table ([3] = 0).  The table and its info go
+ * together to every lookup.  pf_fact_join_groupby joins n fact rows (int32
+ * foreign key, int64 amount) to the table and writes SUM(amount) and
+ * COUNT(amount) (values != -64) per group into sum/cnt (int64[G], zeroed
+ * first); fact rows without a match are dropped.  No wait. */
 int64_t pf_dim_table_slots(int64_t n_dim);
 int pf_dim_table_build(const int32_t* key, const int32_t* grp, int64_t n, int32_t G,
-                       uint64_t* table, int64_t slots, int64_t* err, void* stream);
+                       uint64_t* table, int64_t slots, int64_t* info, void* stream);
 int pf_fact_join_groupby(const int32_t* fk, const int64_t* amt, int64_t n,
-                         const uint64_t* table, int64_t slots, int32_t G, int64_t* sum,
-                         int64_t* cnt, void* stream);
+                         const uint64_t* table, int64_t slots, const int64_t* info, int32_t G,
+                         int64_t* sum, int64_t* cnt, void* stream);
 
 /* One (request, fact rows) item of a serving step.  No counterpart in the
  * reference (the serving tier).  fk/amt: the request's fact rows to add in
- * this step (n of them); table/slots/G: the request's resident dimension
- * table (pf_dim_table_build); sum/cnt: the request's int64[G] accumulators,
- * zeroed by the caller before its first step. */
+ * this step (n of them); table/slots/info/G: the request's resident
+ * dimension table and its info (pf_dim_table_build); sum/cnt: the request's
+ * int64[G] accumulators, zeroed by the caller before its first step. */
 #define PF_SERVE_MAX_ITEMS 40
 typedef struct pf_serve_item {
   const int32_t* fk;
@@ -248,6 +251,7 @@ typedef struct pf_serve_item {
   int64_t n;
   const uint64_t* table;
   int64_t slots;
+  const int64_t* info;
   int64_t* sum;
   int64_t* cnt;
   int32_t G;

@@ -54,7 +54,7 @@ _SIGNATURES = {
     "pf_group_sum_count": (ctypes.c_int, [P, P, I64, I32, P, P, I32, P, P]),
     "pf_dim_table_slots": (I64, [I64]),
     "pf_dim_table_build": (ctypes.c_int, [P, P, I64, I32, P, I64, P, P]),
-    "pf_fact_join_groupby": (ctypes.c_int, [P, P, I64, P, I64, I32, P, P, P]),
+    "pf_fact_join_groupby": (ctypes.c_int, [P, P, I64, P, I64, P, I32, P, P, P]),
     "pf_serve_step": (ctypes.c_int, [P, I32, I32, P]),
     "pf_batch_plan_create": (P, [I32, P, P, P]),
     "pf_batch_plan_destroy": (ctypes.c_int, [P]),

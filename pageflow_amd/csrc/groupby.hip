@@ -14,12 +14,19 @@
 // group hit different banks and addresses), reduces the replicas and adds
 // its partials to the output with one global int64 atomic per group.
 //
-// Serving: a dimension table (int32 key -> int32 group in [0, G)) is an
-// open-addressing table of packed 64-bit slots (key << 32 | group, empty =
-// all ones, which no valid slot equals because a group is never -1) built
+// Serving: a dimension table (int32 key -> int32 group in [0, G)) is built
 // once and kept resident; each request streams its fact rows (int32 foreign
-// key, int64 amount) through one kernel that probes the table and
-// aggregates: 12 bytes a fact row, the algorithmic minimum.
+// key, int64 amount) through one kernel that looks the key up and
+// aggregates: 12 bytes a fact row, the algorithmic minimum.  Two layouts:
+//   dense   when the keys span at most 4n values (and G <= 255): a direct
+//           array of one byte a key value (the group, 0xFF = no row), so the
+//           table of a 2M-row dimension is 2 MB and stays in an XCD's L2
+//           beside the others (dense surrogate keys are the common case);
+//   hashed  otherwise: open addressing over packed 64-bit slots (key << 32
+//           | group, empty = all ones, which no valid slot equals because a
+//           group is never -1).
+// info (int64[4]): [0] groups out of range, [1] repeated keys, [2] the
+// smallest key, [3] the dense key range (0: hashed).
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -106,16 +113,62 @@ __device__ __forceinline__ uint32_t dim_slot(int32_t key, uint32_t mask) {
   return (uint32_t)mix64((uint64_t)(int64_t)key) & mask;
 }
 
+__global__ void k_dim_init(int64_t* info) {
+  if (threadIdx.x < 4) info[threadIdx.x] = threadIdx.x == 2 ? INT32_MAX : (threadIdx.x == 3 ? INT32_MIN : 0);
+}
+
+__global__ void k_dim_minmax(const int32_t* __restrict__ key, int64_t n, int64_t* info) {
+  int32_t lo = INT32_MAX, hi = INT32_MIN;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int32_t k = key[i];
+    lo = k < lo ? k : lo;
+    hi = k > hi ? k : hi;
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const int32_t a = __shfl_xor(lo, o, kWave), b = __shfl_xor(hi, o, kWave);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if (lane_id() == 0) {
+    atomicMin((long long*)&info[2], (long long)lo);
+    atomicMax((long long*)&info[3], (long long)hi);
+  }
+}
+
+// info[3] := the dense key range, or 0 for the hashed layout.
+__global__ void k_dim_layout(int64_t n, int64_t slots, int G, int64_t* info) {
+  const int64_t range = n > 0 ? info[3] - info[2] + 1 : 0;
+  info[3] = (n > 0 && G <= 255 && range <= 4 * n && range <= slots * 8) ? range : 0;
+}
+
 // Inserts each dimension row's (key, group).  A repeated key keeps its
-// first insertion (dimension keys are unique; a repeat sets *dup).
+// first insertion (dimension keys are unique; a repeat counts in info[1]).
 __global__ void k_dim_build(const int32_t* __restrict__ key, const int32_t* __restrict__ grp,
-                            int64_t n, uint64_t* table, uint32_t mask, int G, int64_t* err) {
+                            int64_t n, uint64_t* table, uint32_t mask, int G, int64_t* info) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t k = key[i], g = grp[i];
   if (g < 0 || g >= G) {
-    atomicAdd((unsigned long long*)&err[0], 1ULL);
+    atomicAdd((unsigned long long*)&info[0], 1ULL);
     return;
+  }
+  const int64_t range = info[3];
+  if (range > 0) {  // dense: byte k - min of the array, set by a CAS on its word
+    const uint64_t off = (uint64_t)((int64_t)k - info[2]);
+    uint32_t* wp = (uint32_t*)table + (off >> 2);
+    const int sh = (int)(off & 3) * 8;
+    uint32_t old = *wp;
+    for (;;) {
+      if (((old >> sh) & 0xFFu) != 0xFFu) {
+        atomicAdd((unsigned long long*)&info[1], 1ULL);
+        return;
+      }
+      const uint32_t nw = (old & ~(0xFFu << sh)) | ((uint32_t)g << sh);
+      const uint32_t prev = atomicCAS(wp, old, nw);
+      if (prev == old) return;
+      old = prev;
+    }
   }
   const uint64_t v = ((uint64_t)(uint32_t)k << 32) | (uint32_t)g;
   uint32_t s = dim_slot(k, mask);
@@ -126,7 +179,7 @@ __global__ void k_dim_build(const int32_t* __restrict__ key, const int32_t* __re
                        (unsigned long long)v);
     if (prev == kEmptySlot) return;
     if ((int32_t)(prev >> 32) == k) {
-      atomicAdd((unsigned long long*)&err[1], 1ULL);
+      atomicAdd((unsigned long long*)&info[1], 1ULL);
       return;
     }
     s = (s + 1) & mask;
@@ -140,8 +193,31 @@ __device__ __forceinline__ void fact_rows_into(const int32_t* __restrict__ fk,
                                                const int64_t* __restrict__ amt, int64_t i0,
                                                int64_t i1, int64_t step,
                                                const uint64_t* __restrict__ table, uint32_t mask,
-                                               int64_t* acc_rep) {
+                                               int64_t kmin, int64_t range, int64_t* acc_rep) {
   constexpr int U = 8;
+  if (range > 0) {  // dense: one byte a key value
+    const uint8_t* __restrict__ arr = (const uint8_t*)table;
+    for (int64_t b = i0; b < i1; b += step * U) {
+      int32_t k[U];
+      int64_t v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = b + u * step;
+        k[u] = i < i1 ? fk[i] : 0;
+        v[u] = i < i1 ? amt[i] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint64_t off = (uint64_t)((int64_t)k[u] - kmin);
+        const uint32_t g = b + u * step < i1 && off < (uint64_t)range ? arr[off] : 0xFFu;
+        if (g == 0xFFu) continue;
+        int64_t* a = acc_rep + (int64_t)g * 2;
+        lds_add(a, v[u]);
+        if (v[u] != kNull) lds_add(a + 1, 1);
+      }
+    }
+    return;
+  }
   for (int64_t b = i0; b < i1; b += step * U) {
     int32_t k[U];
     int64_t v[U];
@@ -187,15 +263,15 @@ __device__ __forceinline__ void flush_groups(const int64_t* acc, int G, int R, i
 // (inner join: rows without a dimension match are dropped).
 __global__ __launch_bounds__(GB_NT) void k_fact_join_groupby(
     const int32_t* __restrict__ fk, const int64_t* __restrict__ amt, int64_t n,
-    const uint64_t* __restrict__ table, uint32_t mask, int G, int R, int64_t* sum,
-    int64_t* cnt) {
+    const uint64_t* __restrict__ table, uint32_t mask, const int64_t* __restrict__ info, int G,
+    int R, int64_t* sum, int64_t* cnt) {
   __shared__ int64_t acc[32 * 1024 / 8];
   for (int x = threadIdx.x; x < R * G * 2; x += GB_NT) acc[x] = 0;
   __syncthreads();
   const int rep = (int)(lane_id() % R);
   const int64_t stride = (int64_t)gridDim.x * GB_NT;
   fact_rows_into(fk, amt, blockIdx.x * (int64_t)GB_NT + threadIdx.x, n, stride, table, mask,
-                 acc + (int64_t)rep * G * 2);
+                 info[2], info[3], acc + (int64_t)rep * G * 2);
   __syncthreads();
   flush_groups(acc, G, R, sum, cnt, GB_NT);
 }
@@ -226,7 +302,7 @@ __global__ __launch_bounds__(GB_NT) void k_serve_step(const ServeArgs a) {
   const int64_t r1 = r0 + a.rows_per_wg < it.n ? r0 + a.rows_per_wg : it.n;
   const int rep = (int)(lane_id() % R);
   fact_rows_into(it.fk, it.amt, r0 + threadIdx.x, r1, GB_NT, it.table, (uint32_t)(it.slots - 1),
-                 acc + (int64_t)rep * G * 2);
+                 it.info[2], it.info[3], acc + (int64_t)rep * G * 2);
   __syncthreads();
   flush_groups(acc, G, R, it.sum, it.cnt, GB_NT);
 }
@@ -275,27 +351,31 @@ int64_t pf_dim_table_slots(int64_t n_dim) {
 }
 
 int pf_dim_table_build(const int32_t* key, const int32_t* grp, int64_t n, int32_t G,
-                       uint64_t* table, int64_t slots, int64_t* err, void* stream) {
+                       uint64_t* table, int64_t slots, int64_t* info, void* stream) {
   using namespace pf;
   hipStream_t st = (hipStream_t)stream;
   if (n < 0 || G < 1 || G > GB_LDS_GROUPS)
     return fail("dim_table_build: need n >= 0 and 1 <= G <= 1024");
   if (slots < 2 * n || (slots & (slots - 1)) || slots > (1LL << 32))
     return fail("dim_table_build: slots must be a power of two >= 2n");
-  if (fill_i64((int64_t*)table, slots, -1, st)) return 1;
-  if (fill_i64(err, 2, 0, st)) return 1;
+  if (fill_i64((int64_t*)table, slots, -1, st)) return 1;  // empty slots, absent bytes
+  KTimer _kt("dim_table_build", st);
+  k_dim_init<<<1, 64, 0, st>>>(info);
   if (n > 0) {
-    KTimer _kt("dim_table_build", st);
-    k_dim_build<<<grid_for(n, 256), 256, 0, st>>>(key, grp, n, table, (uint32_t)(slots - 1), G,
-                                                  err);
+    const unsigned g = (unsigned)(grid_for(n, 256) < 1024 ? grid_for(n, 256) : 1024);
+    k_dim_minmax<<<g, 256, 0, st>>>(key, n, info);
   }
+  k_dim_layout<<<1, 1, 0, st>>>(n, slots, G, info);
+  if (n > 0)
+    k_dim_build<<<grid_for(n, 256), 256, 0, st>>>(key, grp, n, table, (uint32_t)(slots - 1), G,
+                                                  info);
   if (launch_check("dim_table_build")) return 1;
   return ok();
 }
 
 int pf_fact_join_groupby(const int32_t* fk, const int64_t* amt, int64_t n,
-                         const uint64_t* table, int64_t slots, int32_t G, int64_t* sum,
-                         int64_t* cnt, void* stream) {
+                         const uint64_t* table, int64_t slots, const int64_t* info, int32_t G,
+                         int64_t* sum, int64_t* cnt, void* stream) {
   using namespace pf;
   hipStream_t st = (hipStream_t)stream;
   if (n < 0 || G < 1 || G > GB_LDS_GROUPS)
@@ -305,7 +385,7 @@ int pf_fact_join_groupby(const int32_t* fk, const int64_t* amt, int64_t n,
   if (n > 0) {
     KTimer _kt("fact_join_groupby", st);
     k_fact_join_groupby<<<groupby_grid(n), GB_NT, 0, st>>>(fk, amt, n, table,
-                                                           (uint32_t)(slots - 1), G,
+                                                           (uint32_t)(slots - 1), info, G,
                                                            replicas_for(G), sum, cnt);
   }
   if (launch_check("fact_join_groupby")) return 1;
@@ -330,7 +410,7 @@ int pf_serve_step(const pf_serve_item* items, int32_t n_items, int32_t rows_per_
         (it.slots & (it.slots - 1)) || it.slots > (1LL << 32))
       return fail("serve_step: item " + std::to_string(i) +
                   ": need n >= 0, 1 <= G <= 1024 and a power-of-two table");
-    if (it.n > 0 && (!it.fk || !it.amt || !it.table || !it.sum || !it.cnt))
+    if (it.n > 0 && (!it.fk || !it.amt || !it.table || !it.info || !it.sum || !it.cnt))
       return fail("serve_step: item " + std::to_string(i) + " has a null pointer");
     a.it[i] = it;
     a.wg_begin[i] = (int32_t)wgs;

@@ -84,8 +84,10 @@ def group_sum_count(gid: torch.Tensor, val: torch.Tensor, G: int,
 
 
 class DimTable:
-    """A resident dimension hash table: int32 key -> int32 group in [0, G).
-    Built once on the device and reused by every request that joins to it."""
+    """A resident dimension table: int32 key -> int32 group in [0, G).
+    Built once on the device and reused by every request that joins to it:
+    a direct byte array when the keys are dense (they span at most 4n
+    values and G <= 255), an open-addressing hash table otherwise."""
 
     def __init__(self, keys: torch.Tensor, groups: torch.Tensor, G: int,
                  stream: Optional[torch.cuda.Stream] = None):
@@ -99,22 +101,29 @@ class DimTable:
         lib = _lib.load()
         self.slots = int(lib.pf_dim_table_slots(self.n))
         self.table = torch.empty(self.slots, dtype=torch.int64, device=self.device)
-        self.err = torch.empty(2, dtype=torch.int64, device=self.device)
+        # [0] groups out of range, [1] repeated keys, [2] smallest key,
+        # [3] dense key range (0: hashed)
+        self.info = torch.empty(4, dtype=torch.int64, device=self.device)
         st = stream if stream is not None else torch.cuda.current_stream(self.device)
         with torch.cuda.device(self.device):
             _lib.check(lib.pf_dim_table_build(_lib.ptr(keys.contiguous()),
                                               _lib.ptr(groups.contiguous()), self.n, self.G,
                                               _lib.ptr(self.table), self.slots,
-                                              _lib.ptr(self.err),
+                                              _lib.ptr(self.info),
                                               ctypes.c_void_p(st.cuda_stream)))
 
     @property
     def nbytes(self) -> int:
         return self.slots * 8
 
+    @property
+    def dense(self) -> bool:
+        """True when the table is a direct byte array (waits)."""
+        return int(self.info[3]) > 0
+
     def check(self) -> None:
         """Raises when a group was out of range or a key repeated (waits)."""
-        bad_group, dup = (int(v) for v in self.err.tolist())
+        bad_group, dup = (int(v) for v in self.info[:2].tolist())
         if bad_group or dup:
             raise ValueError(f"DimTable: {bad_group} groups out of [0, {self.G}), "
                              f"{dup} repeated keys")
@@ -134,7 +143,8 @@ class DimTable:
         lib = _lib.load()
         with torch.cuda.device(self.device):
             _lib.check(lib.pf_fact_join_groupby(_lib.ptr(fk), _lib.ptr(amt), fk.numel(),
-                                                _lib.ptr(self.table), self.slots, self.G,
+                                                _lib.ptr(self.table), self.slots,
+                                                _lib.ptr(self.info), self.G,
                                                 _lib.ptr(out[0]), _lib.ptr(out[1]),
                                                 ctypes.c_void_p(st.cuda_stream)))
         return out

@@ -53,8 +53,8 @@ class ServeItem(ctypes.Structure):
     """pf_serve_item (include/pageflow_join.h)."""
     _fields_ = [("fk", ctypes.c_void_p), ("amt", ctypes.c_void_p), ("n", ctypes.c_int64),
                 ("table", ctypes.c_void_p), ("slots", ctypes.c_int64),
-                ("sum", ctypes.c_void_p), ("cnt", ctypes.c_void_p), ("G", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("info", ctypes.c_void_p), ("sum", ctypes.c_void_p), ("cnt", ctypes.c_void_p),
+                ("G", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 @dataclass
@@ -219,6 +219,7 @@ class ServingEngine:
             it.n = take
             it.table = a.table.table.data_ptr()
             it.slots = a.table.slots
+            it.info = a.table.info.data_ptr()
             it.sum = a.acc[0].data_ptr()
             it.cnt = a.acc[1].data_ptr()
             it.G = d.G

@@ -62,3 +62,37 @@ def test_matched_rows():
     idx = torch.tensor([5, 1, 5, 7], dtype=torch.int64, device="cuda")
     assert ops.matched_rows(idx, 9, True).tolist() == [1, 5, 7]
     assert ops.matched_rows(idx, 9, False).tolist() == [0, 2, 3, 4, 6, 8]
+
+
+@pytest.mark.parametrize("layout", ["dense", "sparse", "dense_keys_many_groups"])
+def test_dimension_table_layouts(layout):
+    rng = np.random.default_rng(7)
+    nd, nf = 30_000, 400_000
+    if layout == "sparse":  # keys spread far wider than 4n values: hashed
+        dkey = rng.choice(2**30, nd, replace=False).astype(np.int32) - 2**29
+    else:  # a permutation of a dense range: a byte array unless G > 255
+        dkey = (rng.permutation(nd) - 1000).astype(np.int32)
+    G = 300 if layout == "dense_keys_many_groups" else 25
+    dgrp = rng.integers(0, G, nd).astype(np.int32)
+    fk = dkey[rng.integers(0, nd, nf)]
+    fk[rng.random(nf) < 0.2] = np.int32(2**31 - 5)  # absent, outside any key range
+    fk[::97] = np.int32(-2**31)
+    amt = rng.integers(100, 10_000_001, nf)
+    amt[::13] = -64
+    t = ops.DimTable(torch.from_numpy(dkey).cuda(), torch.from_numpy(dgrp).cuda(), G)
+    t.check()
+    assert t.dense == (layout == "dense")
+    s, c = t.join_groupby(torch.from_numpy(fk).cuda(), torch.from_numpy(amt).cuda())
+    b, p = oracle.sort_search_pairs(dkey.astype(np.int64), fk.astype(np.int64))
+    ws, wc = oracle.group_sum_count(dgrp[b], amt[p], G)
+    assert np.array_equal(s.cpu().numpy(), ws) and np.array_equal(c.cpu().numpy(), wc)
+
+
+@pytest.mark.parametrize("keys", [[1, 2, 2, 3], [1, 2**30, 2**30, -2**30]])
+def test_repeated_keys_in_either_layout(keys):
+    t = ops.DimTable(torch.tensor(keys, dtype=torch.int32, device="cuda"),
+                     torch.tensor([0, 1, 1, 2], dtype=torch.int32, device="cuda"), 25)
+    assert int(t.info[1]) == 1 and int(t.info[0]) == 0
+    assert t.dense == (keys[1] == 2)
+    with pytest.raises(ValueError):
+        t.check()

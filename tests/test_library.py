@@ -37,7 +37,7 @@ def test_serve_item_mirrors_the_header():
     body = re.search(r"typedef struct pf_serve_item \{(.*?)\} pf_serve_item;", text, re.S).group(1)
     names = re.findall(r"(\w+);", body)
     assert names == [f for f, _ in serving.ServeItem._fields_]
-    assert ctypes.sizeof(serving.ServeItem) == 64
+    assert ctypes.sizeof(serving.ServeItem) == 72
 
 
 def test_extension_module_has_the_reference_functions():
