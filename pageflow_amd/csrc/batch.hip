@@ -72,6 +72,7 @@ struct Plan {
   int64_t n_pitems_plain[2] = {0, 0};  // each group's items of joins probed whole (first)
   int64_t n_pitems_all = 0;
   int64_t total_slots = 0, total_b = 0, total_p = 0;
+  int64_t plain_slots = 0;  // the tables of joins built whole come first in the arena
   std::vector<int64_t> first_item, last_item;  // per join, in the probe item arrays
   // device
   void* dmem = nullptr;
@@ -114,6 +115,27 @@ struct Plan {
   int32_t *d_seg_join = nullptr, *d_shist = nullptr;
   int64_t* d_soffs = nullptr;
   void* d_sscan = nullptr;
+  // the sliced joins' build rows as segments (their tables are built in
+  // LDS, a slice a workgroup), the build slice offsets and the spill list
+  std::vector<RpSeg> bsegs, bsegs1;
+  int64_t bseg_blocks = 0, bseg_entries = 0;
+  RpSeg *d_bsegs = nullptr, *d_bsegs1 = nullptr;
+  int32_t *d_bk_part = nullptr, *d_bv_part = nullptr;
+  int64_t* d_bsoff = nullptr;
+  int64_t* d_spill_n = nullptr;  // spilled entries
+  int32_t* d_spill = nullptr;    // (join, key, row) triples
+  int64_t spill_cap = 0;
+  int64_t n_bitems_plain = 0;    // build items of joins built whole (listed first)
+  std::vector<int64_t> bitem_first, bitem_n;  // per join: its build items
+  // build slices (finer than the probe's: a slice is one or two LDS
+  // sub-ranges): bits per join, offsets base, the (join, slice) list
+  std::vector<int> bbits;
+  std::vector<int64_t> bsoff_base;
+  int64_t bsoff_total = 0;
+  std::vector<int32_t> bpair_join, bpair_slice;
+  std::vector<int64_t> bpair_first;  // per join: its first build pair
+  int32_t *d_bbits = nullptr, *d_bpair_join = nullptr, *d_bpair_slice = nullptr;
+  int64_t* d_bsoff_base = nullptr;
 };
 
 // A work item taken from the queue, as a wave-uniform (scalar) value.  The
@@ -475,6 +497,126 @@ __global__ __launch_bounds__(NT) void bj_probe_sliced(
   }
 }
 
+// ------------------------------------------------------- LDS build --
+//
+// A sliced join's table built a slice a workgroup: the slice's build rows
+// (partitioned by slice like the probe rows) are inserted into LDS, a
+// sub-range of LB_SR slots at a time with LB_OV slots of margin past its end,
+// and each sub-range is written out with plain coalesced stores, empty slots
+// included.  Entries placed in the margin carry into the next sub-range
+// before its own keys; entries that run past the margin, or past the slice's
+// last sub-range, are spilled and inserted afterwards by global CAS from
+// their home slot (bj_spill_insert).  Every key thus sits on its linear
+// probing path with no empty slot before it: the table answers lookups as a
+// CAS-built one does, with no global atomic a row.
+
+constexpr int LB_NT = 1024;
+constexpr int LB_SR = 16384;  // slots a sub-range
+constexpr int LB_OV = 512;    // margin past a sub-range's end
+
+__device__ __forceinline__ void lb_spill(int64_t* spill_n, int32_t* spill, int64_t cap, int j,
+                                         int32_t k, int32_t row) {
+  const int64_t i = (int64_t)atomicAdd((unsigned long long*)spill_n, 1ULL);
+  if (i < cap) {
+    spill[3 * i] = j;
+    spill[3 * i + 1] = k;
+    spill[3 * i + 2] = row;
+  }
+}
+
+__global__ __launch_bounds__(LB_NT) void bj_build_lds(
+    const int32_t* __restrict__ bk_part, const int32_t* __restrict__ bv_part,
+    const int64_t* __restrict__ b_off, const int64_t* __restrict__ t_off,
+    const uint32_t* __restrict__ mask, const int32_t* __restrict__ bbits,
+    const int32_t* __restrict__ pair_join, const int32_t* __restrict__ pair_slice, int32_t p_lo,
+    int32_t p_hi, const int64_t* __restrict__ soff_base,
+    const int64_t* __restrict__ bsoff, uint64_t* __restrict__ table, int64_t* spill_n,
+    int32_t* spill, int64_t spill_cap, int32_t* queue) {
+  __shared__ uint64_t T[LB_SR + LB_OV];
+  __shared__ uint64_t C[LB_OV];
+  __shared__ int64_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = p_lo + atomicAdd(queue, 1);
+    __syncthreads();
+    const int64_t p = uniform_item(s_item);
+    __syncthreads();
+    if (p >= p_hi) return;
+    const int j = pair_join[p];
+    const int sl = pair_slice[p];
+    const uint32_t m = mask[j];
+    const int64_t cap = (int64_t)m + 1;
+    const int64_t SL = cap >> bbits[j];
+    const int SR = SL < LB_SR ? (int)SL : LB_SR;
+    const int64_t* so = bsoff + soff_base[j];
+    const int64_t r0 = so[sl], r1 = so[sl + 1];
+    const int64_t pb = b_off[j];
+    uint64_t* t = table + t_off[j];
+    for (int i = threadIdx.x; i < LB_OV; i += LB_NT) C[i] = kEmpty;
+    for (int64_t base = (int64_t)sl * SL; base < (int64_t)(sl + 1) * SL; base += SR) {
+      __syncthreads();  // C holds the previous sub-range's margin
+      for (int i = threadIdx.x; i < SR + LB_OV; i += LB_NT) T[i] = i < LB_OV ? C[i] : kEmpty;
+      __syncthreads();
+      for (int64_t r = r0 + threadIdx.x; r < r1; r += LB_NT) {
+        const int32_t k = bk_part[pb + r];
+        const uint32_t local = (uint32_t)(bslot(k, m) - (uint32_t)base);
+        if (local >= (uint32_t)SR) continue;  // another sub-range's
+        const int32_t row = bv_part[pb + r];
+        const uint64_t v = ((uint64_t)(uint32_t)k << 32) | (uint32_t)row;
+        for (uint32_t pos = local;; ++pos) {
+          if (pos >= (uint32_t)(SR + LB_OV)) {
+            lb_spill(spill_n, spill, spill_cap, j, k, row);
+            break;
+          }
+          const uint64_t prev = atomicCAS((unsigned long long*)&T[pos],
+                                          (unsigned long long)kEmpty, (unsigned long long)v);
+          if (prev == kEmpty) break;
+          if ((uint32_t)(prev >> 32) == (uint32_t)k) {
+            atomicMin((unsigned long long*)&T[pos], (unsigned long long)v);
+            break;
+          }
+        }
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < SR; i += LB_NT) t[base + i] = T[i];
+      for (int i = threadIdx.x; i < LB_OV; i += LB_NT) C[i] = T[SR + i];
+    }
+    __syncthreads();
+    // entries past the slice's end belong to the next slice's slots
+    for (int i = threadIdx.x; i < LB_OV; i += LB_NT) {
+      const uint64_t e = C[i];
+      if (e != kEmpty) lb_spill(spill_n, spill, spill_cap, j, (int32_t)(e >> 32), (int32_t)(uint32_t)e);
+    }
+  }
+}
+
+// The spilled entries inserted by CAS from their home slot (after every
+// slice of their tables is written).
+__global__ void bj_spill_insert(const int64_t* __restrict__ spill_n,
+                                const int32_t* __restrict__ spill, int64_t cap,
+                                const int64_t* __restrict__ t_off,
+                                const uint32_t* __restrict__ mask, uint64_t* table) {
+  const int64_t n = *spill_n < cap ? *spill_n : cap;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int j = spill[3 * i];
+    const int32_t k = spill[3 * i + 1];
+    const uint64_t v = ((uint64_t)(uint32_t)k << 32) | (uint32_t)spill[3 * i + 2];
+    const uint32_t m = mask[j];
+    uint64_t* t = table + t_off[j];
+    uint32_t s = bslot(k, m);
+    for (;;) {
+      const uint64_t prev = atomicCAS((unsigned long long*)&t[s], (unsigned long long)kEmpty,
+                                      (unsigned long long)v);
+      if (prev == kEmpty) break;
+      if ((uint32_t)(prev >> 32) == (uint32_t)k) {
+        atomicMin((unsigned long long*)&t[s], (unsigned long long)v);
+        break;
+      }
+      s = (s + 1) & m;
+    }
+  }
+}
+
 __global__ void bj_counts(const int64_t* __restrict__ item_pos, const int64_t* __restrict__ first,
                           const int64_t* __restrict__ last, int nj, int64_t* __restrict__ counts) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -518,11 +660,9 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
     P.cap[j] = pow2_at_least(P.tune[P.cls[j]].load * std::max<int64_t>(nb[j], 1));
     P.b_off[j + 1] = P.b_off[j] + nb[j];
     P.p_off[j + 1] = P.p_off[j] + np[j];
-    P.t_off[j + 1] = P.t_off[j] + P.cap[j];
   }
   P.total_b = P.b_off[nj];
   P.total_p = P.p_off[nj];
-  P.total_slots = P.t_off[nj];
   // slices: 2^sbits of a table, each about slice_kb KiB of slots
   P.sbits.assign(nj, 0);
   P.log2cap.assign(nj, 0);
@@ -542,6 +682,18 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
       P.any_sliced = true;
     }
   }
+  // the table arena: the joins built whole first (their part is filled
+  // empty before a run), then the sliced joins' tables, which the LDS build
+  // writes slot by slot
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int j = 0; j < nj; ++j)
+      if ((P.sbits[j] > 0) == (pass == 1)) {
+        P.t_off[j] = P.total_slots;
+        P.total_slots += P.cap[j];
+      }
+    if (pass == 0) P.plain_slots = P.total_slots;
+  }
+  P.t_off[nj] = P.total_slots;
   // segments: the sliced joins in join order
   P.seg_of.assign(nj, -1);
   int64_t cum = 0;
@@ -567,6 +719,45 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
     P.seg_blocks += g.nchunks;
     P.seg_entries += g.nchunks << g.bits;
     cum += np[j];
+  }
+  P.bbits.assign(nj, 0);
+  P.bsoff_base.assign(nj, 0);
+  P.bpair_first.assign(nj, 0);
+  for (int j = 0; j < nj; ++j) {
+    P.bpair_first[j] = (int64_t)P.bpair_join.size();
+    if (P.sbits[j] == 0) continue;
+    // log2(LB_SR) = 14: a build slice of at most 2^14 slots where 8 bits allow
+    int bb = P.log2cap[j] - 14;
+    bb = bb < 0 ? 0 : (bb > 8 ? 8 : bb);
+    P.bbits[j] = bb;
+    P.bsoff_base[j] = P.bsoff_total;
+    P.bsoff_total += (1 << bb) + 1;
+    for (int sl = 0; sl < (1 << bb); ++sl) {
+      P.bpair_join.push_back(j);
+      P.bpair_slice.push_back(sl);
+    }
+  }
+  int64_t bcum = 0;
+  for (int j = 0; j < nj; ++j) {
+    if (P.sbits[j] == 0) continue;
+    RpSeg g{};
+    g.in_off = P.b_off[j];
+    g.n = nb[j];
+    g.nchunks = (nb[j] + kSegChunkRows - 1) / kSegChunkRows;
+    g.bits = P.bbits[j];
+    g.shift = 64 - P.log2cap[j];
+    RpSeg one = g;
+    g.blk_start = P.bseg_blocks;
+    g.hist_off = P.bseg_entries;
+    g.delta = P.b_off[j] - bcum;
+    one.blk_start = 0;
+    one.hist_off = 0;
+    one.delta = P.b_off[j];
+    P.bsegs.push_back(g);
+    P.bsegs1.push_back(one);
+    P.bseg_blocks += g.nchunks;
+    P.bseg_entries += g.nchunks << g.bits;
+    bcum += nb[j];
   }
   // the (join, slice) pairs of each queue, join-major, a one-entry gap after
   // each queue (where the queue's item total is written)
@@ -609,11 +800,22 @@ void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
   // host item lists
   std::vector<int32_t> bj, pj[2], pl[2];
   std::vector<int64_t> bs, ps[2];
-  for (int j = 0; j < njoins; ++j)
-    for (int64_t r = 0; r < P->nb[j]; r += BUILD_ROWS) {
-      bj.push_back(j);
-      bs.push_back(r);
+  // the sliced joins' build items last: bj_build takes the others (the
+  // sliced joins' tables are built in LDS, a slice a workgroup)
+  P->bitem_first.assign(njoins, 0);
+  P->bitem_n.assign(njoins, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int j = 0; j < njoins; ++j) {
+      if ((P->sbits[j] > 0) != (pass == 1)) continue;
+      P->bitem_first[j] = (int64_t)bj.size();
+      for (int64_t r = 0; r < P->nb[j]; r += BUILD_ROWS) {
+        bj.push_back(j);
+        bs.push_back(r);
+      }
+      P->bitem_n[j] = (int64_t)bj.size() - P->bitem_first[j];
     }
+    if (pass == 0) P->n_bitems_plain = (int64_t)bj.size();
+  }
   // in each workgroup-size group the sliced joins' items come last: the
   // probe kernel takes the others, the sliced probe counts those
   for (int pass = 0; pass < 2; ++pass)
@@ -682,9 +884,25 @@ void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
       P->d_segs = c.take<RpSeg>(P->segs.size());
       P->d_segs1 = c.take<RpSeg>(P->segs1.size());
       P->d_seg_join = c.take<int32_t>(P->seg_join.size());
-      P->d_shist = c.take<int32_t>(P->seg_entries);
-      P->d_soffs = c.take<int64_t>(P->seg_entries);
-      P->d_sscan = c.take<char>(scan_i32_scratch(P->seg_entries));
+      const int64_t ent = std::max(P->seg_entries, P->bseg_entries);
+      P->d_shist = c.take<int32_t>(ent);
+      P->d_soffs = c.take<int64_t>(ent);
+      P->d_sscan = c.take<char>(scan_i32_scratch(ent));
+      P->d_bsegs = c.take<RpSeg>(P->bsegs.size());
+      P->d_bsegs1 = c.take<RpSeg>(P->bsegs1.size());
+      P->d_bk_part = c.take<int32_t>(std::max<int64_t>(P->total_b, 1));
+      P->d_bv_part = c.take<int32_t>(std::max<int64_t>(P->total_b, 1));
+      P->d_bsoff = c.take<int64_t>(std::max<int64_t>(P->bsoff_total, 1));
+      P->d_bbits = c.take<int32_t>(njoins);
+      P->d_bsoff_base = c.take<int64_t>(njoins);
+      P->d_bpair_join = c.take<int32_t>(std::max<size_t>(P->bpair_join.size(), 1));
+      P->d_bpair_slice = c.take<int32_t>(std::max<size_t>(P->bpair_slice.size(), 1));
+      // spills: at most every sliced build row (a bound no build reaches)
+      P->spill_cap = 0;
+      for (int j = 0; j < njoins; ++j)
+        if (P->sbits[j] > 0) P->spill_cap += P->nb[j];
+      P->d_spill_n = c.take<int64_t>(1);
+      P->d_spill = c.take<int32_t>(3 * P->spill_cap);
     }
   };
   layout(cv);
@@ -725,7 +943,13 @@ void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
   if (P->any_sliced)
     okc = okc && up(P->d_segs, P->segs.data(), P->segs.size() * sizeof(RpSeg)) &&
           up(P->d_segs1, P->segs1.data(), P->segs1.size() * sizeof(RpSeg)) &&
-          up(P->d_seg_join, P->seg_join.data(), P->seg_join.size() * 4);
+          up(P->d_seg_join, P->seg_join.data(), P->seg_join.size() * 4) &&
+          up(P->d_bsegs, P->bsegs.data(), P->bsegs.size() * sizeof(RpSeg)) &&
+          up(P->d_bsegs1, P->bsegs1.data(), P->bsegs1.size() * sizeof(RpSeg)) &&
+          up(P->d_bbits, P->bbits.data(), njoins * 4) &&
+          up(P->d_bsoff_base, P->bsoff_base.data(), njoins * 8) &&
+          up(P->d_bpair_join, P->bpair_join.data(), P->bpair_join.size() * 4) &&
+          up(P->d_bpair_slice, P->bpair_slice.data(), P->bpair_slice.size() * 4);
   if (!okc) {
     fail("batch plan: upload failed");
     (void)hipFree(P->dmem);
@@ -787,6 +1011,38 @@ __global__ void bj_slice_bounds(const RpSeg* __restrict__ segs, const int32_t* _
         b < B ? offs[g.hist_off + (int64_t)b * g.nchunks] + g.delta - p_off[j] : g.n;
 }
 
+// The sliced joins' tables (all of them, or join `only`'s) built in LDS a
+// slice a workgroup, from their build rows partitioned by slice.
+int build_lds_stage(Plan& P, const int32_t* bkeys, int only, hipStream_t st) {
+  const int nseg = only < 0 ? (int)P.bsegs.size() : 1;
+  const int k = only < 0 ? 0 : P.seg_of[only];
+  const RpSeg* segs = only < 0 ? P.d_bsegs : P.d_bsegs1 + k;
+  const int64_t blocks = only < 0 ? P.bseg_blocks : P.bsegs1[k].nchunks;
+  const int64_t entries = only < 0 ? P.bseg_entries : P.bsegs1[k].nchunks << P.bsegs1[k].bits;
+  if (fill_i64(P.d_spill_n, 1, 0, st)) return 1;
+  if (radix_partition_segments(bkeys, segs, nseg, blocks, entries, P.d_shist, P.d_soffs,
+                               P.d_sscan, P.d_bk_part, P.d_bv_part, st, "bj_slice_hist",
+                               "bj_slice_scatter"))
+    return 1;
+  bj_slice_bounds<<<nseg, 256, 0, st>>>(segs, P.d_seg_join + k, P.d_soffs, P.d_b_off,
+                                        P.d_bsoff_base, P.d_bsoff);
+  if (launch_check("batch build slices")) return 1;
+  {
+    KTimer _kt("bj_build_lds", st);
+    const int32_t lo = only < 0 ? 0 : (int32_t)P.bpair_first[only];
+    const int32_t hi = only < 0 ? (int32_t)P.bpair_join.size()
+                                : lo + (int32_t)(1 << P.bbits[only]);
+    bj_build_lds<<<(unsigned)std::max(1, P.persist / 4), LB_NT, 0, st>>>(
+        P.d_bk_part, P.d_bv_part, P.d_b_off, P.d_t_off, P.d_mask, P.d_bbits, P.d_bpair_join,
+        P.d_bpair_slice, lo, hi, P.d_bsoff_base, P.d_bsoff, P.d_table, P.d_spill_n, P.d_spill,
+        P.spill_cap, P.d_queue + 5);
+  }
+  if (launch_check("batch LDS build")) return 1;
+  bj_spill_insert<<<64, 256, 0, st>>>(P.d_spill_n, P.d_spill, P.spill_cap, P.d_t_off, P.d_mask,
+                                      P.d_table);
+  return launch_check("batch spills");
+}
+
 // The sliced joins' probe rows partitioned by slice (all of them, or join
 // `only`), their items counted from zero, and the queues' chunk scans.
 int slice_stage(Plan& P, const int32_t* pkeys, const SliceQueues& q, int only, hipStream_t st) {
@@ -826,14 +1082,15 @@ int run_batched(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* ou
                 int64_t* out_build, int64_t* counts, hipStream_t st) {
   {
     KTimer _kt("bj_fill", st);
-    bj_fill_tables<<<2048, 256, 0, st>>>(P.d_table, P.total_slots, P.d_queue, 16);
+    bj_fill_tables<<<2048, 256, 0, st>>>(P.d_table, P.plain_slots, P.d_queue, 16);
   }
-  if (P.n_bitems > 0) {
+  if (P.n_bitems_plain > 0) {
     KTimer _kt("bj_build", st);
-    bj_build<<<(unsigned)std::min<int64_t>(P.persist, P.n_bitems), 256, 0, st>>>(
-        bkeys, P.d_b_off, P.d_t_off, P.d_mask, P.d_bitem_join, P.d_bitem_start, P.n_bitems, 0,
-        P.d_table, P.d_queue + 0);
+    bj_build<<<(unsigned)std::min<int64_t>(P.persist, P.n_bitems_plain), 256, 0, st>>>(
+        bkeys, P.d_b_off, P.d_t_off, P.d_mask, P.d_bitem_join, P.d_bitem_start,
+        P.n_bitems_plain, 0, P.d_table, P.d_queue + 0);
   }
+  if (P.any_sliced && build_lds_stage(P, bkeys, -1, st)) return 1;
   if (P.any_sliced && slice_stage(P, pkeys, P.qall, -1, st)) return 1;
   for (int g = 0; g < 2; ++g) {
     if (P.n_pitems_plain[g] == 0) continue;
@@ -881,18 +1138,19 @@ int run_batched(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* ou
 // count as a per-call entry point does.
 int run_per_join(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* out_probe,
                  int64_t* out_build, int64_t* counts, hipStream_t st) {
-  // item ranges per join in the build list
-  std::vector<int64_t> bfirst(P.nj + 1, 0);
-  for (int j = 0; j < P.nj; ++j)
-    bfirst[j + 1] = bfirst[j] + (P.nb[j] + BUILD_ROWS - 1) / BUILD_ROWS;
   for (int j = 0; j < P.nj; ++j) {
-    bj_fill_tables<<<grid_for(P.cap[j], 256) < 2048 ? grid_for(P.cap[j], 256) : 2048, 256, 0,
-                     st>>>(P.d_table + P.t_off[j], P.cap[j], P.d_queue, 16);
-    const int64_t nbi = bfirst[j + 1] - bfirst[j];
-    if (nbi > 0)
+    // (a sliced join's table is written whole by its LDS build)
+    const int64_t fill = P.sbits[j] > 0 ? 0 : P.cap[j];
+    bj_fill_tables<<<std::max<unsigned>(1, std::min<unsigned>(grid_for(fill, 256), 2048)), 256,
+                     0, st>>>(P.d_table + P.t_off[j], fill, P.d_queue, 16);
+    const int64_t nbi = P.bitem_n[j];
+    if (P.sbits[j] > 0) {
+      if (build_lds_stage(P, bkeys, j, st)) return 1;
+    } else if (nbi > 0) {
       bj_build<<<(unsigned)std::min<int64_t>(P.persist, nbi), 256, 0, st>>>(
           bkeys, P.d_b_off, P.d_t_off, P.d_mask, P.d_bitem_join, P.d_bitem_start,
-          bfirst[j + 1], bfirst[j], P.d_table, P.d_queue + 0);
+          P.bitem_first[j] + nbi, P.bitem_first[j], P.d_table, P.d_queue + 0);
+    }
     const int64_t f = P.first_item[j], l = P.last_item[j];
     if (f >= 0) {
       const int64_t n = l - f + 1;
