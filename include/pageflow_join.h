@@ -287,7 +287,9 @@ int pf_serve_step(const pf_serve_item* items, int32_t n_items, int32_t rows_per_
  * (the plain baseline).  pf_batch_capture records mode 0 on `stream` (not
  * the null stream) as a hipGraph bound to these buffers; pf_batch_replay
  * launches it.  pf_batch_plan_info: 0 build rows, 1 probe rows, 2 table
- * slots, 3 build items, 4 probe items, 5 scratch bytes, 6 sliced joins. */
+ * slots, 3 build items, 4 probe items, 5 scratch bytes, 6 sliced joins, 7
+ * the entries the last run's LDS table builds spilled to CAS inserts (waits
+ * for the device). */
 void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
                            const int32_t* tune);
 int pf_batch_plan_destroy(void* plan);

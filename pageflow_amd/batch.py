@@ -104,7 +104,7 @@ class BatchJoin:
     def info(self) -> Dict[str, int]:
         lib = _lib.load()
         names = ["build_rows", "probe_rows", "table_slots", "build_items", "probe_items",
-                 "scratch_bytes", "sliced_joins"]
+                 "scratch_bytes", "sliced_joins", "spilled"]
         return {n: int(lib.pf_batch_plan_info(self.plan, i)) for i, n in enumerate(names)}
 
     def run(self, bkeys: torch.Tensor, pkeys: torch.Tensor, per_join: bool = False,

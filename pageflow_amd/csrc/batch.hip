@@ -990,6 +990,13 @@ int64_t pf_batch_plan_info(void* plan, int32_t what) {
       for (int b : P->sbits) n += b > 0;
       return n;
     }
+    case 7: {  // entries the last run's LDS builds spilled (waits for the device)
+      int64_t n = 0;
+      if (!P->d_spill_n || hipDeviceSynchronize() != hipSuccess ||
+          hipMemcpy(&n, P->d_spill_n, 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return 0;
+      return n;
+    }
     default: return -1;
   }
 }

@@ -123,3 +123,30 @@ def test_sliced_join_order(per_join):
     sl = (h & np.uint64(cap - 1)) >> np.uint64(17 - sbits)
     key = sl.astype(np.int64) * (1 << 32) + p
     assert np.all(np.diff(key) > 0)  # slices ascending, probe rows ascending in a slice
+
+
+@pytest.mark.parametrize("per_join", [False, True])
+def test_lds_build_clusters_spill_and_wrap(per_join):
+    # one 40K-row join: 2^17 slots, probed in 16 slices, built in 8 slices of
+    # 2^14 slots; 1000 keys home in the last 8 slots of build slice 0 and 1000
+    # in the table's last 8 slots, so their clusters run past the 512-slot
+    # margin, into the next slice and around the table's end
+    cap, mask = 1 << 17, (1 << 17) - 1
+    cand = np.arange(1, 1 << 25, dtype=np.int64)
+    slot = (oracle.mix64(cand) & np.uint64(mask)).astype(np.int64)
+    near_slice = cand[(slot >= (1 << 14) - 8) & (slot < (1 << 14))][:1000]
+    near_end = cand[slot >= cap - 8][:1000]
+    rest = np.setdiff1d(cand[:200_000], np.concatenate([near_slice, near_end]))
+    rest = np.random.default_rng(5).permutation(rest)[:38_000]
+    bk = np.concatenate([near_slice, near_end, rest]).astype(np.int32)
+    assert near_slice.size == 1000 and near_end.size == 1000 and np.unique(bk).size == bk.size
+    rng = np.random.default_rng(6)
+    pk = np.concatenate([bk[rng.integers(0, bk.size, 300_000)], -rng.integers(1, 10**6, 20_000)])
+    pk = rng.permutation(pk).astype(np.int32)
+    nb, np_ = np.array([bk.size]), np.array([pk.size])
+    tune = [(2, 4096, 256, 0), (2, 4096, 512, 64), (2, 16384, 512, 0)]
+    bj = BatchJoin(nb, np_, tune)
+    bj.run(torch.from_numpy(bk).cuda(), torch.from_numpy(pk).cuda(), per_join=per_join)
+    torch.cuda.synchronize()
+    assert bj.info()["sliced_joins"] == 1 and bj.info()["spilled"] > 0
+    _check_all(bj, nb, np_, torch.from_numpy(bk), torch.from_numpy(pk))
