@@ -372,10 +372,10 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
             }
             const int li = s_hi[wb + lo] + (j - s_ex[wb + lo]);
             const int64_t o = base + j;
-            p.out_probe[o] = s_pr[wb + lo];
-            p.out_build[o] = rows[li];
-            if (has_pay) p.pay_out[0][o] = pay0[li];
-            if (p.npay > 1) p.pay_out[1][o] = p.pay_src[1][cb + li];
+            stream_store(&p.out_probe[o], (int64_t)s_pr[wb + lo]);
+            stream_store(&p.out_build[o], (int64_t)rows[li]);
+            if (has_pay) stream_store(&p.pay_out[0][o], pay0[li]);
+            if (p.npay > 1) stream_store(&p.pay_out[1][o], p.pay_src[1][cb + li]);
           }
           __syncthreads();  // s_* are rewritten for the next slot
         }
@@ -385,10 +385,10 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
           if (m[k] == 0) continue;
           const int i = i0 + k * HJ_NT + tid;
           const int64_t o = pos + cnt[k * HJ_W + w] + inw[k];
-          p.out_probe[o] = prw[i];
-          p.out_build[o] = rows[hidx[k]];
-          if (has_pay) p.pay_out[0][o] = pay0[hidx[k]];
-          if (p.npay > 1) p.pay_out[1][o] = p.pay_src[1][cb + hidx[k]];
+          stream_store(&p.out_probe[o], (int64_t)prw[i]);
+          stream_store(&p.out_build[o], (int64_t)rows[hidx[k]]);
+          if (has_pay) stream_store(&p.pay_out[0][o], pay0[hidx[k]]);
+          if (p.npay > 1) stream_store(&p.pay_out[1][o], p.pay_src[1][cb + hidx[k]]);
         }
       }
       // the round's last (slot, wave) total, from the wave that holds it

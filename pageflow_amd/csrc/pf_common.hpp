@@ -61,6 +61,11 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return (uint64_t)__ballot(p); }
 
+// A store of an output no kernel of the call reads again: marked
+// nontemporal, so the stream does not displace the staged data in L2.
+template <typename T>
+__device__ __forceinline__ void stream_store(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 // Inclusive prefix sum across the 64 lanes of a wave.
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
