@@ -359,7 +359,7 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
           s_hi[tid] = hidx[k];
           s_pr[tid] = m[k] ? prw[i] : 0;
           const int32_t T = (int32_t)__shfl(inw[k] + m[k], kWave - 1, kWave);
-          __syncthreads();
+          wave_lds_sync();  // a wave reads only its own lanes' entries
           const int64_t base = pos + cnt[k * HJ_W + w];
           for (int j = lane; j < T; j += kWave) {
             int lo = 0, hi = kWave;  // the last lane whose run starts at or before j
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
             if (has_pay) stream_store(&p.pay_out[0][o], pay0[li]);
             if (p.npay > 1) stream_store(&p.pay_out[1][o], p.pay_src[1][cb + li]);
           }
-          __syncthreads();  // s_* are rewritten for the next slot
+          wave_lds_sync();  // s_* are rewritten for the next slot
         }
       } else {
 #pragma unroll

@@ -61,6 +61,13 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return (uint64_t)__ballot(p); }
 
+// Orders a wave's LDS writes before its own later LDS reads (and the reads
+// before later writes) when lanes exchange data through LDS within the wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // A store of an output no kernel of the call reads again: marked
 // nontemporal, so the stream does not displace the staged data in L2.
 template <typename T>
