@@ -157,8 +157,9 @@ def bench_batch(args, dev, emit: bool = True):
     log("generating the 256-join batch")
     nb, np_, bkeys, pkeys, expect = make_batch(device=dev)
     tuples = float(nb.sum() + np_.sum())
-    tune = autotune(nb, np_, bkeys, pkeys, reps=2,
-                    cache=os.path.join(ROOT, "profiles", "batch_autotune.json"), device=dev)
+    cache = os.path.join(os.path.expanduser("~"), ".cache", "pageflow_amd", "batch_autotune.json")
+    os.makedirs(os.path.dirname(cache), exist_ok=True)
+    tune = autotune(nb, np_, bkeys, pkeys, reps=2, cache=cache, device=dev)
     log(f"autotuned per class: {tune}")
     # the ablation: the batch kernels a join at a time, at the better of the
     # batched tuning and the same without probe slices (slicing has fixed
