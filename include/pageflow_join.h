@@ -271,12 +271,14 @@ int pf_serve_step(const pf_serve_item* items, int32_t n_items, int32_t rows_per_
  * independent inner joins on int32 keys (unique builds): nb[j] build and
  * np[j] probe rows, stored back to back in the key buffers.  tune: NULL or
  * 12 int32, per size class of the build (< 32K rows, < 512K, larger) the
- * table slots per build row (2 or 4), the probe rows per work item, the
- * probe/compaction workgroup size (256 or 512) and the probe slice size in
- * KiB (0, or 64 to 2^20): a join whose table is larger has its probe rows
- * partitioned by the top bits of their table slot (at most 8 bits, one
- * pass) and probed slice by slice, each XCD on its own slices.  The plan
- * owns its device scratch (one allocation at creation).
+ * table slots per build row (2 or 4), the probe rows per work item (a power
+ * of two, 256 to 2^20), the probe/compaction workgroup size (256 or 512)
+ * and the probe slice size in KiB (0, or 64 to 2^20): a join whose table is
+ * larger has its probe rows partitioned by the top bits of their table slot
+ * (at most 8 bits, one pass) and probed slice by slice, a work item holding
+ * 128 KiB of its slice's table in LDS (a larger slice is taken in several
+ * such sub-ranges).  The plan owns its device scratch (one allocation at
+ * creation).
  *
  * pf_batch_join writes join j's pairs (join-local int64 probe row, build
  * row; probe-row order, within each slice for a sliced join, slices

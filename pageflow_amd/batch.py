@@ -23,16 +23,18 @@ import torch
 from . import _lib
 
 CLASSES = ("small", "medium", "large")        # build rows < 32K, < 512K, larger
-# per class: (table slots per build row, probe rows an item, workgroup size,
-# probe slice KiB).  A slice size > 0 partitions the join's probe rows by
-# the top bits of their table slot (radix bits = log2(table / slice), one
-# pass) so that each XCD probes an L2-sized part of the table; 0 probes the
-# whole table at random.
-DEFAULT_TUNE = [(2, 4096, 256, 0), (2, 16384, 256, 0), (2, 16384, 256, 2048)]
+# per class: (table slots per build row, probe rows an item (a power of
+# two), workgroup size, probe slice KiB).  A slice size > 0 partitions the
+# join's probe rows by the top bits of their table slot (radix bits =
+# log2(table / slice), at most 8, one pass); a work item then loads 128 KiB
+# of its slice's table into LDS and answers its rows' lookups there (a
+# larger slice is taken in several such sub-ranges, each re-reading the
+# slice's probe keys).  0 probes the whole table at random.
+DEFAULT_TUNE = [(2, 4096, 256, 0), (2, 16384, 256, 0), (2, 16384, 512, 128)]
 CANDIDATES = [(2, 4096, 256, 0), (4, 4096, 256, 0), (2, 16384, 256, 0), (2, 16384, 512, 0),
               (4, 16384, 512, 0), (2, 65536, 512, 0),
-              (2, 16384, 512, 1024), (2, 16384, 512, 2048), (2, 16384, 512, 4096),
-              (4, 16384, 512, 2048)]
+              (2, 16384, 512, 128), (2, 16384, 512, 256), (4, 16384, 512, 128),
+              (2, 65536, 512, 128)]
 
 
 def class_of(nb: int) -> int:

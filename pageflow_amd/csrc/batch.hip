@@ -18,12 +18,16 @@
 //   scan        item_cnt -> item_pos (items are join-major, rows ascending)
 //   bj_compact  items again: ballot-ranked stable compaction of matches
 //   bj_counts   per-join match counts
+// A join whose table exceeds the probe slice size is instead built in LDS
+// a slice at a time (bj_build_lds) and probed by slices held in LDS
+// (bj_probe_lds), both sides partitioned by the top bits of the table slot.
 //
 // Tunables per size class of the build (small < 32K rows, medium < 512K,
 // large): the table's slots per build row (2 or 4, rounded to a power of
-// two), the probe chunk (4096 or 16384 rows), and the workgroup size of the
-// probe and compaction kernels (256 or 512).  pf_batch_plan_create takes
-// them (the autotuner in pageflow_amd/batch.py measures and picks them).
+// two), the probe chunk (a power of two), the workgroup size of the probe
+// and compaction kernels (256 or 512), and the probe slice size.
+// pf_batch_plan_create takes them (the autotuner in pageflow_amd/batch.py
+// measures and picks them).
 //
 // The plain baseline (mode 1) runs the same kernels one join at a time,
 // with the reference's one host wait per join (kernel_utils.cuh:37-40).
@@ -48,12 +52,11 @@ struct Tune {
   int load;      // slots per build row (2 or 4)
   int chunk;     // probe rows an item
   int nt;        // probe / compact workgroup size (256 or 512)
-  int slice_kb;  // 0, or the table bytes (KiB) of a probe slice (sliced probe)
+  int slice_kb;  // 0, or the table bytes (KiB) of a probe slice (sliced probe in LDS)
 };
 
 int class_of(int64_t nb) { return nb < 32768 ? 0 : (nb < 524288 ? 1 : 2); }
 
-constexpr int SLICE_ROWS = 8192;  // probe rows a sliced work item
 constexpr int NXQ = 8;            // queues (XCDs)
 
 struct SliceQueues {
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(NT) void bj_probe(const int32_t* __restrict__ pkeys
     __syncthreads();
     if (item >= item_lo + n_items) return;
     const int j = item_join[item];
-    if (sliced[j]) {  // probed by bj_probe_sliced, which also counts the item
+    if (sliced[j]) {  // probed by bj_probe_lds, which also counts the item
       prev = -1;
       continue;
     }
@@ -362,22 +365,42 @@ __global__ __launch_bounds__(NT) void bj_compact(const int64_t* __restrict__ p_o
 
 // ------------------------------------------------------ sliced probe --
 //
-// A large join's table spans far more than an XCD's L2, so its random
-// probes are bound by 64-byte memory requests.  Its probe rows are first
-// partitioned (stably) by the top bits of their table slot into slices of
-// about an L2's worth of table; work items (join, slice, chunk) go to eight
-// queues, slice s of join j to queue (s + j) % 8, and workgroup b starts on
-// queue b % 8: workgroups are dealt to the eight XCDs round robin, so each
-// XCD's L2 holds the slices it probes.  A workgroup whose queue runs dry
-// takes items from the next queues, so every item is probed once whatever
-// the placement.
+// A large join's table spans far more than an XCD's L2, so random probes
+// of it are bound by 64-byte memory requests.  Its probe rows are instead
+// partitioned (stably) by the top bits of their table slot into slices, and
+// each work item loads a sub-range of PL_SR slots of its slice, plus a
+// margin, from the table into LDS with coalesced loads and answers its
+// probe rows' lookups there.  Items are (join, slice, probe chunk,
+// sub-range): a slice larger than PL_SR slots is taken by several items,
+// each probing the chunk's rows whose slot falls in its sub-range.  A
+// lookup that runs past the margin continues in the table itself.
+//
+// Items of slice s of join j go to queue (s + j) % 8, and workgroup b starts
+// on queue b % 8: workgroups are dealt to the eight XCDs round robin, so the
+// items of one slice (its chunks and sub-ranges) share an XCD's L2.  A
+// workgroup whose queue runs dry takes items from the next queues, so every
+// item is probed once whatever the placement.
 
-// Block x: each pair's chunks and their exclusive scan over the queue's
+constexpr int PL_NT = 1024;
+constexpr int PL_LG = 14;                // log2 of the slots an item holds in LDS
+constexpr int PL_SR = 1 << PL_LG;
+constexpr int PL_OV = 512;               // margin slots past the sub-range's end
+constexpr int PL_ROWS = 1 << 17;         // probe rows a sliced item (one chunk)
+
+// log2 of the sub-ranges of join j's slices
+__device__ __forceinline__ int sub_log(uint32_t m, int sbits) {
+  const int slog = 32 - __clz(m) - sbits;  // log2(slice slots); m = cap - 1
+  return slog > PL_LG ? slog - PL_LG : 0;
+}
+
+// Block x: each pair's items and their exclusive scan over the queue's
 // pairs [lo, hi) into xstart[lo..hi] (xstart[hi] = the queue's items).
 __global__ __launch_bounds__(1024) void bj_slice_scan(const int32_t* __restrict__ pair_join,
                                                       const int32_t* __restrict__ pair_slice,
                                                       const int64_t* __restrict__ soff_base,
                                                       const int64_t* __restrict__ soff,
+                                                      const uint32_t* __restrict__ mask,
+                                                      const int32_t* __restrict__ sbits,
                                                       const SliceQueues q,
                                                       int32_t* __restrict__ xstart) {
   __shared__ int32_t wsum[1024 / kWave];
@@ -388,18 +411,20 @@ __global__ __launch_bounds__(1024) void bj_slice_scan(const int32_t* __restrict_
   __syncthreads();
   for (int p0 = lo; p0 < hi; p0 += 1024) {
     const int p = p0 + (int)threadIdx.x;
-    int nch = 0;
+    int nit = 0;
     if (p < hi) {
-      const int64_t* so = soff + soff_base[pair_join[p]];
+      const int j = pair_join[p];
+      const int64_t* so = soff + soff_base[j];
       const int sl = pair_slice[p];
-      nch = (int)((so[sl + 1] - so[sl] + SLICE_ROWS - 1) / SLICE_ROWS);
+      const int nch = (int)((so[sl + 1] - so[sl] + PL_ROWS - 1) / PL_ROWS);
+      nit = nch << sub_log(mask[j], sbits[j]);
     }
-    const int incl = wave_incl_scan(nch);
+    const int incl = wave_incl_scan(nit);
     if (lane_id() == kWave - 1) wsum[threadIdx.x / kWave] = incl;
     __syncthreads();
     int before = carry;
     for (int w = 0; w < (int)threadIdx.x / kWave; ++w) before += wsum[w];
-    if (p < hi) xstart[p] = before + incl - nch;
+    if (p < hi) xstart[p] = before + incl - nit;
     __syncthreads();
     if (threadIdx.x == 1023) carry = before + incl;
     __syncthreads();
@@ -407,17 +432,21 @@ __global__ __launch_bounds__(1024) void bj_slice_scan(const int32_t* __restrict_
   if (threadIdx.x == 0) xstart[hi] = carry;
 }
 
-template <int NT>
-__global__ __launch_bounds__(NT) void bj_probe_sliced(
+__global__ __launch_bounds__(PL_NT) void bj_probe_lds(
     const int32_t* __restrict__ pk_part, const int64_t* __restrict__ p_off,
     const int64_t* __restrict__ t_off, const uint32_t* __restrict__ mask,
-    const int32_t* __restrict__ pair_join, const int32_t* __restrict__ pair_slice,
-    const int64_t* __restrict__ soff_base, const int64_t* __restrict__ soff,
-    const int32_t* __restrict__ xstart, const SliceQueues q,
-    const int64_t* __restrict__ first, const int32_t* __restrict__ chunk,
+    const int32_t* __restrict__ sbits, const int32_t* __restrict__ pair_join,
+    const int32_t* __restrict__ pair_slice, const int64_t* __restrict__ soff_base,
+    const int64_t* __restrict__ soff, const int32_t* __restrict__ xstart, const SliceQueues q,
+    const int64_t* __restrict__ first, const int32_t* __restrict__ lgchunk,
     const uint64_t* __restrict__ table, int32_t* __restrict__ partner,
     int64_t* __restrict__ item_cnt, int32_t* xqueue) {
-  __shared__ int64_t s_w[4];  // item, pair (or -1: this queue is done), items in the queue
+  __shared__ uint64_t T[PL_SR + PL_OV];
+  // matches per compaction item (chunks of 2^lgchunk partitioned rows, at
+  // least 256) that this item's rows fall in
+  constexpr int NC = PL_ROWS / 256 + 1;
+  __shared__ int32_t ccnt[NC];
+  __shared__ int64_t s_w[2];  // item, pair (or -1: this queue is done)
   int qi = 0;
   int x = (int)(blockIdx.x % NXQ);
   for (;;) {
@@ -440,7 +469,7 @@ __global__ __launch_bounds__(NT) void bj_probe_sliced(
       s_w[0] = it;
       s_w[1] = p;
     }
-    __syncthreads();
+    __syncthreads();  // also: the previous item's lookups and counts are done
     const int64_t it = uniform_item(s_w[0]);
     const int64_t p = uniform_item(s_w[1]);
     __syncthreads();
@@ -451,49 +480,105 @@ __global__ __launch_bounds__(NT) void bj_probe_sliced(
     }
     const int j = pair_join[p];
     const int sl = pair_slice[p];
-    const int64_t* so = soff + soff_base[j];
-    const int64_t r0 = so[sl] + (it - xstart[p]) * (int64_t)SLICE_ROWS;
-    const int64_t r1 = r0 + SLICE_ROWS < so[sl + 1] ? r0 + SLICE_ROWS : so[sl + 1];
-    const int64_t pb = p_off[j];
-    const uint64_t* t = table + t_off[j];
     const uint32_t m = mask[j];
-    const int64_t ci0 = first[j];
-    const int ch = chunk[j];
-    constexpr int U = 8;
-    for (int64_t b0 = r0 + threadIdx.x; b0 < r1; b0 += (int64_t)NT * U) {
-      int32_t k[U];
-      uint32_t sv[U];
-      uint64_t v[U];
+    const int sb = sbits[j];
+    const int lsub = sub_log(m, sb);
+    const int slog = 32 - __clz(m) - sb;
+    const int SR = 1 << (slog < PL_LG ? slog : PL_LG);
+    const int idx = (int)(it - xstart[p]);
+    const int sub = idx & ((1 << lsub) - 1);
+    const int64_t* so = soff + soff_base[j];
+    const int64_t r0 = so[sl] + (int64_t)(idx >> lsub) * PL_ROWS;
+    const int n = (int)(r0 + PL_ROWS < so[sl + 1] ? PL_ROWS : so[sl + 1] - r0);
+    const uint32_t base = ((uint32_t)sl << slog) + ((uint32_t)sub << PL_LG);
+    const uint64_t* __restrict__ t = table + t_off[j];
+    // the sub-range and its margin (wrapping at the table's end), two slots
+    // a load, every load of a thread in flight before the LDS writes: base
+    // and the table are even, so a pair never straddles the wrap
+    {
+      constexpr int LD = (PL_SR + PL_OV + 2 * PL_NT - 1) / (2 * PL_NT);
+      ulonglong2 tv[LD];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t r = b0 + (int64_t)u * NT;
-        k[u] = r < r1 ? pk_part[pb + r] : 0;
-        sv[u] = bslot(k[u], m);
+      for (int q = 0; q < LD; ++q) {
+        const int i = 2 * ((int)threadIdx.x + q * PL_NT);
+        if (i < SR + PL_OV) tv[q] = *(const ulonglong2*)&t[(base + (uint32_t)i) & m];
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u) v[u] = b0 + (int64_t)u * NT < r1 ? t[sv[u]] : kEmpty;
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t r = b0 + (int64_t)u * NT;
-        while (v[u] != kEmpty && (uint32_t)(v[u] >> 32) != (uint32_t)k[u]) {
-          sv[u] = (sv[u] + 1) & m;
-          v[u] = t[sv[u]];
-        }
-        const bool hit = v[u] != kEmpty;
-        if (r < r1) partner[pb + r] = hit ? (int32_t)(uint32_t)v[u] : -1;
-        // the hit counts of the compaction items (chunks of `ch` rows of the
-        // partitioned order) this wave's rows fall in: at most two
-        const int64_t rr = r < r1 ? r : r0;
-        const int64_t cw = __shfl(rr, 0, kWave) / ch;  // the first lane's item
-        const int64_t ci = rr / ch;
-        const uint64_t b_lo = ballot(hit && r < r1 && ci == cw);
-        const uint64_t b_hi = ballot(hit && r < r1 && ci != cw);
-        if (lane_id() == 0) {
-          if (b_lo) atomicAdd((unsigned long long*)&item_cnt[ci0 + cw], (unsigned long long)__popcll(b_lo));
-          if (b_hi) atomicAdd((unsigned long long*)&item_cnt[ci0 + cw + 1], (unsigned long long)__popcll(b_hi));
+      for (int q = 0; q < LD; ++q) {
+        const int i = 2 * ((int)threadIdx.x + q * PL_NT);
+        if (i < SR + PL_OV) {
+          T[i] = tv[q].x;
+          T[i + 1] = tv[q].y;
         }
       }
     }
+    const int lg = lgchunk[j];
+    const int64_t c0 = r0 >> lg;  // the first compaction item this item touches
+    const int nc = (int)(((r0 + n - 1) >> lg) - c0) + 1;
+    for (int i = threadIdx.x; i < nc; i += PL_NT) ccnt[i] = 0;
+    __syncthreads();
+    const int32_t* __restrict__ pk = pk_part + p_off[j] + r0;
+    int32_t* __restrict__ pt = partner + p_off[j] + r0;
+    const int coff = (int)(r0 & ((1 << lg) - 1));  // r0's offset in its compaction item
+    // U rows a thread a round; the next round's keys are loaded while this
+    // round's are looked up
+    constexpr int U = 8;
+    int32_t k[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = (int)threadIdx.x + u * PL_NT;
+      k[u] = pk[i < n ? i : n - 1];
+    }
+    for (int i0 = (int)threadIdx.x; i0 < n; i0 += PL_NT * U) {
+      int32_t kn[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + (U + u) * PL_NT;
+        kn[u] = pk[i < n ? i : n - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = i0 + u * PL_NT;
+        uint32_t pos = bslot(k[u], m) - base;
+        const bool mine = i < n && pos < (uint32_t)SR;
+        int32_t hit = -1;
+        if (mine) {
+          // LDS and table lookups in separate loops: an address that may be
+          // either would be a flat load, which waits for every memory
+          // operation in flight
+          uint64_t v = T[pos];
+          while (v != kEmpty && (uint32_t)(v >> 32) != (uint32_t)k[u]) {
+            if (++pos == (uint32_t)(SR + PL_OV)) break;
+            v = T[pos];
+          }
+          if (pos == (uint32_t)(SR + PL_OV)) {  // ran past the margin
+            v = t[(base + pos) & m];
+            while (v != kEmpty && (uint32_t)(v >> 32) != (uint32_t)k[u]) {
+              ++pos;
+              v = t[(base + pos) & m];
+            }
+          }
+          hit = v != kEmpty ? (int32_t)(uint32_t)v : -1;
+          pt[i] = hit;
+        }
+        // a wave's 64 consecutive rows fall in at most two compaction items
+        // (they hold at least 256 rows); lane 0 holds the wave's first row
+        const int cw = (__shfl(i, 0, kWave) + coff) >> lg;
+        const int ci = (i + coff) >> lg;
+        const uint64_t b_lo = ballot(hit >= 0 && ci == cw);
+        const uint64_t b_hi = ballot(hit >= 0 && ci != cw);
+        if (lane_id() == 0) {
+          if (b_lo) atomicAdd(&ccnt[cw], (int)__popcll(b_lo));
+          if (b_hi) atomicAdd(&ccnt[cw + 1], (int)__popcll(b_hi));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) k[u] = kn[u];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nc; i += PL_NT)
+      if (ccnt[i]) atomicAdd((unsigned long long*)&item_cnt[first[j] + c0 + i],
+                             (unsigned long long)ccnt[i]);
   }
 }
 
@@ -632,7 +717,7 @@ int64_t pow2_at_least(int64_t x) {
 int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int32_t* tune) {
   P.nj = nj;
   for (int c = 0; c < NCLASS; ++c) {
-    P.tune[c] = Tune{2, c == 0 ? 4096 : 16384, 256, c == 2 ? 2048 : 0};
+    P.tune[c] = Tune{2, c == 0 ? 4096 : 16384, c == 2 ? 512 : 256, c == 2 ? 128 : 0};
     if (tune) {
       P.tune[c].load = tune[c * 4 + 0];
       P.tune[c].chunk = tune[c * 4 + 1];
@@ -641,10 +726,10 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
     }
     const Tune& t = P.tune[c];
     if ((t.load != 2 && t.load != 4) || t.chunk < 256 || t.chunk > (1 << 20) ||
-        (t.nt != 256 && t.nt != 512) ||
+        (t.chunk & (t.chunk - 1)) || (t.nt != 256 && t.nt != 512) ||
         (t.slice_kb != 0 && (t.slice_kb < 64 || t.slice_kb > (1 << 20))))
-      return fail("batch plan: tunables are load 2|4, chunk in [256, 2^20], nt 256|512, "
-                  "slice KiB 0 or in [64, 2^20]");
+      return fail("batch plan: tunables are load 2|4, chunk a power of two in [256, 2^20], "
+                  "nt 256|512, slice KiB 0 or in [64, 2^20]");
   }
   P.nb.assign(nb, nb + nj);
   P.np.assign(np, np + nj);
@@ -933,7 +1018,9 @@ void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
   std::vector<int32_t> sliced(njoins), chunk(njoins);
   for (int j = 0; j < njoins; ++j) {
     sliced[j] = P->sbits[j];
-    chunk[j] = P->tune[P->cls[j]].chunk;
+    int lg = 0;
+    while ((1 << lg) < P->tune[P->cls[j]].chunk) ++lg;
+    chunk[j] = lg;  // log2 of the compaction items' rows
   }
   okc = okc && up(P->d_sliced, sliced.data(), njoins * 4) &&
         up(P->d_chunk, chunk.data(), njoins * 4) &&
@@ -1071,15 +1158,16 @@ int slice_stage(Plan& P, const int32_t* pkeys, const SliceQueues& q, int only, h
     return 1;
   bj_slice_bounds<<<nseg, 256, 0, st>>>(segs, P.d_seg_join + k, P.d_soffs, P.d_p_off,
                                         P.d_soff_base, P.d_soff);
-  bj_slice_scan<<<NXQ, 1024, 0, st>>>(P.d_pair_join, P.d_pair_slice, P.d_soff_base, P.d_soff, q,
-                                      P.d_xstart);
+  bj_slice_scan<<<NXQ, 1024, 0, st>>>(P.d_pair_join, P.d_pair_slice, P.d_soff_base, P.d_soff,
+                                      P.d_mask, P.d_sliced, q, P.d_xstart);
   return launch_check("batch slices");
 }
 
 int launch_sliced_probe(Plan& P, const SliceQueues& q, hipStream_t st) {
-  KTimer _kt("bj_probe_sliced", st);
-  bj_probe_sliced<512><<<P.persist, 512, 0, st>>>(
-      P.d_pk_part, P.d_p_off, P.d_t_off, P.d_mask, P.d_pair_join, P.d_pair_slice,
+  KTimer _kt("bj_probe_lds", st);
+  // one 1024-thread workgroup a CU (its LDS holds one sub-range)
+  bj_probe_lds<<<(unsigned)std::max(NXQ, P.persist / 4), PL_NT, 0, st>>>(
+      P.d_pk_part, P.d_p_off, P.d_t_off, P.d_mask, P.d_sliced, P.d_pair_join, P.d_pair_slice,
       P.d_soff_base, P.d_soff, P.d_xstart, q, P.d_first, P.d_chunk, P.d_table, P.d_partner,
       P.d_item_cnt, P.d_queue + 8);
   return launch_check("batch sliced probe");

@@ -128,16 +128,24 @@ __global__ __launch_bounds__(RpShape<8>::NT) void rp_hist_seg(const int32_t* __r
 }
 
 // Lanes of the wave holding the same digit (d == ~0u: not a row).
+template <int MAXB>
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits) {
-  const bool valid = d != ~0u;
-  const uint64_t v = ballot(valid);
-  uint64_t m = valid ? v : ~v;
-  for (int bit = 0; bit < bits; ++bit) {
-    const bool on = (d >> bit) & 1;
-    const uint64_t bal = ballot(on);
-    m &= on ? bal : ~bal;
+  // For each bit (and the valid flag), ballot(bit) XOR (all ones when this
+  // lane's bit is set) has the lanes that differ from this lane in it; the
+  // peers are the lanes that differ in none.  A bit costs a bit extract, a
+  // ballot and an XOR-OR per half of the mask.
+  const uint32_t inv = d == ~0u ? ~0u : 0u;
+  const uint64_t bi = ballot(inv != 0);
+  uint32_t lo = (uint32_t)bi ^ inv, hi = (uint32_t)(bi >> 32) ^ inv;
+#pragma unroll
+  for (int bit = 0; bit < MAXB; ++bit) {
+    if (bit >= bits) break;
+    const uint32_t on = (uint32_t)((int32_t)(d << (31 - bit)) >> 31);  // bit -> 0 or ~0
+    const uint64_t b = ballot(on != 0);
+    lo |= (uint32_t)b ^ on;
+    hi |= (uint32_t)(b >> 32) ^ on;
   }
-  return m;
+  return ~(((uint64_t)hi << 32) | lo);
 }
 
 template <typename K, typename KO, int MODE, int LOGB>
@@ -185,7 +193,7 @@ __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_scatter(
     int rk[IT];
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
-      const uint64_t peers = match_digit(d[k], bits);
+      const uint64_t peers = match_digit<LOGB>(d[k], bits);
       const int below = mbcnt(peers);
       const bool valid = d[k] != ~0u;
       const int base = valid ? (int)wh[w * B + d[k]] : 0;
@@ -319,7 +327,7 @@ __device__ __forceinline__ void rp_scatter8_body(
     int rk[IT];
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
-      const uint64_t peers = match_digit(d[k], bits);
+      const uint64_t peers = match_digit<8>(d[k], bits);
       const int below = mbcnt(peers);
       const bool valid = d[k] != ~0u;
       const int base = valid ? (int)wh[w * B + d[k]] : 0;
@@ -615,7 +623,7 @@ int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
                              void* scan_ws, int32_t* kout, int32_t* vout, hipStream_t st,
                              const char* hist_name, const char* scatter_name) {
   if (nseg <= 0 || nblocks <= 0) return 0;
-  static_assert(kSegChunkRows % (512 * 8) == 0, "a segment chunk is whole scatter tiles");
+  static_assert(kSegChunkRows % (512 * 16) == 0, "a segment chunk is whole scatter tiles");
   {
     KTimer _kt(hist_name, st);
     rp_hist_seg<<<(unsigned)nblocks, RpShape<8>::NT, 0, st>>>(keys, d_segs, nseg, hist);
@@ -624,7 +632,7 @@ int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
   if (scan_i32_to_i64_excl(hist, offs, hist_entries, nullptr, scan_ws, st)) return 1;
   {
     KTimer _kt(scatter_name, st);
-    rp_scatter8_seg<512, 8><<<(unsigned)nblocks, 512, 0, st>>>(keys, d_segs, nseg, offs, kout,
+    rp_scatter8_seg<512, 16><<<(unsigned)nblocks, 512, 0, st>>>(keys, d_segs, nseg, offs, kout,
                                                                vout);
   }
   return launch_check("segmented partition scatter");

@@ -2,7 +2,8 @@
 unique-hash pairs of that join, in the batched pipeline, in the per-join
 baseline, and on hipGraph replays (several, with eager work between them);
 every tunable shape gives the same answer; a join probed by slices writes
-its pairs in (slice, probe row) order."""
+its pairs in (slice, probe row) order, with slices held in LDS whole or in
+sub-ranges."""
 from __future__ import annotations
 
 import numpy as np
@@ -103,14 +104,16 @@ def test_empty_and_tiny_joins():
 
 
 @pytest.mark.parametrize("per_join", [False, True])
-def test_sliced_join_order(per_join):
-    # one 40K-row join, slices of 64 KiB of its 1 MiB table: 16 slices
+@pytest.mark.parametrize("slice_kb,sbits", [(64, 4), (512, 1)])
+def test_sliced_join_order(per_join, slice_kb, sbits):
+    # one 40K-row join, slices of its 1 MiB table: 16 slices of 64 KiB (each
+    # held in LDS whole), or 2 of 512 KiB (each taken in 4 sub-ranges)
     nb, np_ = np.array([40_000]), np.array([320_000])
     g = torch.Generator().manual_seed(3)
     bkeys = (torch.randperm(40_000, generator=g) * 3 - 5).to(torch.int32)
     pkeys = bkeys[torch.randint(0, 40_000, (320_000,), generator=g)].clone()
     pkeys[::7] = 2  # absent (not 1 mod 3)
-    tune = [(2, 4096, 256, 0), (2, 4096, 512, 64), (2, 16384, 512, 0)]
+    tune = [(2, 4096, 256, 0), (2, 4096, 512, slice_kb), (2, 16384, 512, 0)]
     bj = BatchJoin(nb, np_, tune)
     assert bj.info()["sliced_joins"] == 1
     bj.run(bkeys.cuda(), pkeys.cuda(), per_join=per_join)
@@ -118,7 +121,7 @@ def test_sliced_join_order(per_join):
     _check_all(bj, nb, np_, bkeys, pkeys)
     p, _ = bj.pairs(0)
     p = p.cpu().numpy()
-    cap, sbits = 1 << 17, 4  # pow2 >= 2 * 40000 slots; 1 MiB >> 4 = 64 KiB
+    cap = 1 << 17  # pow2 >= 2 * 40000 slots (1 MiB)
     h = oracle.mix64(pkeys.numpy().astype(np.int64)[p])
     sl = (h & np.uint64(cap - 1)) >> np.uint64(17 - sbits)
     key = sl.astype(np.int64) * (1 << 32) + p
