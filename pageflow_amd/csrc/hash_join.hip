@@ -287,10 +287,16 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
       int hidx[IT];
       int m[IT];
       uint64_t hv[IT];
+      // the unique write pass loads the round's probe rows here too: a
+      // load among the stores below would wait for every store before it
+      constexpr bool PR = W1 && !MULTI;
+      int32_t pr[PR ? IT : 1];
 #pragma unroll
       for (int k = 0; k < IT; ++k) {
         const int i = i0 + k * HJ_NT + tid;
-        hv[k] = ph[i < np_item ? i : np_item - 1];
+        const int ic = i < np_item ? i : np_item - 1;
+        hv[k] = ph[ic];
+        if (PR) pr[k] = prw[ic];
       }
 #pragma unroll
       for (int k = 0; k < IT; ++k) {
@@ -312,12 +318,12 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
       }
       // output offsets in row order: within a wave by lane, across (round
       // slot, wave) by one wave's scan of the HJ_IT * HJ_W totals
-      int64_t inw[HJ_IT];
+      int inw[HJ_IT];  // a wave's pairs in a round slot: at most 64 * CAP_M
 #pragma unroll
       for (int k = 0; k < HJ_IT; ++k) {
-        int64_t incl, tot;
+        int incl, tot;
         if (MULTI) {
-          incl = wave_incl_scan((int64_t)m[k]);
+          incl = wave_incl_scan(m[k]);
           tot = __shfl(incl, kWave - 1, kWave);
         } else {
           const uint64_t bal = ballot(m[k] != 0);
@@ -355,7 +361,7 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
 #pragma unroll
         for (int k = 0; k < HJ_IT; ++k) {
           const int i = i0 + k * HJ_NT + tid;
-          s_ex[tid] = (int32_t)inw[k];
+          s_ex[tid] = inw[k];
           s_hi[tid] = hidx[k];
           s_pr[tid] = m[k] ? prw[i] : 0;
           const int32_t T = (int32_t)__shfl(inw[k] + m[k], kWave - 1, kWave);
@@ -383,9 +389,8 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
 #pragma unroll
         for (int k = 0; k < HJ_IT; ++k) {
           if (m[k] == 0) continue;
-          const int i = i0 + k * HJ_NT + tid;
           const int64_t o = pos + cnt[k * HJ_W + w] + inw[k];
-          stream_store(&p.out_probe[o], (int64_t)prw[i]);
+          stream_store(&p.out_probe[o], (int64_t)pr[k]);
           stream_store(&p.out_build[o], (int64_t)rows[hidx[k]]);
           if (has_pay) stream_store(&p.pay_out[0][o], pay0[hidx[k]]);
           if (p.npay > 1) stream_store(&p.pay_out[1][o], p.pay_src[1][cb + hidx[k]]);
