@@ -85,6 +85,20 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
   return v;
 }
 
+// The same for 32-bit values with DPP: Hillis-Steele row shifts by 1, 2, 4
+// and 8 lanes inside each row of 16 (a lane with no source adds 0), then
+// lane 15 of rows 0 and 2 broadcast into rows 1 and 3, and lane 31 into
+// rows 2 and 3.  Six VALU adds, no LDS permutes.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
