@@ -275,10 +275,10 @@ int pf_serve_step(const pf_serve_item* items, int32_t n_items, int32_t rows_per_
  * of two, 256 to 2^20), the probe/compaction workgroup size (256 or 512)
  * and the probe slice size in KiB (0, or 64 to 2^20): a join whose table is
  * larger has its probe rows partitioned by the top bits of their table slot
- * (at most 8 bits, one pass) and probed slice by slice, a work item holding
- * 128 KiB of its slice's table in LDS (a larger slice is taken in several
- * such sub-ranges).  The plan owns its device scratch (one allocation at
- * creation).
+ * (at most 8 bits, one pass), its build rows too, and joined slice by
+ * slice, a work item building the table of 2^14 slots of its slice in LDS
+ * (a larger slice is taken in several such sub-ranges).  The plan owns its
+ * device scratch (one allocation at creation).
  *
  * pf_batch_join writes join j's pairs (join-local int64 probe row, build
  * row; probe-row order, within each slice for a sliced join, slices
@@ -290,7 +290,7 @@ int pf_serve_step(const pf_serve_item* items, int32_t n_items, int32_t rows_per_
  * the null stream) as a hipGraph bound to these buffers; pf_batch_replay
  * launches it.  pf_batch_plan_info: 0 build rows, 1 probe rows, 2 table
  * slots, 3 build items, 4 probe items, 5 scratch bytes, 6 sliced joins, 7
- * the entries the last run's LDS table builds spilled to CAS inserts (waits
+ * the last run's LDS tables whose build rows took more than one pass (waits
  * for the device). */
 void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
                            const int32_t* tune);

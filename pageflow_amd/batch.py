@@ -26,10 +26,11 @@ CLASSES = ("small", "medium", "large")        # build rows < 32K, < 512K, larger
 # per class: (table slots per build row, probe rows an item (a power of
 # two), workgroup size, probe slice KiB).  A slice size > 0 partitions the
 # join's probe rows by the top bits of their table slot (radix bits =
-# log2(table / slice), at most 8, one pass); a work item then loads 128 KiB
-# of its slice's table into LDS and answers its rows' lookups there (a
-# larger slice is taken in several such sub-ranges, each re-reading the
-# slice's probe keys).  0 probes the whole table at random.
+# log2(table / slice), at most 8, one pass), and so are its build rows; a
+# work item then builds the table of 2^14 slots (128 KiB) of its slice in
+# LDS from the slice's build rows and answers its probe rows' lookups there
+# (a larger slice is taken in several such sub-ranges, each re-reading the
+# slice's probe keys).  0 probes a whole table in HBM at random.
 DEFAULT_TUNE = [(2, 4096, 256, 0), (2, 16384, 256, 0), (2, 16384, 512, 128)]
 CANDIDATES = [(2, 4096, 256, 0), (4, 4096, 256, 0), (2, 16384, 256, 0), (2, 16384, 512, 0),
               (4, 16384, 512, 0), (2, 65536, 512, 0),
@@ -106,7 +107,7 @@ class BatchJoin:
     def info(self) -> Dict[str, int]:
         lib = _lib.load()
         names = ["build_rows", "probe_rows", "table_slots", "build_items", "probe_items",
-                 "scratch_bytes", "sliced_joins", "spilled"]
+                 "scratch_bytes", "sliced_joins", "lds_overflow"]
         return {n: int(lib.pf_batch_plan_info(self.plan, i)) for i, n in enumerate(names)}
 
     def run(self, bkeys: torch.Tensor, pkeys: torch.Tensor, per_join: bool = False,

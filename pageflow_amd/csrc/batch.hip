@@ -18,9 +18,9 @@
 //   scan        item_cnt -> item_pos (items are join-major, rows ascending)
 //   bj_compact  items again: ballot-ranked stable compaction of matches
 //   bj_counts   per-join match counts
-// A join whose table exceeds the probe slice size is instead built in LDS
-// a slice at a time (bj_build_lds) and probed by slices held in LDS
-// (bj_probe_lds), both sides partitioned by the top bits of the table slot.
+// A join whose table exceeds the probe slice size has no table in HBM: both
+// sides are partitioned by the top bits of the table slot, and bj_probe_lds
+// builds each slice's table in LDS and probes it there.
 //
 // Tunables per size class of the build (small < 32K rows, medium < 512K,
 // large): the table's slots per build row (2 or 4, rounded to a power of
@@ -118,27 +118,20 @@ struct Plan {
   int32_t *d_seg_join = nullptr, *d_shist = nullptr;
   int64_t* d_soffs = nullptr;
   void* d_sscan = nullptr;
-  // the sliced joins' build rows as segments (their tables are built in
-  // LDS, a slice a workgroup), the build slice offsets and the spill list
+  // the sliced joins' build rows as segments, partitioned by the same
+  // slices as their probe rows (each probe item builds its LDS table from
+  // its slice's build rows), and the build slice offsets
   std::vector<RpSeg> bsegs, bsegs1;
   int64_t bseg_blocks = 0, bseg_entries = 0;
   RpSeg *d_bsegs = nullptr, *d_bsegs1 = nullptr;
   int32_t *d_bk_part = nullptr, *d_bv_part = nullptr;
   int64_t* d_bsoff = nullptr;
-  int64_t* d_spill_n = nullptr;  // spilled entries
-  int32_t* d_spill = nullptr;    // (join, key, row) triples
-  int64_t spill_cap = 0;
-  int64_t n_bitems_plain = 0;    // build items of joins built whole (listed first)
-  std::vector<int64_t> bitem_first, bitem_n;  // per join: its build items
-  // build slices (finer than the probe's: a slice is one or two LDS
-  // sub-ranges): bits per join, offsets base, the (join, slice) list
-  std::vector<int> bbits;
   std::vector<int64_t> bsoff_base;
   int64_t bsoff_total = 0;
-  std::vector<int32_t> bpair_join, bpair_slice;
-  std::vector<int64_t> bpair_first;  // per join: its first build pair
-  int32_t *d_bbits = nullptr, *d_bpair_join = nullptr, *d_bpair_slice = nullptr;
   int64_t* d_bsoff_base = nullptr;
+  int64_t* d_overflow = nullptr;  // LDS tables that took more than one pass
+  int64_t n_bitems_plain = 0;    // build items of joins built whole (listed first)
+  std::vector<int64_t> bitem_first, bitem_n;  // per join: its build items
 };
 
 // A work item taken from the queue, as a wave-uniform (scalar) value.  The
@@ -366,14 +359,17 @@ __global__ __launch_bounds__(NT) void bj_compact(const int64_t* __restrict__ p_o
 // ------------------------------------------------------ sliced probe --
 //
 // A large join's table spans far more than an XCD's L2, so random probes
-// of it are bound by 64-byte memory requests.  Its probe rows are instead
-// partitioned (stably) by the top bits of their table slot into slices, and
-// each work item loads a sub-range of PL_SR slots of its slice, plus a
-// margin, from the table into LDS with coalesced loads and answers its
-// probe rows' lookups there.  Items are (join, slice, probe chunk,
-// sub-range): a slice larger than PL_SR slots is taken by several items,
-// each probing the chunk's rows whose slot falls in its sub-range.  A
-// lookup that runs past the margin continues in the table itself.
+// of a global table would be bound by 64-byte memory requests.  Both sides
+// of such a join are instead partitioned (stably, one pass) by the top bits
+// of the key's table slot into slices, and each work item builds the table
+// of one sub-range of 2^14 slots of its slice in LDS, from the slice's
+// build rows whose slot falls in it, then answers its probe rows' lookups
+// there.  Items are (join, slice, probe chunk, sub-range): a slice larger
+// than 2^14 slots is taken by several items, each probing the chunk's rows
+// whose slot falls in its sub-range.  An LDS table holds 7/8 of its slots;
+// a sub-range with more build rows (keys clustered by their hash) is joined
+// in several passes of build rows in slice order, a probe row keeping the
+// first (lowest) build row that answers it.
 //
 // Items of slice s of join j go to queue (s + j) % 8, and workgroup b starts
 // on queue b % 8: workgroups are dealt to the eight XCDs round robin, so the
@@ -382,9 +378,8 @@ __global__ __launch_bounds__(NT) void bj_compact(const int64_t* __restrict__ p_o
 // item is probed once whatever the placement.
 
 constexpr int PL_NT = 1024;
-constexpr int PL_LG = 14;                // log2 of the slots an item holds in LDS
+constexpr int PL_LG = 14;                // log2 of the slots of an item's LDS table
 constexpr int PL_SR = 1 << PL_LG;
-constexpr int PL_OV = 512;               // margin slots past the sub-range's end
 constexpr int PL_ROWS = 1 << 17;         // probe rows a sliced item (one chunk)
 
 // log2 of the sub-ranges of join j's slices
@@ -432,25 +427,47 @@ __global__ __launch_bounds__(1024) void bj_slice_scan(const int32_t* __restrict_
   if (threadIdx.x == 0) xstart[hi] = carry;
 }
 
+// (key, row) into an LDS table of msk + 1 slots from slot pos, linear
+// probing; a repeated key keeps its lowest row.  The table has an empty slot.
+__device__ __forceinline__ void lds_insert(uint64_t* T, uint32_t pos, uint32_t msk, int32_t k,
+                                           int32_t row) {
+  const uint64_t v = ((uint64_t)(uint32_t)k << 32) | (uint32_t)row;
+  for (;;) {
+    const uint64_t prev = atomicCAS((unsigned long long*)&T[pos], (unsigned long long)kEmpty,
+                                    (unsigned long long)v);
+    if (prev == kEmpty) return;
+    if ((uint32_t)(prev >> 32) == (uint32_t)k) {
+      atomicMin((unsigned long long*)&T[pos], (unsigned long long)v);
+      return;
+    }
+    pos = (pos + 1) & msk;
+  }
+}
+
 __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
     const int32_t* __restrict__ pk_part, const int64_t* __restrict__ p_off,
-    const int64_t* __restrict__ t_off, const uint32_t* __restrict__ mask,
+    const int32_t* __restrict__ bk_part, const int32_t* __restrict__ bv_part,
+    const int64_t* __restrict__ b_off, const int64_t* __restrict__ bsoff_base,
+    const int64_t* __restrict__ bsoff, const uint32_t* __restrict__ mask,
     const int32_t* __restrict__ sbits, const int32_t* __restrict__ pair_join,
     const int32_t* __restrict__ pair_slice, const int64_t* __restrict__ soff_base,
     const int64_t* __restrict__ soff, const int32_t* __restrict__ xstart, const SliceQueues q,
     const int64_t* __restrict__ first, const int32_t* __restrict__ lgchunk,
-    const uint64_t* __restrict__ table, int32_t* __restrict__ partner,
-    int64_t* __restrict__ item_cnt, int32_t* xqueue) {
-  __shared__ uint64_t T[PL_SR + PL_OV];
+    int32_t* __restrict__ partner, int64_t* __restrict__ item_cnt, int64_t* overflow,
+    int32_t* xqueue) {
+  __shared__ uint64_t T[PL_SR];
   // matches per compaction item (chunks of 2^lgchunk partitioned rows, at
   // least 256) that this item's rows fall in
   constexpr int NC = PL_ROWS / 256 + 1;
+  constexpr int W = PL_NT / kWave;
   __shared__ int32_t ccnt[NC];
+  __shared__ int32_t s_red[W];
   __shared__ int64_t s_w[2];  // item, pair (or -1: this queue is done)
+  const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   int qi = 0;
   int x = (int)(blockIdx.x % NXQ);
   for (;;) {
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       const int lo = q.lo[x], hi = q.hi[x];
       const int n = xstart[hi];
       const int it = atomicAdd(&xqueue[x], 1);
@@ -485,220 +502,134 @@ __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
     const int lsub = sub_log(m, sb);
     const int slog = 32 - __clz(m) - sb;
     const int SR = 1 << (slog < PL_LG ? slog : PL_LG);
+    const int cap = SR - SR / 8;  // build rows a pass holds
     const int idx = (int)(it - xstart[p]);
     const int sub = idx & ((1 << lsub) - 1);
     const int64_t* so = soff + soff_base[j];
     const int64_t r0 = so[sl] + (int64_t)(idx >> lsub) * PL_ROWS;
-    const int n = (int)(r0 + PL_ROWS < so[sl + 1] ? PL_ROWS : so[sl + 1] - r0);
+    const int n = __builtin_amdgcn_readfirstlane(
+        (int)(r0 + PL_ROWS < so[sl + 1] ? PL_ROWS : so[sl + 1] - r0));
     const uint32_t base = ((uint32_t)sl << slog) + ((uint32_t)sub << PL_LG);
-    const uint64_t* __restrict__ t = table + t_off[j];
-    // the sub-range and its margin (wrapping at the table's end), two slots
-    // a load, every load of a thread in flight before the LDS writes: base
-    // and the table are even, so a pair never straddles the wrap
-    {
-      constexpr int LD = (PL_SR + PL_OV + 2 * PL_NT - 1) / (2 * PL_NT);
-      ulonglong2 tv[LD];
+    // the slice's build rows
+    const int64_t* bso = bsoff + bsoff_base[j];
+    const int32_t* __restrict__ bk = bk_part + b_off[j] + bso[sl];
+    const int32_t* __restrict__ bv = bv_part + b_off[j] + bso[sl];
+    const int nbs = __builtin_amdgcn_readfirstlane((int)(bso[sl + 1] - bso[sl]));
+    // the build rows of this sub-range (all of the slice's when it is one)
+    int nin = nbs;
+    if (lsub > 0) {
+      int c = 0;
+      for (int r = tid; r < nbs; r += PL_NT) c += (bslot(bk[r], m) - base) < (uint32_t)SR;
+      c = wave_sum(c);
+      if (lane == 0) s_red[w] = c;
+      __syncthreads();
+      nin = 0;
 #pragma unroll
-      for (int q = 0; q < LD; ++q) {
-        const int i = 2 * ((int)threadIdx.x + q * PL_NT);
-        if (i < SR + PL_OV) tv[q] = *(const ulonglong2*)&t[(base + (uint32_t)i) & m];
-      }
-#pragma unroll
-      for (int q = 0; q < LD; ++q) {
-        const int i = 2 * ((int)threadIdx.x + q * PL_NT);
-        if (i < SR + PL_OV) {
-          T[i] = tv[q].x;
-          T[i + 1] = tv[q].y;
-        }
-      }
+      for (int v = 0; v < W; ++v) nin += s_red[v];
+      nin = __builtin_amdgcn_readfirstlane(nin);
+      __syncthreads();
     }
+    const int npass = nin <= cap ? 1 : (nin + cap - 1) / cap;
+    if (npass > 1 && tid == 0) atomicAdd((unsigned long long*)overflow, 1ULL);
     const int lg = lgchunk[j];
     const int64_t c0 = r0 >> lg;  // the first compaction item this item touches
     const int nc = (int)(((r0 + n - 1) >> lg) - c0) + 1;
-    for (int i = threadIdx.x; i < nc; i += PL_NT) ccnt[i] = 0;
-    __syncthreads();
+    for (int i = tid; i < nc; i += PL_NT) ccnt[i] = 0;
     const int32_t* __restrict__ pk = pk_part + p_off[j] + r0;
     int32_t* __restrict__ pt = partner + p_off[j] + r0;
     const int coff = (int)(r0 & ((1 << lg) - 1));  // r0's offset in its compaction item
-    // U rows a thread a round; the next round's keys are loaded while this
-    // round's are looked up
-    constexpr int U = 8;
-    int32_t k[U];
+    for (int pass = 0; pass < npass; ++pass) {
+      for (int i = tid; i < SR; i += PL_NT) T[i] = kEmpty;
+      __syncthreads();
+      if (npass == 1) {
+        for (int r = tid; r < nbs; r += PL_NT) {
+          const int32_t k = bk[r];
+          const uint32_t pos = bslot(k, m) - base;
+          if (pos < (uint32_t)SR) lds_insert(T, pos, SR - 1, k, bv[r]);
+        }
+      } else {
+        // pass `pass` takes the sub-range's build rows of rank [pass * cap,
+        // (pass + 1) * cap) in slice order
+        int before = 0;
+        for (int rb = 0; rb < nbs; rb += PL_NT) {
+          const int r = rb + tid;
+          const int32_t k = r < nbs ? bk[r] : 0;
+          const uint32_t pos = bslot(k, m) - base;
+          const bool in = r < nbs && pos < (uint32_t)SR;
+          const uint64_t bal = ballot(in);
+          if (lane == 0) s_red[w] = (int)__popcll(bal);
+          __syncthreads();
+          int pre = 0, tot = 0;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int i = (int)threadIdx.x + u * PL_NT;
-      k[u] = pk[i < n ? i : n - 1];
-    }
-    for (int i0 = (int)threadIdx.x; i0 < n; i0 += PL_NT * U) {
-      int32_t kn[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = i0 + (U + u) * PL_NT;
-        kn[u] = pk[i < n ? i : n - 1];
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = i0 + u * PL_NT;
-        uint32_t pos = bslot(k[u], m) - base;
-        const bool mine = i < n && pos < (uint32_t)SR;
-        int32_t hit = -1;
-        if (mine) {
-          // LDS and table lookups in separate loops: an address that may be
-          // either would be a flat load, which waits for every memory
-          // operation in flight
-          uint64_t v = T[pos];
-          while (v != kEmpty && (uint32_t)(v >> 32) != (uint32_t)k[u]) {
-            if (++pos == (uint32_t)(SR + PL_OV)) break;
-            v = T[pos];
+          for (int v = 0; v < W; ++v) {
+            const int c = s_red[v];
+            pre += v < w ? c : 0;
+            tot += c;
           }
-          if (pos == (uint32_t)(SR + PL_OV)) {  // ran past the margin
-            v = t[(base + pos) & m];
+          const int rank = before + pre + mbcnt(bal);
+          if (in && rank >= pass * cap && rank < (pass + 1) * cap)
+            lds_insert(T, pos, SR - 1, k, bv[r]);
+          before += tot;
+          __syncthreads();
+        }
+      }
+      __syncthreads();
+      // U rows a thread a round; the next round's keys are loaded while this
+      // round's are looked up
+      constexpr int U = 8;
+      int32_t k[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int i = tid + u * PL_NT;
+        k[u] = pk[i < n ? i : n - 1];
+      }
+      for (int i0 = tid; i0 < n; i0 += PL_NT * U) {
+        int32_t kn[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int i = i0 + (U + u) * PL_NT;
+          kn[u] = pk[i < n ? i : n - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int i = i0 + u * PL_NT;
+          uint32_t pos = bslot(k[u], m) - base;
+          const bool mine = i < n && pos < (uint32_t)SR;
+          int32_t hit = -1;  // a match new in this pass
+          if (mine) {
+            uint64_t v = T[pos];
             while (v != kEmpty && (uint32_t)(v >> 32) != (uint32_t)k[u]) {
-              ++pos;
-              v = t[(base + pos) & m];
+              pos = (pos + 1) & (uint32_t)(SR - 1);
+              v = T[pos];
             }
+            hit = v != kEmpty ? (int32_t)(uint32_t)v : -1;
+            if (pass == 0)
+              pt[i] = hit;
+            else if (hit >= 0 && pt[i] < 0)  // this thread's own earlier write
+              pt[i] = hit;
+            else
+              hit = -1;
           }
-          hit = v != kEmpty ? (int32_t)(uint32_t)v : -1;
-          pt[i] = hit;
+          // a wave's 64 consecutive rows fall in at most two compaction
+          // items (they hold at least 256 rows); lane 0 holds the wave's
+          // first row
+          const int cw = (__shfl(i, 0, kWave) + coff) >> lg;
+          const int ci = (i + coff) >> lg;
+          const uint64_t b_lo = ballot(hit >= 0 && ci == cw);
+          const uint64_t b_hi = ballot(hit >= 0 && ci != cw);
+          if (lane == 0) {
+            if (b_lo) atomicAdd(&ccnt[cw], (int)__popcll(b_lo));
+            if (b_hi) atomicAdd(&ccnt[cw + 1], (int)__popcll(b_hi));
+          }
         }
-        // a wave's 64 consecutive rows fall in at most two compaction items
-        // (they hold at least 256 rows); lane 0 holds the wave's first row
-        const int cw = (__shfl(i, 0, kWave) + coff) >> lg;
-        const int ci = (i + coff) >> lg;
-        const uint64_t b_lo = ballot(hit >= 0 && ci == cw);
-        const uint64_t b_hi = ballot(hit >= 0 && ci != cw);
-        if (lane_id() == 0) {
-          if (b_lo) atomicAdd(&ccnt[cw], (int)__popcll(b_lo));
-          if (b_hi) atomicAdd(&ccnt[cw + 1], (int)__popcll(b_hi));
-        }
-      }
 #pragma unroll
-      for (int u = 0; u < U; ++u) k[u] = kn[u];
+        for (int u = 0; u < U; ++u) k[u] = kn[u];
+      }
+      __syncthreads();  // the table is rebuilt by the next pass
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nc; i += PL_NT)
+    for (int i = tid; i < nc; i += PL_NT)
       if (ccnt[i]) atomicAdd((unsigned long long*)&item_cnt[first[j] + c0 + i],
                              (unsigned long long)ccnt[i]);
-  }
-}
-
-// ------------------------------------------------------- LDS build --
-//
-// A sliced join's table built a slice a workgroup: the slice's build rows
-// (partitioned by slice like the probe rows) are inserted into LDS, a
-// sub-range of LB_SR slots at a time with LB_OV slots of margin past its end,
-// and each sub-range is written out with plain coalesced stores, empty slots
-// included.  Entries placed in the margin carry into the next sub-range
-// before its own keys; entries that run past the margin, or past the slice's
-// last sub-range, are spilled and inserted afterwards by global CAS from
-// their home slot (bj_spill_insert).  Every key thus sits on its linear
-// probing path with no empty slot before it: the table answers lookups as a
-// CAS-built one does, with no global atomic a row.
-
-constexpr int LB_NT = 1024;
-constexpr int LB_SR = 16384;  // slots a sub-range
-constexpr int LB_OV = 512;    // margin past a sub-range's end
-
-__device__ __forceinline__ void lb_spill(int64_t* spill_n, int32_t* spill, int64_t cap, int j,
-                                         int32_t k, int32_t row) {
-  const int64_t i = (int64_t)atomicAdd((unsigned long long*)spill_n, 1ULL);
-  if (i < cap) {
-    spill[3 * i] = j;
-    spill[3 * i + 1] = k;
-    spill[3 * i + 2] = row;
-  }
-}
-
-__global__ __launch_bounds__(LB_NT) void bj_build_lds(
-    const int32_t* __restrict__ bk_part, const int32_t* __restrict__ bv_part,
-    const int64_t* __restrict__ b_off, const int64_t* __restrict__ t_off,
-    const uint32_t* __restrict__ mask, const int32_t* __restrict__ bbits,
-    const int32_t* __restrict__ pair_join, const int32_t* __restrict__ pair_slice, int32_t p_lo,
-    int32_t p_hi, const int64_t* __restrict__ soff_base,
-    const int64_t* __restrict__ bsoff, uint64_t* __restrict__ table, int64_t* spill_n,
-    int32_t* spill, int64_t spill_cap, int32_t* queue) {
-  __shared__ uint64_t T[LB_SR + LB_OV];
-  __shared__ uint64_t C[LB_OV];
-  __shared__ int64_t s_item;
-  for (;;) {
-    if (threadIdx.x == 0) s_item = p_lo + atomicAdd(queue, 1);
-    __syncthreads();
-    const int64_t p = uniform_item(s_item);
-    __syncthreads();
-    if (p >= p_hi) return;
-    const int j = pair_join[p];
-    const int sl = pair_slice[p];
-    const uint32_t m = mask[j];
-    const int64_t cap = (int64_t)m + 1;
-    const int64_t SL = cap >> bbits[j];
-    const int SR = SL < LB_SR ? (int)SL : LB_SR;
-    const int64_t* so = bsoff + soff_base[j];
-    const int64_t r0 = so[sl], r1 = so[sl + 1];
-    const int64_t pb = b_off[j];
-    uint64_t* t = table + t_off[j];
-    for (int i = threadIdx.x; i < LB_OV; i += LB_NT) C[i] = kEmpty;
-    for (int64_t base = (int64_t)sl * SL; base < (int64_t)(sl + 1) * SL; base += SR) {
-      __syncthreads();  // C holds the previous sub-range's margin
-      for (int i = threadIdx.x; i < SR + LB_OV; i += LB_NT) T[i] = i < LB_OV ? C[i] : kEmpty;
-      __syncthreads();
-      for (int64_t r = r0 + threadIdx.x; r < r1; r += LB_NT) {
-        const int32_t k = bk_part[pb + r];
-        const uint32_t local = (uint32_t)(bslot(k, m) - (uint32_t)base);
-        if (local >= (uint32_t)SR) continue;  // another sub-range's
-        const int32_t row = bv_part[pb + r];
-        const uint64_t v = ((uint64_t)(uint32_t)k << 32) | (uint32_t)row;
-        for (uint32_t pos = local;; ++pos) {
-          if (pos >= (uint32_t)(SR + LB_OV)) {
-            lb_spill(spill_n, spill, spill_cap, j, k, row);
-            break;
-          }
-          const uint64_t prev = atomicCAS((unsigned long long*)&T[pos],
-                                          (unsigned long long)kEmpty, (unsigned long long)v);
-          if (prev == kEmpty) break;
-          if ((uint32_t)(prev >> 32) == (uint32_t)k) {
-            atomicMin((unsigned long long*)&T[pos], (unsigned long long)v);
-            break;
-          }
-        }
-      }
-      __syncthreads();
-      for (int i = threadIdx.x; i < SR; i += LB_NT) t[base + i] = T[i];
-      for (int i = threadIdx.x; i < LB_OV; i += LB_NT) C[i] = T[SR + i];
-    }
-    __syncthreads();
-    // entries past the slice's end belong to the next slice's slots
-    for (int i = threadIdx.x; i < LB_OV; i += LB_NT) {
-      const uint64_t e = C[i];
-      if (e != kEmpty) lb_spill(spill_n, spill, spill_cap, j, (int32_t)(e >> 32), (int32_t)(uint32_t)e);
-    }
-  }
-}
-
-// The spilled entries inserted by CAS from their home slot (after every
-// slice of their tables is written).
-__global__ void bj_spill_insert(const int64_t* __restrict__ spill_n,
-                                const int32_t* __restrict__ spill, int64_t cap,
-                                const int64_t* __restrict__ t_off,
-                                const uint32_t* __restrict__ mask, uint64_t* table) {
-  const int64_t n = *spill_n < cap ? *spill_n : cap;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int j = spill[3 * i];
-    const int32_t k = spill[3 * i + 1];
-    const uint64_t v = ((uint64_t)(uint32_t)k << 32) | (uint32_t)spill[3 * i + 2];
-    const uint32_t m = mask[j];
-    uint64_t* t = table + t_off[j];
-    uint32_t s = bslot(k, m);
-    for (;;) {
-      const uint64_t prev = atomicCAS((unsigned long long*)&t[s], (unsigned long long)kEmpty,
-                                      (unsigned long long)v);
-      if (prev == kEmpty) break;
-      if ((uint32_t)(prev >> 32) == (uint32_t)k) {
-        atomicMin((unsigned long long*)&t[s], (unsigned long long)v);
-        break;
-      }
-      s = (s + 1) & m;
-    }
   }
 }
 
@@ -767,17 +698,13 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
       P.any_sliced = true;
     }
   }
-  // the table arena: the joins built whole first (their part is filled
-  // empty before a run), then the sliced joins' tables, which the LDS build
-  // writes slot by slot
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int j = 0; j < nj; ++j)
-      if ((P.sbits[j] > 0) == (pass == 1)) {
-        P.t_off[j] = P.total_slots;
-        P.total_slots += P.cap[j];
-      }
-    if (pass == 0) P.plain_slots = P.total_slots;
+  // the table arena holds the joins probed whole (filled empty before a
+  // run); a sliced join's table lives only in LDS, a sub-range an item
+  for (int j = 0; j < nj; ++j) {
+    P.t_off[j] = P.total_slots;
+    if (P.sbits[j] == 0) P.total_slots += P.cap[j];
   }
+  P.plain_slots = P.total_slots;
   P.t_off[nj] = P.total_slots;
   // segments: the sliced joins in join order
   P.seg_of.assign(nj, -1);
@@ -805,22 +732,11 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
     P.seg_entries += g.nchunks << g.bits;
     cum += np[j];
   }
-  P.bbits.assign(nj, 0);
   P.bsoff_base.assign(nj, 0);
-  P.bpair_first.assign(nj, 0);
   for (int j = 0; j < nj; ++j) {
-    P.bpair_first[j] = (int64_t)P.bpair_join.size();
     if (P.sbits[j] == 0) continue;
-    // log2(LB_SR) = 14: a build slice of at most 2^14 slots where 8 bits allow
-    int bb = P.log2cap[j] - 14;
-    bb = bb < 0 ? 0 : (bb > 8 ? 8 : bb);
-    P.bbits[j] = bb;
     P.bsoff_base[j] = P.bsoff_total;
-    P.bsoff_total += (1 << bb) + 1;
-    for (int sl = 0; sl < (1 << bb); ++sl) {
-      P.bpair_join.push_back(j);
-      P.bpair_slice.push_back(sl);
-    }
+    P.bsoff_total += (1 << P.sbits[j]) + 1;
   }
   int64_t bcum = 0;
   for (int j = 0; j < nj; ++j) {
@@ -829,7 +745,7 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
     g.in_off = P.b_off[j];
     g.n = nb[j];
     g.nchunks = (nb[j] + kSegChunkRows - 1) / kSegChunkRows;
-    g.bits = P.bbits[j];
+    g.bits = P.sbits[j];
     g.shift = 64 - P.log2cap[j];
     RpSeg one = g;
     g.blk_start = P.bseg_blocks;
@@ -978,16 +894,8 @@ void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
       P->d_bk_part = c.take<int32_t>(std::max<int64_t>(P->total_b, 1));
       P->d_bv_part = c.take<int32_t>(std::max<int64_t>(P->total_b, 1));
       P->d_bsoff = c.take<int64_t>(std::max<int64_t>(P->bsoff_total, 1));
-      P->d_bbits = c.take<int32_t>(njoins);
       P->d_bsoff_base = c.take<int64_t>(njoins);
-      P->d_bpair_join = c.take<int32_t>(std::max<size_t>(P->bpair_join.size(), 1));
-      P->d_bpair_slice = c.take<int32_t>(std::max<size_t>(P->bpair_slice.size(), 1));
-      // spills: at most every sliced build row (a bound no build reaches)
-      P->spill_cap = 0;
-      for (int j = 0; j < njoins; ++j)
-        if (P->sbits[j] > 0) P->spill_cap += P->nb[j];
-      P->d_spill_n = c.take<int64_t>(1);
-      P->d_spill = c.take<int32_t>(3 * P->spill_cap);
+      P->d_overflow = c.take<int64_t>(1);
     }
   };
   layout(cv);
@@ -1033,10 +941,7 @@ void* pf_batch_plan_create(int32_t njoins, const int64_t* nb, const int64_t* np,
           up(P->d_seg_join, P->seg_join.data(), P->seg_join.size() * 4) &&
           up(P->d_bsegs, P->bsegs.data(), P->bsegs.size() * sizeof(RpSeg)) &&
           up(P->d_bsegs1, P->bsegs1.data(), P->bsegs1.size() * sizeof(RpSeg)) &&
-          up(P->d_bbits, P->bbits.data(), njoins * 4) &&
-          up(P->d_bsoff_base, P->bsoff_base.data(), njoins * 8) &&
-          up(P->d_bpair_join, P->bpair_join.data(), P->bpair_join.size() * 4) &&
-          up(P->d_bpair_slice, P->bpair_slice.data(), P->bpair_slice.size() * 4);
+          up(P->d_bsoff_base, P->bsoff_base.data(), njoins * 8);
   if (!okc) {
     fail("batch plan: upload failed");
     (void)hipFree(P->dmem);
@@ -1077,10 +982,10 @@ int64_t pf_batch_plan_info(void* plan, int32_t what) {
       for (int b : P->sbits) n += b > 0;
       return n;
     }
-    case 7: {  // entries the last run's LDS builds spilled (waits for the device)
+    case 7: {  // the last run's LDS tables that took more than one pass (waits)
       int64_t n = 0;
-      if (!P->d_spill_n || hipDeviceSynchronize() != hipSuccess ||
-          hipMemcpy(&n, P->d_spill_n, 8, hipMemcpyDeviceToHost) != hipSuccess)
+      if (!P->d_overflow || hipDeviceSynchronize() != hipSuccess ||
+          hipMemcpy(&n, P->d_overflow, 8, hipMemcpyDeviceToHost) != hipSuccess)
         return 0;
       return n;
     }
@@ -1105,36 +1010,22 @@ __global__ void bj_slice_bounds(const RpSeg* __restrict__ segs, const int32_t* _
         b < B ? offs[g.hist_off + (int64_t)b * g.nchunks] + g.delta - p_off[j] : g.n;
 }
 
-// The sliced joins' tables (all of them, or join `only`'s) built in LDS a
-// slice a workgroup, from their build rows partitioned by slice.
-int build_lds_stage(Plan& P, const int32_t* bkeys, int only, hipStream_t st) {
+// The sliced joins' build rows (all of them, or join `only`'s) partitioned
+// by slice, as their probe rows are.
+int build_part_stage(Plan& P, const int32_t* bkeys, int only, hipStream_t st) {
   const int nseg = only < 0 ? (int)P.bsegs.size() : 1;
   const int k = only < 0 ? 0 : P.seg_of[only];
   const RpSeg* segs = only < 0 ? P.d_bsegs : P.d_bsegs1 + k;
   const int64_t blocks = only < 0 ? P.bseg_blocks : P.bsegs1[k].nchunks;
   const int64_t entries = only < 0 ? P.bseg_entries : P.bsegs1[k].nchunks << P.bsegs1[k].bits;
-  if (fill_i64(P.d_spill_n, 1, 0, st)) return 1;
+  if (fill_i64(P.d_overflow, 1, 0, st)) return 1;
   if (radix_partition_segments(bkeys, segs, nseg, blocks, entries, P.d_shist, P.d_soffs,
                                P.d_sscan, P.d_bk_part, P.d_bv_part, st, "bj_slice_hist",
                                "bj_slice_scatter"))
     return 1;
   bj_slice_bounds<<<nseg, 256, 0, st>>>(segs, P.d_seg_join + k, P.d_soffs, P.d_b_off,
                                         P.d_bsoff_base, P.d_bsoff);
-  if (launch_check("batch build slices")) return 1;
-  {
-    KTimer _kt("bj_build_lds", st);
-    const int32_t lo = only < 0 ? 0 : (int32_t)P.bpair_first[only];
-    const int32_t hi = only < 0 ? (int32_t)P.bpair_join.size()
-                                : lo + (int32_t)(1 << P.bbits[only]);
-    bj_build_lds<<<(unsigned)std::max(1, P.persist / 4), LB_NT, 0, st>>>(
-        P.d_bk_part, P.d_bv_part, P.d_b_off, P.d_t_off, P.d_mask, P.d_bbits, P.d_bpair_join,
-        P.d_bpair_slice, lo, hi, P.d_bsoff_base, P.d_bsoff, P.d_table, P.d_spill_n, P.d_spill,
-        P.spill_cap, P.d_queue + 5);
-  }
-  if (launch_check("batch LDS build")) return 1;
-  bj_spill_insert<<<64, 256, 0, st>>>(P.d_spill_n, P.d_spill, P.spill_cap, P.d_t_off, P.d_mask,
-                                      P.d_table);
-  return launch_check("batch spills");
+  return launch_check("batch build slices");
 }
 
 // The sliced joins' probe rows partitioned by slice (all of them, or join
@@ -1167,9 +1058,9 @@ int launch_sliced_probe(Plan& P, const SliceQueues& q, hipStream_t st) {
   KTimer _kt("bj_probe_lds", st);
   // one 1024-thread workgroup a CU (its LDS holds one sub-range)
   bj_probe_lds<<<(unsigned)std::max(NXQ, P.persist / 4), PL_NT, 0, st>>>(
-      P.d_pk_part, P.d_p_off, P.d_t_off, P.d_mask, P.d_sliced, P.d_pair_join, P.d_pair_slice,
-      P.d_soff_base, P.d_soff, P.d_xstart, q, P.d_first, P.d_chunk, P.d_table, P.d_partner,
-      P.d_item_cnt, P.d_queue + 8);
+      P.d_pk_part, P.d_p_off, P.d_bk_part, P.d_bv_part, P.d_b_off, P.d_bsoff_base, P.d_bsoff,
+      P.d_mask, P.d_sliced, P.d_pair_join, P.d_pair_slice, P.d_soff_base, P.d_soff, P.d_xstart,
+      q, P.d_first, P.d_chunk, P.d_partner, P.d_item_cnt, P.d_overflow, P.d_queue + 8);
   return launch_check("batch sliced probe");
 }
 
@@ -1185,7 +1076,7 @@ int run_batched(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* ou
         bkeys, P.d_b_off, P.d_t_off, P.d_mask, P.d_bitem_join, P.d_bitem_start,
         P.n_bitems_plain, 0, P.d_table, P.d_queue + 0);
   }
-  if (P.any_sliced && build_lds_stage(P, bkeys, -1, st)) return 1;
+  if (P.any_sliced && build_part_stage(P, bkeys, -1, st)) return 1;
   if (P.any_sliced && slice_stage(P, pkeys, P.qall, -1, st)) return 1;
   for (int g = 0; g < 2; ++g) {
     if (P.n_pitems_plain[g] == 0) continue;
@@ -1234,13 +1125,13 @@ int run_batched(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* ou
 int run_per_join(Plan& P, const int32_t* bkeys, const int32_t* pkeys, int64_t* out_probe,
                  int64_t* out_build, int64_t* counts, hipStream_t st) {
   for (int j = 0; j < P.nj; ++j) {
-    // (a sliced join's table is written whole by its LDS build)
+    // (a sliced join has no table in the arena)
     const int64_t fill = P.sbits[j] > 0 ? 0 : P.cap[j];
     bj_fill_tables<<<std::max<unsigned>(1, std::min<unsigned>(grid_for(fill, 256), 2048)), 256,
                      0, st>>>(P.d_table + P.t_off[j], fill, P.d_queue, 16);
     const int64_t nbi = P.bitem_n[j];
     if (P.sbits[j] > 0) {
-      if (build_lds_stage(P, bkeys, j, st)) return 1;
+      if (build_part_stage(P, bkeys, j, st)) return 1;
     } else if (nbi > 0) {
       bj_build<<<(unsigned)std::min<int64_t>(P.persist, nbi), 256, 0, st>>>(
           bkeys, P.d_b_off, P.d_t_off, P.d_mask, P.d_bitem_join, P.d_bitem_start,

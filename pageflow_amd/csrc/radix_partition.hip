@@ -139,11 +139,12 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits) {
   uint32_t lo = (uint32_t)bi ^ inv, hi = (uint32_t)(bi >> 32) ^ inv;
 #pragma unroll
   for (int bit = 0; bit < MAXB; ++bit) {
-    if (bit >= bits) break;
-    const uint32_t on = (uint32_t)((int32_t)(d << (31 - bit)) >> 31);  // bit -> 0 or ~0
-    const uint64_t b = ballot(on != 0);
-    lo |= (uint32_t)b ^ on;
-    hi |= (uint32_t)(b >> 32) ^ on;
+    if (bit < bits) {  // uniform
+      const uint32_t on = (uint32_t)((int32_t)(d << (31 - bit)) >> 31);  // bit -> 0 or ~0
+      const uint64_t b = ballot(on != 0);
+      lo |= (uint32_t)b ^ on;
+      hi |= (uint32_t)(b >> 32) ^ on;
+    }
   }
   return ~(((uint64_t)hi << 32) | lo);
 }

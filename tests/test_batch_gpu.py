@@ -3,7 +3,7 @@ unique-hash pairs of that join, in the batched pipeline, in the per-join
 baseline, and on hipGraph replays (several, with eager work between them);
 every tunable shape gives the same answer; a join probed by slices writes
 its pairs in (slice, probe row) order, with slices held in LDS whole or in
-sub-ranges."""
+sub-ranges, and in several passes when a table's build rows overflow it."""
 from __future__ import annotations
 
 import numpy as np
@@ -129,27 +129,27 @@ def test_sliced_join_order(per_join, slice_kb, sbits):
 
 
 @pytest.mark.parametrize("per_join", [False, True])
-def test_lds_build_clusters_spill_and_wrap(per_join):
-    # one 40K-row join: 2^17 slots, probed in 16 slices, built in 8 slices of
-    # 2^14 slots; 1000 keys home in the last 8 slots of build slice 0 and 1000
-    # in the table's last 8 slots, so their clusters run past the 512-slot
-    # margin, into the next slice and around the table's end
+@pytest.mark.parametrize("slice_kb,n_hot,hot_slots", [(64, 9000, 1 << 13), (512, 15000, 1 << 14)])
+def test_lds_table_passes(per_join, slice_kb, n_hot, hot_slots):
+    # one 40K-row join: 2^17 slots.  n_hot keys home in the table's first
+    # hot_slots slots, which is one LDS table (a 64 KiB slice of 2^13 slots,
+    # or the first 2^14-slot sub-range of a 512 KiB slice): more build rows
+    # than the 7/8 of its slots a pass holds, so it is joined in passes; the
+    # hot keys' clusters also wrap around the table's end
     cap, mask = 1 << 17, (1 << 17) - 1
-    cand = np.arange(1, 1 << 25, dtype=np.int64)
+    cand = np.arange(1, 1 << 22, dtype=np.int64)
     slot = (oracle.mix64(cand) & np.uint64(mask)).astype(np.int64)
-    near_slice = cand[(slot >= (1 << 14) - 8) & (slot < (1 << 14))][:1000]
-    near_end = cand[slot >= cap - 8][:1000]
-    rest = np.setdiff1d(cand[:200_000], np.concatenate([near_slice, near_end]))
-    rest = np.random.default_rng(5).permutation(rest)[:38_000]
-    bk = np.concatenate([near_slice, near_end, rest]).astype(np.int32)
-    assert near_slice.size == 1000 and near_end.size == 1000 and np.unique(bk).size == bk.size
+    hot = cand[slot < hot_slots][:n_hot]
+    rest = np.random.default_rng(5).permutation(cand[slot >= hot_slots][:400_000])[:40_000 - n_hot]
+    bk = np.random.default_rng(7).permutation(np.concatenate([hot, rest])).astype(np.int32)
+    assert hot.size == n_hot and np.unique(bk).size == bk.size == 40_000
     rng = np.random.default_rng(6)
     pk = np.concatenate([bk[rng.integers(0, bk.size, 300_000)], -rng.integers(1, 10**6, 20_000)])
     pk = rng.permutation(pk).astype(np.int32)
     nb, np_ = np.array([bk.size]), np.array([pk.size])
-    tune = [(2, 4096, 256, 0), (2, 4096, 512, 64), (2, 16384, 512, 0)]
+    tune = [(2, 4096, 256, 0), (2, 4096, 512, slice_kb), (2, 16384, 512, 0)]
     bj = BatchJoin(nb, np_, tune)
     bj.run(torch.from_numpy(bk).cuda(), torch.from_numpy(pk).cuda(), per_join=per_join)
     torch.cuda.synchronize()
-    assert bj.info()["sliced_joins"] == 1 and bj.info()["spilled"] > 0
+    assert bj.info()["sliced_joins"] == 1 and bj.info()["lds_overflow"] > 0
     _check_all(bj, nb, np_, torch.from_numpy(bk), torch.from_numpy(pk))
