@@ -96,6 +96,47 @@ def test_empty_request_and_absent_keys():
     assert np.array_equal(r[1].sum, want_s) and np.array_equal(r[1].cnt, want_c)
 
 
+def test_dimension_layouts_staged_l2_and_hashed():
+    # dense dimensions staged in LDS (<= 65536 key values, one at exactly
+    # that many, one with keys from 1000 on), a dense one looked up in L2
+    # (100K key values) and a hashed one (sparse keys)
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev).manual_seed(21)
+    dims = {
+        0: torch.randperm(300, generator=g, device=dev) + 1000,
+        1: torch.randperm(65536, generator=g, device=dev) + 1,
+        2: torch.randperm(100_000, generator=g, device=dev) + 1,
+        3: (torch.randperm(20_000, generator=g, device=dev) + 1) * 7919,
+    }
+    eng = serving.ServingEngine(max_active=6, step_rows=1 << 18, rows_per_wg=1 << 14)
+    G = 25
+    groups = {}
+    for d, k in dims.items():
+        k = k.to(torch.int32)
+        grp = torch.randint(0, G, (k.numel(),), generator=g, device=dev, dtype=torch.int32)
+        dims[d], groups[d] = k, grp
+        eng.register_dim(d, k, grp, G)
+    reqs = []
+    for r in range(12):
+        d = r % 4
+        k = dims[d]
+        nf = 50_000 + 37_000 * r
+        fk = k[torch.randint(0, k.numel(), (nf,), generator=g, device=dev)]
+        fk[::9] = -5  # absent
+        amt = torch.randint(100, 10**7, (nf,), generator=g, device=dev)
+        amt[::13] = -64
+        reqs.append(serving.Request(r, d, fk, amt))
+    res = eng.run(reqs, realtime=False)
+    eng.check_tables()
+    assert [eng.dims[d].table.dense for d in range(4)] == [True, True, True, False]
+    for req, r in zip(reqs, res):
+        dk = dims[req.dim].cpu().numpy().astype(np.int64)
+        b, p = oracle.sort_search_pairs(dk, req.fk.cpu().numpy().astype(np.int64))
+        ws, wc = oracle.group_sum_count(groups[req.dim].cpu().numpy()[b],
+                                        req.amt.cpu().numpy()[p], G)
+        assert np.array_equal(r.sum, ws) and np.array_equal(r.cnt, wc), req.rid
+
+
 def test_step_rejects_bad_items():
     it = (serving.ServeItem * 1)()
     it[0].n = 10
