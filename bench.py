@@ -420,6 +420,108 @@ def bench_serve(args, dev, emit: bool = True):
     return res
 
 
+def bench_uva(args, dev, emit: bool = True):
+    """The out-of-core payload gather of a join's late materialisation: an
+    int64 payload column of args.uva_rows rows in pinned host memory (64 MiB
+    segments), gathered on the GPU over its host link by args.uva_out row
+    ids (uniform, 1% null rows -64, as an outer join emits), and the masked
+    read of a scan over the same column (30% of the rows kept).  The link,
+    not HBM, bounds both: the roofline peak is the measured host-to-device
+    copy rate of a pinned buffer.  Useful bytes: 8 a gathered row read over
+    the link (nulls are not read); the link moves whole 128-byte host lines,
+    so its roofline counts the distinct lines holding a read row."""
+    import numpy as np
+    import torch
+    from pageflow_amd import _lib
+    from pageflow_amd import zero_copy_reader as zc
+
+    n, n_out = args.uva_rows, args.uva_out
+    seg = 1 << 23  # rows a 64 MiB segment
+    log(f"pinning a {n * 8 / 2**30:.1f} GiB host column ({n} int64 rows)")
+    parts = []
+    for a in range(0, n, seg):
+        t = torch.empty(min(seg, n - a), dtype=torch.int64, pin_memory=True)
+        torch.arange(a, a + t.numel(), dtype=torch.int64, out=t)
+        t.mul_(3).add_(7)  # row r holds 3r + 7
+        parts.append(t)
+    ptrs, rows = zc.segments(parts, device=dev)
+    g = torch.Generator(device=dev).manual_seed(6)
+    idx = torch.randint(0, n, (n_out,), generator=g, device=dev)
+    idx[::100] = -64
+    mask = torch.rand(n, generator=g, device=dev) < 0.3
+    kept = int(mask.sum())
+    mres = torch.empty(kept, dtype=torch.int64, device=dev)
+
+    def gather():
+        return zc.zero_copy_rearrange(ptrs, rows, idx, 0, n, torch.int64, 1)
+
+    out = gather()
+    want = torch.where(idx == -64, torch.full_like(idx, -64), idx * 3 + 7)
+    assert torch.equal(out.view(-1), want), "gather disagrees with the column's closed form"
+    zc.zero_copy_mask(ptrs, rows, mask, mres, 0, n)
+    assert torch.equal(mres, torch.nonzero(mask).view(-1) * 3 + 7)
+    del out, want
+    torch.cuda.synchronize()
+    steps = max(1, args.steps)
+    _lib.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gather()
+    torch.cuda.synchronize()
+    t_gather = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        zc.zero_copy_mask(ptrs, rows, mask, mres, 0, n)
+    torch.cuda.synchronize()
+    t_mask = (time.perf_counter() - t0) / steps
+    prof = _lib.profile_read()
+    _lib.profile(False)
+    # the link's practical peak: a pinned host -> device copy of 1 GiB
+    h = parts[0] if parts[0].numel() * 8 >= 1 << 30 else torch.empty(1 << 27, dtype=torch.int64,
+                                                                      pin_memory=True)
+    d = torch.empty(h.numel(), dtype=torch.int64, device=dev)
+    d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d = h.numel() * 8 * 3 / (time.perf_counter() - t0) / 1e9
+    read = 8.0 * (n_out - (n_out + 99) // 100)
+    g_gbps = read / t_gather / 1e9
+    m_gbps = 8.0 * kept / t_mask / 1e9
+    # what the link must move: every 128-byte host line holding a read row
+    line_rows = 16
+    g_lines = int(torch.unique(idx[idx != -64] // line_rows).numel())
+    m_lines = int(torch.unique(torch.nonzero(mask).view(-1) // line_rows).numel())
+    g_link = g_lines * 128.0 / t_gather / 1e9
+    m_link = m_lines * 128.0 / t_mask / 1e9
+    kern = {k: {"total_ms": v[0] / steps, "launches": v[1] / steps} for k, v in prof.items()}
+    res = {"column_rows": n, "gather_rows": n_out, "null_every": 100, "mask_kept": kept,
+           "gather_ms": t_gather * 1e3, "gather_rows_per_s": n_out / t_gather,
+           "gather_link_GBps": g_gbps, "mask_ms": t_mask * 1e3, "mask_link_GBps": m_gbps,
+           "h2d_copy_GBps": h2d, "gather_lines_GBps": g_link, "mask_lines_GBps": m_link,
+           "gather_frac_of_h2d": g_link / h2d, "mask_frac_of_h2d": m_link / h2d,
+           "kernels_ms": kern}
+    log(f"uva: gather {n_out} rows {t_gather * 1e3:.2f} ms ({g_gbps:.1f} GB/s of rows, "
+        f"{g_link:.1f} GB/s of 128-B host lines = {g_link / h2d:.2f} of a {h2d:.1f} GB/s H2D "
+        f"copy); mask {kept} rows {t_mask * 1e3:.2f} ms ({m_gbps:.1f} GB/s of rows, "
+        f"{m_link / h2d:.2f} of the copy)")
+    if emit:
+        line = {"metric": "zero-copy gather rows/s over the host link",
+                "value": n_out / t_gather, "unit": "rows/s", "n_gpus": 1, "steps": steps,
+                "warmup": 1, "ms_per_step": t_gather * 1e3, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic",
+                "config": {"workload": f"gather {n_out} of {n} int64 rows of a pinned host "
+                                       "column (64 MiB segments, 1% null rows)"},
+                "roofline": {"bound": "host link", "kernel": "zc_gather", "achieved": g_link,
+                             "peak": h2d, "unit": "GB/s", "frac": g_link / h2d,
+                             "traffic": g_lines * 128},
+                "uva": res}
+        print(json.dumps(line), flush=True)
+    return res
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -430,8 +532,10 @@ def main() -> None:
     ap.add_argument("--bits", type=int, default=None)
     ap.add_argument("--cpu-sample", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["join", "partition", "batch", "serve", "skew"],
+    ap.add_argument("--workload", choices=["join", "partition", "batch", "serve", "skew", "uva"],
                     default="join")
+    ap.add_argument("--uva-rows", type=int, default=1 << 28)
+    ap.add_argument("--uva-out", type=int, default=1 << 26)
     ap.add_argument("--skew-build", type=int, default=10_000_000)
     ap.add_argument("--skew-distinct", type=int, default=2_500_000)
     ap.add_argument("--skew-probe", type=int, default=400_000_000)
@@ -469,6 +573,9 @@ def main() -> None:
         return
     if args.workload == "skew":
         bench_skew(args, dev)
+        return
+    if args.workload == "uva":
+        bench_uva(args, dev)
         return
     nb, np_ = args.build_rows, args.probe_rows
     log(f"generating {nb} build / {np_} probe rows")
@@ -554,6 +661,8 @@ def main() -> None:
         tiers["batch"] = bench_batch(args, dev, emit=False)
         torch.cuda.empty_cache()
         tiers["serving"] = bench_serve(args, dev, emit=False)
+        torch.cuda.empty_cache()
+        tiers["uva"] = bench_uva(args, dev, emit=False)
 
     line = {
         "metric": "join tuples/s + % HBM roofline, batched vs per-join launch; p99 ms (serving)",

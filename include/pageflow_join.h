@@ -266,6 +266,46 @@ typedef struct pf_serve_item {
 int pf_serve_step(const pf_serve_item* items, int32_t n_items, int32_t rows_per_wg,
                   void* stream);
 
+/* ------------------------------------------------- zero-copy reads --
+ * Replace the reference's zero-copy reader extension
+ * (IO/zero_copy_reader.cu:224-318, pybind module TQPlib.zero_copy_reader):
+ * reads of a column held in pinned host memory, by the GPU over its host
+ * link.  The column is num_segs segments: seg_ptrs[s] (device int64) the
+ * device address of segment s's first row, seg_rows[s] (device int64,
+ * num_segs + 1 entries, ascending) its first row in the column's row space,
+ * seg_rows[num_segs] the end.  Row i of a read is row row_start + i.  Rows
+ * are width elements of elem_bytes (1, 2, 4, 8; is_float: 4 or 8).  A row
+ * outside every segment is not read: its output is -64 and it is counted
+ * into *bad (device int64, added to); the reference traps instead.  No
+ * wait.
+ *
+ * pf_device_alias replaces device_alias (zero_copy_reader.cu:224-233): the
+ * device address of a pinned host pointer. */
+int pf_device_alias(const void* host, void** dev);
+
+/* Replaces zero_copy_rearrange (zero_copy_reader.cu:267-292, kernel
+ * :87-120): out[r] (n rows) = row row_start + rearrange[r], or -64 in every
+ * element when rearrange[r] is -64 (the null row).  n < 2^31.  The indices
+ * are radix-sorted on the GPU first (from 2^16 of them), so host rows are
+ * read in ascending order. */
+size_t pf_zero_copy_rearrange_workspace_bytes(int64_t n);
+int pf_zero_copy_rearrange(const int64_t* seg_ptrs, const int64_t* seg_rows, int32_t num_segs,
+                           int64_t row_start, int64_t row_stop, const int64_t* rearrange,
+                           int64_t n, int32_t elem_bytes, int32_t is_float, int64_t width,
+                           void* out, int64_t* bad, void* workspace, size_t workspace_bytes,
+                           void* stream);
+
+/* Replaces zero_copy_mask (zero_copy_reader.cu:235-265, kernel :54-80):
+ * out[k] = row row_start + i for the k-th i < row_stop - row_start with
+ * mask[i] != 0 (uint8, one a row); at most n_out rows are written and the
+ * kept count is stored into *kept (device int64). */
+size_t pf_zero_copy_mask_workspace_bytes(int64_t n);
+int pf_zero_copy_mask(const int64_t* seg_ptrs, const int64_t* seg_rows, int32_t num_segs,
+                      int64_t row_start, int64_t row_stop, const uint8_t* mask, int32_t elem_bytes,
+                      int32_t is_float, int64_t width, void* out, int64_t n_out,
+                      int64_t* kept, int64_t* bad, void* workspace, size_t workspace_bytes,
+                      void* stream);
+
 /* ---------------------------------------------------------- batching --
  * No counterpart in the reference (the batched tier).  A plan for njoins
  * independent inner joins on int32 keys (unique builds): nb[j] build and

@@ -29,6 +29,10 @@ Parity pinning (what checks each function besides the formula itself):
 * ``sort_search_pairs``: the dense build x probe comparison
   (``test_join_core.py:263-281``).
 * ``group_sum_count``: closed form (per-group sums by brute force).
+* ``zero_copy_rearrange`` / ``zero_copy_mask``: the reference test's closed
+  forms (``tests/IO/test_scan_regions.py:187-218``: ``x[lo:hi][idx]`` with
+  null rows -64, ``x[lo:hi][mask]``) over the same column cut into
+  segments of random sizes.
 """
 from __future__ import annotations
 
@@ -283,6 +287,54 @@ def gather_with_null(src, idx) -> np.ndarray:
     out = np.empty(max(i.shape[0], 1), np.int64)
     _lib().o_gather_i64(_p(s), s.shape[0], _p(i), i.shape[0], _p(out))
     return out[:i.shape[0]].copy()
+
+
+# ------------------------------------------------------- zero-copy reads --
+
+def _segment_row(parts: Sequence[np.ndarray], first_row: int, row: int):
+    """The segment holding row-space row ``row`` and its row there, or None
+    outside every segment (zero_copy_reader.cu:41-52: the last segment
+    whose first row is <= row)."""
+    start = first_row
+    for p in parts:
+        if start <= row < start + p.shape[0]:
+            return p, row - start
+        start += p.shape[0]
+    return None
+
+
+def zero_copy_rearrange(parts: Sequence[np.ndarray], first_row: int, rearrange,
+                        row_start: int) -> np.ndarray:
+    """zero_copy_reader.cu:87-120: row r of the result is row
+    ``row_start + rearrange[r]`` of a column held as segments ``parts``
+    (2-D numpy arrays, back to back from row ``first_row``), or -64 in every
+    element where ``rearrange[r]`` is -64.  A row outside every segment
+    raises (the reference traps).  Order of host reads is not observable."""
+    idx = np.asarray(rearrange, dtype=np.int64)
+    width, dtype = parts[0].shape[1], parts[0].dtype
+    out = np.empty((idx.shape[0], width), dtype=dtype)
+    # segments back to back: row-space row first_row + k is row k of their
+    # concatenation, each segment holding [start, start + len)
+    column = np.concatenate(parts)
+    rows = row_start + idx
+    valid = idx != NULL
+    outside = valid & ((rows < first_row) | (rows >= first_row + column.shape[0]))
+    if outside.any():
+        r = int(rows[np.nonzero(outside)[0][0]])
+        assert _segment_row(parts, first_row, r) is None
+        raise IndexError(f"row {r} lies outside every segment")
+    out[valid] = column[rows[valid] - first_row]
+    # -64 converted to the column's type as C converts it (unsigned: wraps)
+    out[~valid] = np.array(NULL, dtype=np.int64).astype(dtype)
+    return out
+
+
+def zero_copy_mask(parts: Sequence[np.ndarray], first_row: int, mask,
+                   row_start: int) -> np.ndarray:
+    """zero_copy_reader.cu:54-80: the rows ``row_start + i`` for each i
+    with ``mask[i]`` true, in order."""
+    keep = np.nonzero(np.asarray(mask, dtype=bool))[0].astype(np.int64)
+    return zero_copy_rearrange(parts, first_row, keep, row_start)
 
 
 # ------------------------------------------------------------- group-by --

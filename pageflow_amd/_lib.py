@@ -56,6 +56,13 @@ _SIGNATURES = {
     "pf_dim_table_build": (ctypes.c_int, [P, P, I64, I32, P, I64, P, P]),
     "pf_fact_join_groupby": (ctypes.c_int, [P, P, I64, P, I64, P, I32, P, P, P]),
     "pf_serve_step": (ctypes.c_int, [P, I32, I32, P]),
+    "pf_device_alias": (ctypes.c_int, [P, ctypes.POINTER(P)]),
+    "pf_zero_copy_rearrange_workspace_bytes": (SZ, [I64]),
+    "pf_zero_copy_rearrange": (ctypes.c_int, [P, P, I32, I64, I64, P, I64, I32, I32, I64, P, P,
+                                              P, SZ, P]),
+    "pf_zero_copy_mask_workspace_bytes": (SZ, [I64]),
+    "pf_zero_copy_mask": (ctypes.c_int, [P, P, I32, I64, I64, P, I32, I32, I64, P, I64, P, P,
+                                         P, SZ, P]),
     "pf_batch_plan_create": (P, [I32, P, P, P]),
     "pf_batch_plan_destroy": (ctypes.c_int, [P]),
     "pf_batch_plan_info": (I64, [P, I32]),

@@ -250,3 +250,38 @@ def test_geometric_duplicates_reach_the_exact_row_count():
         assert c.sum() == nb and c.min() >= 1 and c.size == nd
         again = geometric_duplicates(nd, nb, 0.25, np.random.default_rng(seed))
         assert np.array_equal(c, again)
+
+
+def _cut(x, sizes):
+    out, a = [], 0
+    for s in sizes:
+        out.append(x[a:a + s])
+        a += s
+    return out
+
+
+def test_zero_copy_rearrange_matches_the_reference_closed_form():
+    # tests/IO/test_scan_regions.py:187-201: indices count from the slice's
+    # first row; a null index stays null (want = x[lo:hi][idx], nulls -64)
+    rng = np.random.default_rng(5)
+    x = rng.integers(-10**12, 10**12, size=(3000, 2))
+    parts = _cut(x, [700, 1, 1299, 1000])
+    lo, hi = 701, 2307
+    idx = rng.integers(0, hi - lo, 500)
+    idx[::97] = oracle.NULL
+    want = x[lo:hi][np.clip(idx, 0, None)]
+    want[idx == oracle.NULL] = oracle.NULL
+    assert np.array_equal(oracle.zero_copy_rearrange(parts, 0, idx, lo), want)
+    # a row space that starts past 0, and a row outside every segment
+    assert np.array_equal(oracle.zero_copy_rearrange(parts, 50, idx, lo + 50), want)
+    with pytest.raises(IndexError):
+        oracle.zero_copy_rearrange(parts, 0, np.array([3000 - lo]), lo)
+
+
+def test_zero_copy_mask_matches_the_reference_closed_form():
+    # tests/IO/test_scan_regions.py:204-218: the kept rows of a slice, in order
+    x = np.arange(3000, dtype=np.int32).reshape(-1, 1) * 3
+    parts = _cut(x, [1000, 999, 1001])
+    lo, hi = 123, 2911
+    mask = (np.arange(3000) % 7 == 3)[lo:hi]
+    assert np.array_equal(oracle.zero_copy_mask(parts, 0, mask, lo), x[lo:hi][mask])
