@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -124,6 +125,8 @@ def bench_partition(args, dev) -> None:
         log(f"bits={bits} shape={shape}: {ms:.3f} ms ({sweep[key]['GBps']:.0f} GB/s), "
             f"scatter {sc:.3f} ms, hist {hi:.3f} ms")
     best = min(sweep, key=lambda b: sweep[b]["ms"])
+    del kout, vout
+    agg = bench_agg_pass1(args, dev, probe)
     line = {
         "metric": "radix partition rows/s (640M int64 keys + int32 row ids)",
         "value": sweep[best]["rows_per_s"], "unit": "rows/s", "n_gpus": 1,
@@ -137,8 +140,90 @@ def bench_partition(args, dev) -> None:
                      "unit": "GB/s", "frac": sweep[best]["scatter_GBps"] / HBM_PEAK_GBPS,
                      "traffic": None},
         "sweep": sweep,
+        "agg_pass1": agg,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_agg_pass1(args, dev, h_all):
+    """Pass 1 of the partitioned aggregation on one tick: the first 128M of
+    the large join's probe keys as the grouping hash, with an int64 and an
+    int32 value, put into the padded layout of B = 256 buckets
+    (_bucket_ids, then _partition_outputs: three outputs, one kernel call).
+    Algorithmic bytes: the hash read for the bucket ids and the ids written
+    (8 + 4), then the ids and each output read once and written once
+    (4 + 2 (8 + 8 + 4)): 56 B a row."""
+    import torch
+    from pageflow_amd import _lib
+    from pageflow_amd.operators import aggregate as agg
+
+    n, B = min(h_all.numel(), 128_000_000), 256
+    h = h_all[:n]
+    g = torch.Generator(device=dev).manual_seed(9)
+    v64 = torch.randint(0, 1 << 40, (n,), generator=g, device=dev)
+    v32 = torch.randint(0, 1 << 30, (n,), generator=g, device=dev, dtype=torch.int32)
+    bucket = agg._bucket_ids(h, B)
+    counts_t = torch.bincount(bucket.long(), minlength=B)
+    counts = counts_t.tolist()
+
+    def run():
+        b = agg._bucket_ids(h, B)
+        return agg._partition_outputs([h, v64, v32], b, counts, counts_t)
+
+    outs = run()
+    # each bucket's hashes all map to it, and the rows keep their order
+    starts = outs[0].bucket_starts()
+    for k in (0, 77, 255):
+        part = outs[0].tensor[starts[k]:starts[k] + counts[k]]
+        assert bool((agg._bucket_ids(part, B) == k).all())
+    del outs
+    torch.cuda.synchronize()
+    _lib.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    prof = _lib.profile_read()
+    _lib.profile(False)
+    sc = prof["ps_scatter"][0] / prof["ps_scatter"][1]
+    gbps = 56.0 * n / (ms * 1e-3) / 1e9
+
+    # the reference's own steps in torch on the same GPU (aggregate.py:120-152):
+    # a stable sort of the bucket ids, each row's position, the padding of
+    # the buckets before it, one index_copy_ an output
+    def torch_steps():
+        b = agg._bucket_ids(h, B)
+        order = torch.sort(b, stable=True).indices
+        pos = torch.empty_like(order)
+        pos[order] = torch.arange(n, dtype=order.dtype, device=dev)
+        res = []
+        for t in (h, v64, v32):
+            unit = 512 // math.gcd(t.element_size(), 512)
+            pad = (counts_t + unit - 1) // unit * unit - counts_t
+            shift = torch.cumsum(pad, 0) - pad
+            dest = pos + shift.index_select(0, b.long())
+            out = torch.empty(int((counts_t + pad).sum()), dtype=t.dtype, device=dev)
+            out.index_copy_(0, dest, t)
+            res.append(out)
+        return res
+
+    torch_steps()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        torch_steps()
+    torch.cuda.synchronize()
+    torch_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    sc_gbps = 44.0 * n / (sc * 1e-3) / 1e9  # the scatter: ids + 3 outputs in, 3 outputs out
+    log(f"aggregation pass 1: {n} rows into {B} buckets {ms:.2f} ms ({gbps:.0f} GB/s), "
+        f"ps_scatter {sc:.2f} ms ({sc_gbps:.0f} GB/s); the reference's steps in torch "
+        f"{torch_ms:.2f} ms")
+    return {"rows": n, "B": B, "ms": ms, "rows_per_s": n / (ms * 1e-3), "GBps": gbps,
+            "torch_reference_steps_ms": torch_ms, "speedup_vs_torch_steps": torch_ms / ms,
+            "frac": gbps / HBM_PEAK_GBPS, "ps_scatter_ms": sc, "ps_scatter_GBps": sc_gbps,
+            "ps_scatter_frac": sc_gbps / HBM_PEAK_GBPS,
+            "kernels_ms": {k: v[0] / args.steps for k, v in prof.items()}}
 
 
 def bench_batch(args, dev, emit: bool = True):

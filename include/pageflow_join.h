@@ -84,6 +84,22 @@ int pf_partition_sort(const void* const* cols_in, void* const* cols_out,
                       int64_t* host_out, void* workspace, size_t workspace_bytes,
                       void* stream);
 
+/* The partition of a partitioned aggregation's pass-1 tick, with the
+ * buckets given.  Replaces operators/aggregate.py:120-152
+ * (_partition_outputs; its buckets come from _bucket_ids, pf_bucket_ids
+ * above).  cols_in[c]: n rows of row_bytes[c] bytes (ncols <= 6, at most 4
+ * row widths); bucket: int32[n], each row's bucket in [0, B).  cols_out[c]
+ * is sized n + B*(u-1) rows as above; rows keep their order inside a bucket
+ * and each non-empty bucket starts on a whole lcm(row bytes, 512)-byte unit,
+ * padding unwritten.  host_out: pinned int64[B + 6]: the B bucket counts,
+ * each row-width class's padded total (classes by first appearance), -1,
+ * and the rows whose bucket was out of range (dropped).  Waits once. */
+size_t pf_partition_by_bucket_workspace_bytes(int64_t n, int32_t B);
+int pf_partition_by_bucket(const void* const* cols_in, void* const* cols_out,
+                           const int32_t* row_bytes, int32_t ncols, const int32_t* bucket,
+                           int64_t n, int32_t B, int64_t* host_out, void* workspace,
+                           size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------- unique hash --
  * Replaces tqpkernels.join_unique_hash (operators/kernels/join_core.cu:301-351;
  * Python caller operators/join_core.py:110-111).

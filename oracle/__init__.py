@@ -29,6 +29,8 @@ Parity pinning (what checks each function besides the formula itself):
 * ``sort_search_pairs``: the dense build x probe comparison
   (``test_join_core.py:263-281``).
 * ``group_sum_count``: closed form (per-group sums by brute force).
+* ``partition_outputs``: each bucket's rows in source order (brute force per
+  bucket) at the starts ``padded_layout`` gives, padding unused.
 * ``zero_copy_rearrange`` / ``zero_copy_mask``: the reference test's closed
   forms (``tests/IO/test_scan_regions.py:187-218``: ``x[lo:hi][idx]`` with
   null rows -64, ``x[lo:hi][mask]``) over the same column cut into
@@ -148,6 +150,34 @@ def partition(cols: Sequence[np.ndarray], key_pos: int, mask: Optional[np.ndarra
     splits = np.cumsum(counts)[:-1]
     outs = [list(np.split(c[order], splits)) for c in list(cols) + [gidx]]
     return counts.tolist(), outs
+
+
+def partition_outputs(outs: Sequence[np.ndarray], bucket, B: int
+                      ) -> Tuple[List[int], List[np.ndarray]]:
+    """A partitioned aggregation's pass-1 tick in the padded layout with the
+    buckets given (operators/aggregate.py:120-152): the bucket counts and,
+    per output, the padded array (padding rows 0; the reference leaves them
+    unwritten).  Restates the reference's steps: a stable sort of the
+    buckets, each row's sorted position, and the padding of the buckets
+    before its own added to it."""
+    b = np.asarray(bucket, dtype=np.int64)
+    n = b.shape[0]
+    counts = np.bincount(b, minlength=B).astype(np.int64)
+    order = np.argsort(b, kind="stable")
+    pos = np.empty(n, np.int64)
+    pos[order] = np.arange(n)
+    padded = []
+    for t in outs:
+        rb = t.dtype.itemsize * int(np.prod(t.shape[1:], dtype=np.int64))
+        unit = rows_per_unit(rb)
+        pad = -(-counts // unit) * unit - counts
+        shift = np.cumsum(pad) - pad
+        dest = pos + shift[b]
+        total = int(counts.sum() + pad.sum())
+        out = np.zeros((total,) + t.shape[1:], dtype=t.dtype)
+        out[dest] = t
+        padded.append(out)
+    return counts.tolist(), padded
 
 
 def padded_layout(counts: Sequence[int], row_bytes: int) -> Tuple[List[int], int]:

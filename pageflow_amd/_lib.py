@@ -34,6 +34,8 @@ _SIGNATURES = {
     "pf_partition_sort_workspace_bytes": (SZ, [I64, I64, I32, I32, I32, I32]),
     "pf_partition_sort": (ctypes.c_int, [PP, PP, P, P, I32, I32, P, I64, I64, I64, I32, I32,
                                          P, P, P, SZ, P]),
+    "pf_partition_by_bucket_workspace_bytes": (SZ, [I64, I32]),
+    "pf_partition_by_bucket": (ctypes.c_int, [PP, PP, P, I32, P, I64, I32, P, P, SZ, P]),
     "pf_join_unique_hash_workspace_bytes": (SZ, [I64, I64, I32]),
     "pf_join_unique_hash": (ctypes.c_int, [P, P, I64, P, P, I64, I32, P, P, P, P, SZ, P]),
     "pf_join_semi_anti_workspace_bytes": (SZ, [I64, I64, I32]),

@@ -60,7 +60,8 @@ struct Cols {
 
 struct Params {
   Cols c;
-  int64_t* gout;
+  int64_t* gout;          // null: no gidx output
+  const int32_t* bucket;  // the rows' buckets given (pf_partition_by_bucket), or null
   int gcls;
   int64_t goff;
   const int64_t* key;
@@ -74,7 +75,8 @@ struct Params {
   int64_t nblk;
   int32_t* hist;          // [B][nblk], scanned in place
   const int64_t* bases;   // [MAXCLS][B]
-  int32_t* flags;         // [1] prologue count != n_valid, [2] counts > n_valid, [3] prologue total
+  int32_t* flags;         // [1] prologue count != n_valid, [2] counts > n_valid, [3] prologue
+                          // total, [4] given buckets out of range
   int check;              // bit 0: flags[1] applies, bit 1: flags[2] applies
 };
 
@@ -281,8 +283,17 @@ __global__ __launch_bounds__(NT) void ps_hist(Params p) {
   for (int k = 0; k < IT; ++k) {
     if (valid[k]) {
       const int64_t i = seg + k * kWave + lane;
-      const int64_t key = p.key[col_index<SRC>(p, p.key_comp, i, jl[k], coff)];
-      atomicAdd(&h[bucket_of(key, p.logB)], 1);
+      int b;
+      if (p.bucket) {  // given: no mask, so row i itself
+        b = p.bucket[i];
+        if ((unsigned)b >= (unsigned)p.B) {  // dropped, and reported
+          atomicAdd(&p.flags[4], 1);
+          continue;
+        }
+      } else {
+        b = bucket_of(p.key[col_index<SRC>(p, p.key_comp, i, jl[k], coff)], p.logB);
+      }
+      atomicAdd(&h[b], 1);
     }
   }
   __syncthreads();
@@ -330,7 +341,7 @@ struct Rpa {
 __global__ __launch_bounds__(NT) void ps_finish(const int64_t* __restrict__ counts, int B,
                                                 Rpa rpa, int ncls, int64_t* __restrict__ bases,
                                                 int64_t* host_out, int32_t* flags,
-                                                int64_t n_valid, int prologue) {
+                                                int64_t n_valid, int prologue, int given) {
   __shared__ int64_t tmp[NW + 1];
   const int tid = threadIdx.x;
   // B <= 512 = NT: one bucket a thread
@@ -349,6 +360,7 @@ __global__ __launch_bounds__(NT) void ps_finish(const int64_t* __restrict__ coun
   if (tid == 0) {
     if (total > n_valid) flags[2] = 1;
     store_host(&host_out[B + MAXCLS], prologue ? (int64_t)flags[3] : -1);
+    if (given) store_host(&host_out[B + MAXCLS + 1], (int64_t)flags[4]);
   }
 }
 
@@ -401,8 +413,14 @@ __global__ __launch_bounds__(NT) void ps_scatter(Params p) {
 #pragma unroll
   for (int k = 0; k < IT; ++k) {
     const int64_t i = seg + k * kWave + lane;
-    bk[k] = valid[k] ? bucket_of(p.key[col_index<SRC>(p, p.key_comp, i, jl[k], coff)], p.logB)
-                     : -1;
+    if (!valid[k]) {
+      bk[k] = -1;
+    } else if (p.bucket) {
+      const int b = p.bucket[i];
+      bk[k] = (unsigned)b < (unsigned)B ? b : -1;
+    } else {
+      bk[k] = bucket_of(p.key[col_index<SRC>(p, p.key_comp, i, jl[k], coff)], p.logB);
+    }
   }
   // stable rank inside (wave, bucket): rounds in row order, lanes in order
   int rk[IT];
@@ -510,10 +528,12 @@ __global__ __launch_bounds__(NT) void ps_scatter(Params p) {
       }
     }
   }
-  const int64_t* gbg = gb + p.gcls * B;
-  for (int s = tid; s < tvalid; s += NT) {
-    const int64_t i = tile_base + sidx[s];
-    p.gout[gbg[sb[s]] + s] = p.goff + (SRC ? (int64_t)p.src[i] : i);
+  if (p.gout) {
+    const int64_t* gbg = gb + p.gcls * B;
+    for (int s = tid; s < tvalid; s += NT) {
+      const int64_t i = tile_base + sidx[s];
+      p.gout[gbg[sb[s]] + s] = p.goff + (SRC ? (int64_t)p.src[i] : i);
+    }
   }
 }
 
@@ -533,7 +553,7 @@ Layout carve_layout(void* ws, int64_t ndom, int64_t n_valid, int B, bool use_sel
                     bool needs_j) {
   Carve cv(ws);
   Layout l;
-  l.flags = cv.take<int32_t>(4);
+  l.flags = cv.take<int32_t>(8);
   l.counts = cv.take<int64_t>(B);
   l.bases = cv.take<int64_t>((size_t)MAXCLS * B);
   const int64_t dom = use_sel ? n_valid : ndom;
@@ -560,22 +580,29 @@ size_t pf_partition_sort_workspace_bytes(int64_t ndom, int64_t n_valid, int32_t 
   return pf::carve_layout(nullptr, ndom, n_valid, B, use_sel, needs_j).bytes;
 }
 
-int pf_partition_sort(const void* const* cols_in, void* const* cols_out,
-                      const int32_t* row_bytes, const int32_t* comp, int32_t ncols,
-                      int32_t key_idx, const uint8_t* mask, int64_t ndom, int64_t n_valid,
-                      int64_t goff, int32_t B, int32_t select_first, int64_t* gidx_out,
-                      int64_t* host_out, void* workspace, size_t workspace_bytes,
-                      void* stream) {
-  using namespace pf;
-  hipStream_t st = (hipStream_t)stream;
+}  // extern "C"
+
+namespace pf {
+namespace {
+
+// The partition of both entry points: buckets from the key column's mix64,
+// or (given) from the bucket column, with no key, mask or gidx.
+int partition_impl(const void* const* cols_in, void* const* cols_out, const int32_t* row_bytes,
+                   const int32_t* comp, int32_t ncols, int32_t key_idx, bool given,
+                   const int32_t* bucket, const uint8_t* mask, int64_t ndom, int64_t n_valid, int64_t goff, int32_t B,
+                   int32_t select_first, int64_t* gidx_out, int64_t* host_out, void* workspace,
+                   size_t workspace_bytes, hipStream_t st) {
   if (ncols < 1 || ncols > MAXC)
     return fail("partition_sort: 1-" + std::to_string(MAXC) + " columns, got " +
                 std::to_string(ncols));
   if (B < 1 || B > BMAX || (B & (B - 1)))
     return fail("partition_sort: B must be a power of two in [1, " + std::to_string(BMAX) +
                 "], got " + std::to_string(B));
-  if (key_idx < 0 || key_idx >= ncols) return fail("partition_sort: key index out of range");
-  if (row_bytes[key_idx] != 8) return fail("partition_sort: the key must be 8-byte int64");
+  if (!given) {
+    if (key_idx < 0 || key_idx >= ncols)
+      return fail("partition_sort: key index out of range");
+    if (row_bytes[key_idx] != 8) return fail("partition_sort: the key must be 8-byte int64");
+  }
   if (ndom < 0 || ndom >= (1LL << 31))
     return fail("partition_sort: a tick must hold fewer than 2^31 rows");
   if (n_valid < 0 || n_valid > ndom)
@@ -608,7 +635,7 @@ int pf_partition_sort(const void* const* cols_in, void* const* cols_out,
     any_comp |= comp[c] != 0;
   }
   p.c.ncols = ncols;
-  p.gcls = cls_of(8);
+  if (gidx_out) p.gcls = cls_of(8);
   if (bad_cls)
     return fail("partition_sort: more than " + std::to_string(MAXCLS) + " row widths");
   const bool use_sel = select_first && has_mask;
@@ -621,9 +648,10 @@ int pf_partition_sort(const void* const* cols_in, void* const* cols_out,
   if (host_alias(host_out, &hdev)) return 1;
 
   p.gout = gidx_out;
+  p.bucket = bucket;
   p.goff = goff;
-  p.key = (const int64_t*)cols_in[key_idx];
-  p.key_comp = comp[key_idx] ? 1 : 0;
+  p.key = given ? nullptr : (const int64_t*)cols_in[key_idx];
+  p.key_comp = !given && comp[key_idx] ? 1 : 0;
   p.mask = mask;
   p.ndom = ndom;
   p.B = B;
@@ -633,7 +661,7 @@ int pf_partition_sort(const void* const* cols_in, void* const* cols_out,
   p.logB = logB;
   p.flags = l.flags;
   p.check = ((use_sel || p.needs_j) ? 1 : 0) | (has_mask ? 2 : 0);
-  if (fill_i32(l.flags, 4, 0, st)) return 1;
+  if (fill_i32(l.flags, 8, 0, st)) return 1;
 
   if (use_sel) {
     const int64_t mblk = ndom > 0 ? (ndom + MS_ROWS - 1) / MS_ROWS : 1;
@@ -667,7 +695,7 @@ int pf_partition_sort(const void* const* cols_in, void* const* cols_out,
   for (int c = 0; c < MAXCLS; ++c) rpa.v[c] = 1;
   for (int c = 0; c < ncls; ++c) rpa.v[c] = rows_per_unit(rb_cls[c]);
   ps_finish<<<1, NT, 0, st>>>(l.counts, B, rpa, ncls, l.bases, (int64_t*)hdev, l.flags,
-                              n_valid, (use_sel || p.needs_j) ? 1 : 0);
+                              n_valid, (use_sel || p.needs_j) ? 1 : 0, given ? 1 : 0);
   if (launch_check("partition_sort")) return 1;
   hipEvent_t ev = stream_event(st);
   if (!ev) return fail("partition_sort: cannot create an event");
@@ -683,6 +711,42 @@ int pf_partition_sort(const void* const* cols_in, void* const* cols_out,
   // the call's one host wait: the counts, while the scatter runs
   if (hip_check(hipEventSynchronize(ev), "partition_sort")) return 1;
   return ok();
+}
+
+}  // namespace
+}  // namespace pf
+
+extern "C" {
+
+int pf_partition_sort(const void* const* cols_in, void* const* cols_out,
+                      const int32_t* row_bytes, const int32_t* comp, int32_t ncols,
+                      int32_t key_idx, const uint8_t* mask, int64_t ndom, int64_t n_valid,
+                      int64_t goff, int32_t B, int32_t select_first, int64_t* gidx_out,
+                      int64_t* host_out, void* workspace, size_t workspace_bytes,
+                      void* stream) {
+  using namespace pf;
+  if (!gidx_out) return fail("partition_sort: no gidx output");
+  return partition_impl(cols_in, cols_out, row_bytes, comp, ncols, key_idx, false, nullptr,
+                        mask, ndom, n_valid, goff, B, select_first, gidx_out, host_out,
+                        workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+size_t pf_partition_by_bucket_workspace_bytes(int64_t n, int32_t B) {
+  if (B < 1) B = 1;
+  if (B > pf::BMAX) B = pf::BMAX;
+  return pf::carve_layout(nullptr, n, n, B, false, false).bytes;
+}
+
+int pf_partition_by_bucket(const void* const* cols_in, void* const* cols_out,
+                           const int32_t* row_bytes, int32_t ncols, const int32_t* bucket,
+                           int64_t n, int32_t B, int64_t* host_out, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+  using namespace pf;
+  if (!bucket && n > 0) return fail("partition_by_bucket: no bucket column");
+  const int32_t comp[MAXC] = {0, 0, 0, 0, 0, 0};
+  return partition_impl(cols_in, cols_out, row_bytes, comp, ncols, -1, true, bucket, nullptr, n,
+                        n, 0, B, 0, nullptr, host_out, workspace, workspace_bytes,
+                        (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -285,3 +285,23 @@ def test_zero_copy_mask_matches_the_reference_closed_form():
     lo, hi = 123, 2911
     mask = (np.arange(3000) % 7 == 3)[lo:hi]
     assert np.array_equal(oracle.zero_copy_mask(parts, 0, mask, lo), x[lo:hi][mask])
+
+
+def test_partition_outputs_is_the_stable_padded_layout():
+    # operators/aggregate.py:120-152: bucket b's rows in source order from a
+    # whole lcm(row bytes, 512)-byte unit; checked per bucket by brute force
+    rng = np.random.default_rng(3)
+    n, B = 5000, 16
+    b = rng.integers(0, B, n)
+    b[:700] = 5  # a large bucket
+    outs = [rng.integers(-10**9, 10**9, n), rng.integers(0, 255, (n, 3)).astype(np.uint8),
+            rng.standard_normal(n).astype(np.float32)]
+    counts, padded = oracle.partition_outputs(outs, b, B)
+    assert counts == np.bincount(b, minlength=B).tolist()
+    for t, p in zip(outs, padded):
+        rb = t.dtype.itemsize * int(np.prod(t.shape[1:]))
+        starts, total = oracle.padded_layout(counts, rb)
+        assert p.shape[0] == total
+        for k in range(B):
+            want = t[[i for i in range(n) if b[i] == k]]
+            assert np.array_equal(p[starts[k]:starts[k] + counts[k]], want), (rb, k)
