@@ -411,6 +411,7 @@ def bench_skew(args, dev, emit: bool = True):
            "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in prof.items()}}
     log(f"skew: inner {t_inner * 1e3:.2f} ms ({pairs} pairs), semi {t_semi * 1e3:.2f} ms, "
         f"anti {t_anti * 1e3:.2f} ms")
+    res["residual"] = bench_residual(args, dev, ob, op, nb, np_, timed)
     if emit:
         line = {"metric": "skewed multi-match join tuples/s", "value": (nb + np_) / t_inner,
                 "unit": "tuples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -426,6 +427,58 @@ def bench_skew(args, dev, emit: bool = True):
         print(json.dumps(line), flush=True)
     del out_p, out_b, plan
     return res
+
+
+def bench_residual(args, dev, ob, op, nb: int, np_: int, timed):
+    """A residual condition on the skewed join's pairs: build payload <
+    probe payload (int64, values in [0, 1000)), evaluated on every pair and
+    the kept pairs compacted in pair order.  The fused filter (one
+    evaluation pass, a scan, one compaction pass) against the reference's
+    torch steps on the same GPU (two gathers, lt, the mask AND, nonzero, two
+    index selects: operators/hashjoin.py:428-439).  Algorithmic bytes: the
+    evaluation 32 B a pair (two indexes, two operands) + a mask bit, the
+    compaction a mask bit + 32 B a kept pair (two indexes read, two
+    written)."""
+    import torch
+    from pageflow_amd import _lib
+    from pageflow_amd.operators import residual
+
+    g = torch.Generator(device=dev).manual_seed(11)
+    bpay = torch.randint(0, 1000, (nb,), generator=g, device=dev)
+    ppay = torch.randint(0, 1000, (np_,), generator=g, device=dev)
+    n = ob.numel()
+    prog = residual.compile_conditions({11: bpay}, {12: ppay}, [['b_pay#11', '<', 'p_pay#12']])
+    _lib.profile(True)
+    t_fused, (kb, kp) = timed(lambda: residual.filter_pairs(prog, ob, op), args.steps)
+    prof = _lib.profile_read()
+    _lib.profile(False)
+    kept = kb.numel()
+
+    def torch_steps():
+        mask = torch.ones(n, dtype=torch.bool, device=dev)
+        mask &= torch.lt(bpay[ob], ppay[op])
+        pos = torch.nonzero(mask, as_tuple=False).view(-1)
+        return ob[pos], op[pos]
+
+    t_torch, (tb, tp) = timed(torch_steps, max(1, args.steps // 2))
+    assert torch.equal(kb, tb) and torch.equal(kp, tp), "fused residual differs from torch"
+    del tb, tp
+    log(f"residual: {n} pairs -> {kept}, fused {t_fused * 1e3:.2f} ms, "
+        f"torch steps {t_torch * 1e3:.2f} ms")
+    kms = {k: v[0] / max(v[1], 1) for k, v in prof.items()}
+    b_eval = 32.0 * n + n / 8.0
+    b_comp = n / 8.0 + 32.0 * kept
+    out = {"pairs": n, "kept": kept, "fused_ms": t_fused * 1e3, "torch_steps_ms": t_torch * 1e3,
+           "speedup": t_torch / t_fused, "pairs_per_s": n / t_fused,
+           "fused_GBps": (b_eval + b_comp) / t_fused / 1e9, "kernels_ms": kms}
+    if "rf_eval" in kms:
+        out["rf_eval_GBps"] = b_eval / (kms["rf_eval"] * 1e-3) / 1e9
+        out["rf_eval_frac"] = out["rf_eval_GBps"] / HBM_PEAK_GBPS
+    if "rf_compact" in kms:
+        out["rf_compact_GBps"] = b_comp / (kms["rf_compact"] * 1e-3) / 1e9
+    del kb, kp, bpay, ppay
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_serve(args, dev, emit: bool = True):

@@ -226,6 +226,72 @@ size_t pf_select_rows_workspace_bytes(int64_t n);
 int pf_select_rows(const uint8_t* flags, int64_t n, int32_t want, int64_t* out,
                    int64_t* host_out, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------- residual filter --
+ * Replaces the residual step on a join's pairs: the per-condition
+ * evaluate_join masks, their `&=`, the nonzero and the two index selects of
+ * operators/hashjoin.py:264-290 (_apply_fast_path_conditions) and :428-439
+ * (the sort/search core), with evaluate_join itself at :235-242 and the
+ * comparison semantics of operators/operator_utils.py:438-478.
+ *
+ * The conditions come as one postfix program of n_ins instructions, three
+ * int64 words each (ins[3*k .. 3*k+2]):
+ *   word 0, bits 0-7: PF_RES_CMP, PF_RES_AND, PF_RES_OR, PF_RES_NOT,
+ *     PF_RES_TRUE (AND/OR pop two results and push one, NOT flips the top,
+ *     TRUE pushes true, CMP pushes a comparison);
+ *   for CMP: bits 8-15 the relation (PF_REL_*), PF_RES_F64 compares in
+ *     float64 (else int64), PF_RES_A_COL / PF_RES_B_COL: operand a / b is a
+ *     column slot (else a constant: the int64 value, or the float64's bits
+ *     under PF_RES_F64); words 1 and 2 are operands a and b.
+ * PF_REL_CLOSE is torch.isclose(a, b, rtol=1e-5, atol=1e-5) in float64,
+ * the reference's float `=`.  The program leaves exactly one result, at
+ * most 32 deep; at most PF_RES_MAX_INS instructions.
+ * Column slot s (n_cols <= PF_RES_MAX_COLS): cols[s] (device, col_rows[s]
+ * elements of col_types[s], PF_COL_*), read at left_idx[i] when
+ * col_sides[s] is 0, else at right_idx[i].  An integer column converts to
+ * float64 in a float64 comparison; a float64 column only compares in
+ * float64.
+ *
+ * pf_pair_filter_count evaluates the program on the n pairs (left_idx[i],
+ * right_idx[i]), writes mask_out[i] (optional, bool bytes: evaluate_join's
+ * result), keeps its per-pair results in the workspace, and reads back
+ * host_out[0] = pairs kept, host_out[1] = pair indexes outside their
+ * column's rows (read as 0; the caller raises); one wait.
+ * pf_pair_filter_write, on the same workspace, writes the kept pairs in
+ * pair order into left_out / right_out (int64[host_out[0]]); no wait. */
+#define PF_RES_CMP 1
+#define PF_RES_AND 2
+#define PF_RES_OR 3
+#define PF_RES_NOT 4
+#define PF_RES_TRUE 5
+#define PF_REL_EQ 0
+#define PF_REL_NE 1
+#define PF_REL_LT 2
+#define PF_REL_LE 3
+#define PF_REL_GT 4
+#define PF_REL_GE 5
+#define PF_REL_CLOSE 6
+#define PF_RES_F64 (1LL << 16)
+#define PF_RES_A_COL (1LL << 17)
+#define PF_RES_B_COL (1LL << 18)
+#define PF_COL_I64 0
+#define PF_COL_I32 1
+#define PF_COL_I16 2
+#define PF_COL_I8 3
+#define PF_COL_U8 4
+#define PF_COL_F64 5
+#define PF_RES_MAX_COLS 16
+#define PF_RES_MAX_INS 64
+size_t pf_pair_filter_workspace_bytes(int64_t n);
+int pf_pair_filter_count(const int64_t* left_idx, const int64_t* right_idx, int64_t n,
+                         const void* const* cols, const int32_t* col_types,
+                         const int32_t* col_sides, const int64_t* col_rows, int32_t n_cols,
+                         const int64_t* ins, int32_t n_ins, uint8_t* mask_out,
+                         int64_t* host_out, void* workspace, size_t workspace_bytes,
+                         void* stream);
+int pf_pair_filter_write(const int64_t* left_idx, const int64_t* right_idx, int64_t n,
+                         int64_t* left_out, int64_t* right_out, void* workspace,
+                         void* stream);
+
 /* ---------------------------------------------------------- group-by --
  * Replaces the SUM/COUNT evaluation of operators/operator_utils.py:342-399
  * under KeyedAggregate.pass1 (operators/aggregate.py:1511-1563), in exact

@@ -377,3 +377,133 @@ def group_sum_count(gid, val, G: int) -> Tuple[np.ndarray, np.ndarray]:
     c = np.empty(G, np.int64)
     _lib().o_group_sum_count(_p(g), _p(v), g.shape[0], G, _p(s), _p(c))
     return s, c
+
+
+# ------------------------------------------------- residual conditions --
+
+_EPS = 1e-5  # constants.py:8
+_REL = {'<=': np.less_equal, '>=': np.greater_equal, '<': np.less, '>': np.greater,
+        '=': np.equal, '<>': np.not_equal}
+_ARITH = {'+', '-', '*', '/'}
+
+
+def _normalize(tree):
+    while isinstance(tree, list) and len(tree) == 1:  # operator_utils.py:22-26
+        tree = tree[0]
+    return tree
+
+
+def _col_id(token) -> Optional[int]:
+    """parsing/expressions.py:17-32."""
+    import re
+    if not isinstance(token, str) or token.startswith('Brand#'):
+        return None
+    head = token.split(' as', 1)[0] if ' as' in token else token
+    m = re.search(r'#(\d+)L?$', head)
+    return int(m.group(1)) if m else None
+
+
+def isclose_f64(a, b, rtol: float = 1e-5, atol: float = 1e-5) -> np.ndarray:
+    """torch.isclose(a, b, rtol, atol) on float64 (equal_nan False): equal,
+    or a finite |a - b| at most atol + |rtol * b|."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    with np.errstate(invalid="ignore", over="ignore"):
+        actual = np.abs(a - b)
+        allowed = atol + np.abs(rtol * b)
+        return (a == b) | (np.isfinite(actual) & (actual <= allowed))
+
+
+def _compare(lhs, rhs, op: str):
+    """ops[op](lhs, rhs) with torch's operand rules
+    (operator_utils.py:476-478): the left operand is a tensor; a Python int
+    is cast to an integer tensor's dtype (wrapping) or to float64; a Python
+    float against an integer tensor compares in float32 (the default
+    dtype); two tensors compare in their promoted dtype."""
+    if not isinstance(lhs, np.ndarray):
+        raise TypeError(f"{op}: the left operand is a literal")
+    if isinstance(rhs, np.ndarray):
+        return _REL[op](lhs, rhs)
+    if isinstance(rhs, (bool, np.bool_)):
+        raise TypeError(f"{op}: a bool literal")
+    if isinstance(rhs, int):
+        if lhs.dtype.kind == 'f':
+            return _REL[op](lhs, np.float64(rhs))
+        if not -(1 << 63) <= rhs < (1 << 63):
+            raise OverflowError(f"{op}: {rhs} outside int64")
+        return _REL[op](lhs, np.array([rhs], dtype=np.int64).astype(lhs.dtype)[0])
+    if isinstance(rhs, float):
+        if lhs.dtype.kind == 'f':
+            return _REL[op](lhs, np.float64(rhs))
+        return _REL[op](lhs.astype(np.float32), np.float32(rhs))
+    raise TypeError(f"{op}: operand {rhs!r}")
+
+
+def residual_mask(left: dict, right: dict, tree, left_m, right_m) -> np.ndarray:
+    """evaluate_join (hashjoin.py:235-242) through _evaluate_core in filter
+    mode (operator_utils.py:235-478): a column id the left side holds is read
+    at left_m, otherwise at right_m; the value of the condition per pair."""
+    import datetime
+    lm, rm = _i64(left_m), _i64(right_m)
+    n = lm.shape[0]
+
+    def leaf(cid):
+        t = left[cid][lm] if cid in left else right[cid][rm]
+        return np.squeeze(t)
+
+    def ev(t):
+        t = _normalize(t)
+        if isinstance(t, str):  # :250-262
+            if len(t) == 10 and t.count('-') == 2 and t.replace('-', '').isdigit():
+                y, m, d = (int(x) for x in t.split('-'))  # conversion.py:24-27
+                return (datetime.date(y, m, d) - datetime.date(1990, 1, 1)).days
+            if t.replace('.', '').replace('-', '').replace('E', '').isdigit():
+                return float(t) if '.' in t else int(t)
+            cid = _col_id(t)
+            return leaf(cid) if cid is not None else t
+        if len(t) == 3 and t[1] == '#':  # :272-276
+            return leaf(int(t[-1]))
+        if len(t) == 2:
+            if t[0] == 'NOT':  # :309-310
+                return ~ev(t[1])
+            if t[0] == 'isnotnull':  # :312-315
+                return np.ones(n, dtype=bool)
+            raise NotImplementedError(f"function {t[0]}")
+        op = t[-2]
+        if op == 'AS':  # :435-436
+            return ev(t[0])
+        if op in ('AND', 'OR'):
+            a, b = ev(t[0]), ev(t[-1])
+            return np.logical_and(a, b) if op == 'AND' else np.logical_or(a, b)
+        if op in _ARITH or op in ('IN', 'LIKE'):
+            raise NotImplementedError(f"operation {op}")
+        if op not in _REL:
+            raise ValueError(f"Unsupported operation: {op}")
+        lhs, rhs = ev(t[0]), ev(t[-1])
+        if isinstance(rhs, float) and op in ('<=', '>='):  # :458-462
+            return _compare(lhs, rhs + _EPS if op == '<=' else rhs - _EPS, op)
+        if isinstance(rhs, str):  # :464-472
+            if len(rhs) == 1:
+                return _compare(lhs, ord(rhs), op)
+            raise NotImplementedError("string comparison")
+        if op == '=' and isinstance(lhs, np.ndarray) and lhs.dtype == np.float64:  # :474-477
+            if isinstance(rhs, float):
+                rhs = np.full(lhs.shape, rhs, dtype=np.float64)
+            if not isinstance(rhs, np.ndarray) or rhs.dtype != np.float64:
+                raise TypeError("isclose: both operands float64 tensors")
+            return isclose_f64(lhs, rhs)
+        return _compare(lhs, rhs, op)
+
+    out = ev(tree)
+    return np.broadcast_to(np.asarray(out, dtype=bool), (n,)).copy()
+
+
+def residual_filter(left: dict, right: dict, conditions, left_m, right_m
+                    ) -> Tuple[np.ndarray, np.ndarray]:
+    """hashjoin.py:276-290 / :428-439: the pairs every condition keeps, in
+    pair order."""
+    lm, rm = _i64(left_m), _i64(right_m)
+    keep = np.ones(lm.shape[0], dtype=bool)
+    for cond in conditions:
+        keep &= residual_mask(left, right, cond, lm, rm)
+    return lm[keep], rm[keep]

@@ -53,6 +53,10 @@ _SIGNATURES = {
     "pf_mark_rows": (ctypes.c_int, [P, I64, P, I64, I32, P]),
     "pf_select_rows_workspace_bytes": (SZ, [I64]),
     "pf_select_rows": (ctypes.c_int, [P, I64, I32, P, P, P, SZ, P]),
+    "pf_pair_filter_workspace_bytes": (SZ, [I64]),
+    "pf_pair_filter_count": (ctypes.c_int, [P, P, I64, P, P, P, P, I32, P, I32, P, P, P, SZ,
+                                            P]),
+    "pf_pair_filter_write": (ctypes.c_int, [P, P, I64, P, P, P, P]),
     "pf_group_sum_count": (ctypes.c_int, [P, P, I64, I32, P, P, I32, P, P]),
     "pf_dim_table_slots": (I64, [I64]),
     "pf_dim_table_build": (ctypes.c_int, [P, P, I64, I32, P, I64, P, P]),
@@ -150,6 +154,13 @@ def void_array(ptrs):
 
 def i32_array(vals):
     arr = (ctypes.c_int32 * max(len(vals), 1))()
+    for i, v in enumerate(vals):
+        arr[i] = int(v)
+    return arr
+
+
+def i64_array(vals):
+    arr = (ctypes.c_int64 * max(len(vals), 1))()
     for i, v in enumerate(vals):
         arr[i] = int(v)
     return arr

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <type_traits>
 
 #include "pf_common.hpp"
 #include "../../include/pageflow_join.h"
@@ -457,6 +458,281 @@ int build_set(const SetWs& s, const int64_t* bk, const int64_t* bv, int64_t nb, 
   return launch_check("semi_anti build");
 }
 
+// ---------------------------------------------------- residual filter --
+//
+// The residual conditions on a join's pairs in one pass: every thread
+// interprets the same postfix program (its words are kernel arguments, read
+// through scalar loads, so the dispatch on opcode, relation and column type
+// is uniform) for RF_G pairs at a time, the operand gathers of those pairs
+// in flight together; a per-pair bit stack holds the partial results.  The
+// pass writes one 64-bit ballot word per (round, wave) of a 4096-pair tile
+// and the tile's count; after the device scan the compaction pass reads the
+// words and writes the kept pairs at their scanned positions, reading only
+// the kept pairs' indexes.
+
+constexpr int RF_G = 4;  // pairs a thread evaluates together
+static_assert(CT_IT % RF_G == 0, "rounds split into groups");
+
+struct RfProgram {
+  const void* col[PF_RES_MAX_COLS];
+  int64_t rows[PF_RES_MAX_COLS];
+  int32_t type[PF_RES_MAX_COLS];
+  int32_t side[PF_RES_MAX_COLS];
+  int64_t ins[PF_RES_MAX_INS][3];
+  int32_t n_ins;
+};
+
+// torch.isclose(a, b, rtol=1e-5, atol=1e-5) in float64: a == b, or a
+// finite |a - b| within atol + |rtol * b|, each step rounded once.
+__device__ __forceinline__ bool rf_close(double a, double b) {
+  if (a == b) return true;
+  const double err = fabs(__dsub_rn(a, b));
+  const double tol = __dadd_rn(1e-5, fabs(__dmul_rn(1e-5, b)));
+  return isfinite(err) && err <= tol;
+}
+
+__device__ __forceinline__ bool rf_rel_i(int rel, int64_t a, int64_t b) {
+  switch (rel) {
+    case PF_REL_EQ: return a == b;
+    case PF_REL_NE: return a != b;
+    case PF_REL_LT: return a < b;
+    case PF_REL_LE: return a <= b;
+    case PF_REL_GT: return a > b;
+    default: return a >= b;
+  }
+}
+
+__device__ __forceinline__ bool rf_rel_f(int rel, double a, double b) {
+  switch (rel) {
+    case PF_REL_EQ: return a == b;
+    case PF_REL_NE: return a != b;
+    case PF_REL_LT: return a < b;
+    case PF_REL_LE: return a <= b;
+    case PF_REL_GT: return a > b;
+    case PF_REL_GE: return a >= b;
+    default: return rf_close(a, b);
+  }
+}
+
+// Column values at idx: integers sign- (uint8 zero-) extended, float64 as
+// its bits; F64 converts integers to float64 bits.  An index outside the
+// column reads 0 and counts in bad.
+template <typename T, bool F64>
+__device__ __forceinline__ void rf_fetch(const void* p, int64_t rows,
+                                         const int64_t (&idx)[RF_G], const bool (&live)[RF_G],
+                                         int64_t (&v)[RF_G], int& bad) {
+  const T* c = (const T*)p;
+#pragma unroll
+  for (int j = 0; j < RF_G; ++j) {
+    const bool in = live[j] && (uint64_t)idx[j] < (uint64_t)rows;
+    bad += live[j] && !in;
+    const T x = in ? c[idx[j]] : (T)0;
+    if constexpr (sizeof(T) == 8 && !std::is_integral<T>::value)
+      v[j] = __double_as_longlong(x);
+    else if constexpr (F64)
+      v[j] = __double_as_longlong((double)x);
+    else
+      v[j] = (int64_t)x;
+  }
+}
+
+template <bool F64>
+__device__ __forceinline__ void rf_operand(const RfProgram& P, bool is_col, int64_t w,
+                                           const int64_t (&li)[RF_G],
+                                           const int64_t (&ri)[RF_G],
+                                           const bool (&live)[RF_G], int64_t (&v)[RF_G],
+                                           int& bad) {
+  if (!is_col) {
+#pragma unroll
+    for (int j = 0; j < RF_G; ++j) v[j] = w;
+    return;
+  }
+  const int s = (int)w;
+  const bool right = P.side[s] != 0;
+  int64_t idx[RF_G];
+#pragma unroll
+  for (int j = 0; j < RF_G; ++j) idx[j] = right ? ri[j] : li[j];
+  const void* p = P.col[s];
+  const int64_t rows = P.rows[s];
+  switch (P.type[s]) {
+    case PF_COL_I64: rf_fetch<int64_t, F64>(p, rows, idx, live, v, bad); break;
+    case PF_COL_I32: rf_fetch<int32_t, F64>(p, rows, idx, live, v, bad); break;
+    case PF_COL_I16: rf_fetch<int16_t, F64>(p, rows, idx, live, v, bad); break;
+    case PF_COL_I8: rf_fetch<int8_t, F64>(p, rows, idx, live, v, bad); break;
+    case PF_COL_U8: rf_fetch<uint8_t, F64>(p, rows, idx, live, v, bad); break;
+    default: rf_fetch<double, F64>(p, rows, idx, live, v, bad); break;
+  }
+}
+
+__global__ __launch_bounds__(CT_NT) void k_rf_eval(const int64_t* __restrict__ left,
+                                                   const int64_t* __restrict__ right, int64_t n,
+                                                   const RfProgram P,
+                                                   uint64_t* __restrict__ words,
+                                                   uint8_t* __restrict__ mask_out,
+                                                   int32_t* __restrict__ tilecnt,
+                                                   int64_t* __restrict__ bad_out) {
+  __shared__ int red[CT_W];
+  const int64_t base = (int64_t)blockIdx.x * CT_TILE;
+  const int w = threadIdx.x / kWave;
+  int kept = 0, bad = 0;
+#pragma unroll 1
+  for (int g = 0; g < CT_IT; g += RF_G) {
+    int64_t li[RF_G], ri[RF_G];
+    bool live[RF_G];
+    uint32_t stk[RF_G];
+#pragma unroll
+    for (int j = 0; j < RF_G; ++j) {
+      const int64_t i = base + (int64_t)(g + j) * CT_NT + threadIdx.x;
+      live[j] = i < n;
+      li[j] = live[j] ? left[i] : 0;
+      ri[j] = live[j] ? right[i] : 0;
+      stk[j] = 0;
+    }
+#pragma unroll 1
+    for (int pc = 0; pc < P.n_ins; ++pc) {
+      const int64_t w0 = P.ins[pc][0];
+      const int opc = (int)(w0 & 0xff);
+      if (opc == PF_RES_CMP) {
+        const int rel = (int)((w0 >> 8) & 0xff);
+        int64_t a[RF_G], b[RF_G];
+        if (w0 & PF_RES_F64) {
+          rf_operand<true>(P, (w0 & PF_RES_A_COL) != 0, P.ins[pc][1], li, ri, live, a, bad);
+          rf_operand<true>(P, (w0 & PF_RES_B_COL) != 0, P.ins[pc][2], li, ri, live, b, bad);
+#pragma unroll
+          for (int j = 0; j < RF_G; ++j)
+            stk[j] = (stk[j] << 1) |
+                     (uint32_t)rf_rel_f(rel, __longlong_as_double(a[j]),
+                                        __longlong_as_double(b[j]));
+        } else {
+          rf_operand<false>(P, (w0 & PF_RES_A_COL) != 0, P.ins[pc][1], li, ri, live, a, bad);
+          rf_operand<false>(P, (w0 & PF_RES_B_COL) != 0, P.ins[pc][2], li, ri, live, b, bad);
+#pragma unroll
+          for (int j = 0; j < RF_G; ++j) stk[j] = (stk[j] << 1) | (uint32_t)rf_rel_i(rel, a[j], b[j]);
+        }
+      } else if (opc == PF_RES_AND) {
+#pragma unroll
+        for (int j = 0; j < RF_G; ++j) {
+          const uint32_t t = stk[j] & 1u;
+          stk[j] >>= 1;
+          stk[j] &= ~1u | t;
+        }
+      } else if (opc == PF_RES_OR) {
+#pragma unroll
+        for (int j = 0; j < RF_G; ++j) {
+          const uint32_t t = stk[j] & 1u;
+          stk[j] = (stk[j] >> 1) | t;
+        }
+      } else if (opc == PF_RES_NOT) {
+#pragma unroll
+        for (int j = 0; j < RF_G; ++j) stk[j] ^= 1u;
+      } else {  // PF_RES_TRUE
+#pragma unroll
+        for (int j = 0; j < RF_G; ++j) stk[j] = (stk[j] << 1) | 1u;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < RF_G; ++j) {
+      const bool keep = live[j] && (stk[j] & 1u);
+      const uint64_t bal = ballot(keep);
+      if (lane_id() == 0) words[(base + (int64_t)(g + j) * CT_NT) / kWave + w] = bal;
+      if (mask_out && live[j]) mask_out[base + (int64_t)(g + j) * CT_NT + threadIdx.x] = keep;
+      kept += keep;
+    }
+  }
+  kept = block_sum_ct(kept, red);
+  if (threadIdx.x == 0) tilecnt[blockIdx.x] = kept;
+  bad = block_sum_ct(bad, red);
+  if (threadIdx.x == 0 && bad) atomicAdd((unsigned long long*)bad_out, (unsigned long long)bad);
+}
+
+// The kept pairs of a tile at its scanned offset, in pair order.
+__global__ __launch_bounds__(CT_NT) void k_rf_compact(const int64_t* __restrict__ left,
+                                                      const int64_t* __restrict__ right,
+                                                      const uint64_t* __restrict__ words,
+                                                      const int64_t* __restrict__ offs,
+                                                      int64_t* __restrict__ left_out,
+                                                      int64_t* __restrict__ right_out) {
+  __shared__ int cnt[kWave];
+  const int64_t base = (int64_t)blockIdx.x * CT_TILE;
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  bool keep[CT_IT];
+#pragma unroll
+  for (int j = 0; j < CT_IT; ++j)
+    keep[j] = (words[(int64_t)blockIdx.x * (CT_TILE / kWave) + j * CT_W + w] >> lane) & 1u;
+  int64_t pos[CT_IT];
+  tile_positions(keep, offs[blockIdx.x], pos, cnt);
+#pragma unroll
+  for (int j = 0; j < CT_IT; ++j) {
+    if (keep[j]) {
+      const int64_t i = base + j * CT_NT + threadIdx.x;
+      left_out[pos[j]] = left[i];
+      right_out[pos[j]] = right[i];
+    }
+  }
+}
+
+struct RfWs {
+  uint64_t* words;  // tiles * 64 ballot words
+  int32_t* tilecnt;
+  int64_t* offs;
+  int64_t* dev;  // [0] kept pairs, [1] indexes outside their column
+  void* scan;
+  size_t bytes;
+};
+
+RfWs carve_rf(void* ws, int64_t n) {
+  Carve cv(ws);
+  RfWs s;
+  const int64_t tiles = n > 0 ? (n + CT_TILE - 1) / CT_TILE : 1;
+  s.words = cv.take<uint64_t>(tiles * (CT_TILE / kWave));
+  s.tilecnt = cv.take<int32_t>(tiles);
+  s.offs = cv.take<int64_t>(tiles);
+  s.dev = cv.take<int64_t>(2);
+  s.scan = cv.take<char>(scan_i32_scratch(tiles));
+  s.bytes = cv.bytes();
+  return s;
+}
+
+// Checks the program: opcodes, relations, slots, operand domains and the
+// stack depth (at most 32, exactly one result).
+int rf_validate(const RfProgram& P, int n_cols) {
+  int depth = 0;
+  for (int k = 0; k < P.n_ins; ++k) {
+    const int64_t w0 = P.ins[k][0];
+    const int opc = (int)(w0 & 0xff);
+    const bool f64 = (w0 & PF_RES_F64) != 0;
+    if (opc == PF_RES_CMP) {
+      const int rel = (int)((w0 >> 8) & 0xff);
+      if (rel < PF_REL_EQ || rel > PF_REL_CLOSE || (rel == PF_REL_CLOSE && !f64))
+        return fail("pair_filter: instruction " + std::to_string(k) + " has an unknown relation");
+      for (int o = 0; o < 2; ++o) {
+        if (!(w0 & (o ? PF_RES_B_COL : PF_RES_A_COL))) continue;
+        const int64_t s = P.ins[k][1 + o];
+        if (s < 0 || s >= n_cols)
+          return fail("pair_filter: instruction " + std::to_string(k) + " names column slot " +
+                      std::to_string(s) + " of " + std::to_string(n_cols));
+        if (P.type[s] == PF_COL_F64 && !f64)
+          return fail("pair_filter: a float64 column compares in float64 only");
+      }
+      ++depth;
+    } else if (opc == PF_RES_TRUE) {
+      ++depth;
+    } else if (opc == PF_RES_AND || opc == PF_RES_OR) {
+      if (depth < 2) return fail("pair_filter: AND/OR on fewer than two results");
+      --depth;
+    } else if (opc == PF_RES_NOT) {
+      if (depth < 1) return fail("pair_filter: NOT on no result");
+    } else {
+      return fail("pair_filter: instruction " + std::to_string(k) + " has opcode " +
+                  std::to_string(opc));
+    }
+    if (depth > 32) return fail("pair_filter: the program is deeper than 32 results");
+  }
+  if (depth != 1) return fail("pair_filter: the program leaves " + std::to_string(depth) +
+                              " results, not one");
+  return 0;
+}
+
 }  // namespace
 }  // namespace pf
 
@@ -655,6 +931,77 @@ int pf_join_expand_write(const int64_t* sorted_ids, const int64_t* start, int64_
   k_expand<<<grid_for(n_flat, EX_TILE), EX_NT, 0, st>>>(sorted_ids, start, ex, m, n_flat, cs,
                                                         cr);
   if (launch_check("join_expand")) return 1;
+  return ok();
+}
+
+size_t pf_pair_filter_workspace_bytes(int64_t n) { return pf::carve_rf(nullptr, n).bytes; }
+
+int pf_pair_filter_count(const int64_t* left_idx, const int64_t* right_idx, int64_t n,
+                         const void* const* cols, const int32_t* col_types,
+                         const int32_t* col_sides, const int64_t* col_rows, int32_t n_cols,
+                         const int64_t* ins, int32_t n_ins, uint8_t* mask_out,
+                         int64_t* host_out, void* workspace, size_t workspace_bytes,
+                         void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (n < 0) return fail("pair_filter: negative pairs");
+  if (n_cols < 0 || n_cols > PF_RES_MAX_COLS)
+    return fail("pair_filter: at most " + std::to_string(PF_RES_MAX_COLS) + " columns");
+  if (n_ins < 1 || n_ins > PF_RES_MAX_INS)
+    return fail("pair_filter: 1 to " + std::to_string(PF_RES_MAX_INS) + " instructions");
+  if (!ins || (n_cols > 0 && (!cols || !col_types || !col_sides || !col_rows)))
+    return fail("pair_filter: missing program or column arrays");
+  if (n > 0 && (!left_idx || !right_idx)) return fail("pair_filter: missing pair indexes");
+  if (workspace_bytes < pf_pair_filter_workspace_bytes(n))
+    return fail("pair_filter: workspace too small");
+  RfProgram P{};
+  for (int s = 0; s < n_cols; ++s) {
+    if (col_types[s] < PF_COL_I64 || col_types[s] > PF_COL_F64)
+      return fail("pair_filter: column slot " + std::to_string(s) + " has an unknown type");
+    if (col_rows[s] < 0 || (col_rows[s] > 0 && !cols[s]))
+      return fail("pair_filter: column slot " + std::to_string(s) + " has no rows pointer");
+    P.col[s] = cols[s];
+    P.rows[s] = col_rows[s];
+    P.type[s] = col_types[s];
+    P.side[s] = col_sides[s] != 0;
+  }
+  P.n_ins = n_ins;
+  for (int k = 0; k < n_ins; ++k)
+    for (int q = 0; q < 3; ++q) P.ins[k][q] = ins[3 * k + q];
+  if (rf_validate(P, n_cols)) return 1;
+  void* hdev = nullptr;
+  if (host_alias(host_out, &hdev)) return 1;
+  RfWs s = carve_rf(workspace, n);
+  const int64_t tiles = (n + CT_TILE - 1) / CT_TILE;
+  if (fill_i64(s.dev, 2, 0, st)) return 1;
+  if (n > 0) {
+    KTimer _kt("rf_eval", st);
+    k_rf_eval<<<(unsigned)tiles, CT_NT, 0, st>>>(left_idx, right_idx, n, P, s.words, mask_out,
+                                                  s.tilecnt, s.dev + 1);
+    if (launch_check("pair_filter eval")) return 1;
+    if (scan_i32_to_i64_excl(s.tilecnt, s.offs, tiles, s.dev, s.scan, st)) return 1;
+  }
+  k_readback<<<1, 64, 0, st>>>(s.dev, 2, (int64_t*)hdev);
+  if (launch_check("pair_filter")) return 1;
+  if (wait_stream(st, "pair_filter")) return 1;
+  return ok();
+}
+
+int pf_pair_filter_write(const int64_t* left_idx, const int64_t* right_idx, int64_t n,
+                         int64_t* left_out, int64_t* right_out, void* workspace,
+                         void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (n < 0) return fail("pair_filter: negative pairs");
+  if (n == 0) return ok();
+  RfWs s = carve_rf(workspace, n);
+  const int64_t tiles = (n + CT_TILE - 1) / CT_TILE;
+  {
+    KTimer _kt("rf_compact", st);
+    k_rf_compact<<<(unsigned)tiles, CT_NT, 0, st>>>(left_idx, right_idx, s.words, s.offs,
+                                                     left_out, right_out);
+  }
+  if (launch_check("pair_filter write")) return 1;
   return ok();
 }
 

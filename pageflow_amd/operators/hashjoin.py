@@ -19,11 +19,13 @@ global indexes when asked).  The path taken, in order:
    build bucket is too large for its LDS sort, the sort/search program
    itself (a device sort, two searches and the expand kernel).
 
-The join-type tails (right-semi / right-anti / right-outer) and the payload
-gather (with the null row for right-outer) run on the library's kernels.
-The residual evaluator covers comparisons (=, <>, !=, <, <=, >, >=) of
-column references and integer literals under AND, OR and NOT; the
-reference's general expression evaluator is outside this path.
+The residual conditions on the pairs compile to one device program
+(``operators.residual``): a kernel evaluates every condition on every pair
+and the kept pairs are compacted in pair order, where the reference gathers,
+compares, ANDs masks and selects per condition (hashjoin.py:235-290,
+:428-439).  The join-type tails (right-semi / right-anti / right-outer) and
+the payload gather (with the null row for right-outer) run on the library's
+kernels.
 """
 from __future__ import annotations
 
@@ -35,6 +37,7 @@ from .. import join as _join
 from .. import ops
 from ..layout import NULL
 from . import join_core
+from . import residual as _residual
 
 LEFT, RIGHT = 0, 1
 
@@ -69,21 +72,8 @@ def _join_core_fallback(reason: str):
 
 # ------------------------------------------------------------ expressions --
 
-def normalize_expr_tree(tree):
-    """Unwraps single-item list wrappers of a parsed condition."""
-    while isinstance(tree, list) and len(tree) == 1:
-        tree = tree[0]
-    return tree
-
-
-def column_ref(token: str) -> Optional[int]:
-    """The column id of ``name#id`` / ``#id``, else None."""
-    if not isinstance(token, str) or '#' not in token:
-        return None
-    try:
-        return int(token.rsplit('#', 1)[1])
-    except ValueError:
-        return None
+normalize_expr_tree = _residual.normalize_expr_tree
+column_ref = _residual.column_ref
 
 
 def _proof_is_covered(unique_key_sets, selected_ids) -> bool:
@@ -201,47 +191,12 @@ def _try_hash_semi_anti(tensors_left, tensors_right, left_key, right_key, condit
     return join_core.semi_anti(build, probe, anti, summary)
 
 
-_CMP = {'=': torch.eq, '==': torch.eq, '<>': torch.ne, '!=': torch.ne, '<': torch.lt,
-        '<=': torch.le, '>': torch.gt, '>=': torch.ge}
-
-
-def evaluate_join(tensors_left, tensors_right, tree, leftidx, rightidx) -> torch.Tensor:
-    """A residual condition on pairs: a column id the left side holds is
+def evaluate_join(device_name, tensorsa, tensorsb, tree, leftidx, rightidx) -> torch.Tensor:
+    """One residual condition's value on the pairs (leftidx[i], rightidx[i])
+    as a bool mask (hashjoin.py:235-242): a column id the left side holds is
     read on the left rows, otherwise on the right rows."""
-    tree = normalize_expr_tree(tree)
-
-    def value(tok):
-        tok = normalize_expr_tree(tok)
-        cid = column_ref(tok) if isinstance(tok, str) else None
-        if cid is not None:
-            if cid in tensors_left:
-                return ops.gather_rows(tensors_left[cid], leftidx)
-            return ops.gather_rows(tensors_right[cid], rightidx)
-        if isinstance(tok, str):
-            try:
-                return int(tok)
-            except ValueError:
-                pass
-        if isinstance(tok, int):
-            return tok
-        raise NotImplementedError(f"residual operand {tok!r} is outside the join path's "
-                                  "evaluator")
-
-    if isinstance(tree, list) and len(tree) == 2 and tree[0] == 'NOT':
-        return ~evaluate_join(tensors_left, tensors_right, tree[1], leftidx, rightidx)
-    if isinstance(tree, list) and len(tree) == 3 and tree[1] in ('AND', 'OR'):
-        a = evaluate_join(tensors_left, tensors_right, tree[0], leftidx, rightidx)
-        b = evaluate_join(tensors_left, tensors_right, tree[2], leftidx, rightidx)
-        return a & b if tree[1] == 'AND' else a | b
-    if isinstance(tree, list) and len(tree) == 3 and tree[1] in _CMP:
-        a, b = value(tree[0]), value(tree[2])
-        res = _CMP[tree[1]](torch.as_tensor(a, device=leftidx.device)
-                            if isinstance(a, int) else a, b)
-        if res.dim() == 0:
-            res = res.expand(leftidx.numel())
-        return res
-    raise NotImplementedError(f"residual condition {tree!r} is outside the join path's "
-                              "evaluator")
+    prog = _residual.compile_conditions(tensorsa, tensorsb, [tree])
+    return _residual.evaluate_pairs(prog, leftidx, rightidx)
 
 
 def _condition_col_id(token):
@@ -256,12 +211,12 @@ def _is_hashed_equality_condition(condition, hashed_pairs) -> bool:
 
 
 def _filter_pairs(tensors_left, tensors_right, conditions, left_m, right_m):
+    """The pairs every condition keeps, in pair order: one program, one
+    evaluation pass and one compaction over the pairs."""
     if not conditions or left_m.numel() == 0:
         return left_m, right_m
-    keep = torch.ones(left_m.numel(), dtype=torch.bool, device=left_m.device)
-    for cond in conditions:
-        keep &= evaluate_join(tensors_left, tensors_right, cond, left_m, right_m)
-    return left_m[keep], right_m[keep]
+    prog = _residual.compile_conditions(tensors_left, tensors_right, conditions)
+    return _residual.filter_pairs(prog, left_m, right_m)
 
 
 def _apply_fast_path_conditions(tensors_left, tensors_right, condition_list, left_matched,

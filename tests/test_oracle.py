@@ -305,3 +305,134 @@ def test_partition_outputs_is_the_stable_padded_layout():
         for k in range(B):
             want = t[[i for i in range(n) if b[i] == k]]
             assert np.array_equal(p[starts[k]:starts[k] + counts[k]], want), (rb, k)
+
+
+# ------------------------------------------------- residual conditions --
+
+def _raises(fn):
+    try:
+        return False, fn()
+    except (TypeError, RuntimeError, OverflowError, ValueError, NotImplementedError,
+            AttributeError) as e:
+        return True, e
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_residual_mask_is_the_torch_evaluation(seed):
+    """The oracle's comparison rules (numpy with torch's scalar casts,
+    promotions and isclose restated) give torch's own answers on random
+    condition trees over every column dtype and literal kind; where torch
+    raises, the oracle raises."""
+    from residual_cases import condition, sides, torch_mask
+    rng = np.random.default_rng(seed)
+    left, right = sides(rng, 300, 200)
+    n = 2000
+    lm = rng.integers(0, 300, n)
+    rm = rng.integers(0, 200, n)
+    agreed = raised = 0
+    for _ in range(150):
+        tree = condition(rng, allow_unsupported=True)
+        t_err, want = _raises(lambda: torch_mask(left, right, tree, lm, rm))
+        o_err, got = _raises(lambda: oracle.residual_mask(left, right, tree, lm, rm))
+        assert t_err == o_err, (tree, want, got)
+        if t_err:
+            raised += 1
+            continue
+        assert got.dtype == bool and got.shape == (n,)
+        assert np.array_equal(got, want), tree
+        agreed += 1
+    assert agreed > 80 and raised > 3
+
+
+def test_isclose_is_torch_isclose_at_the_tolerance_boundary():
+    import torch
+    rng = np.random.default_rng(3)
+    b = np.concatenate([rng.standard_normal(2000) * 10.0 ** rng.integers(-8, 8, 2000),
+                        [0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, 1e308, 5e-324]])
+    with np.errstate(invalid="ignore", over="ignore"):
+        tol = 1e-5 + np.abs(1e-5 * b)
+        cols = [b, b + tol, b - tol, np.nextafter(b + tol, np.inf),
+                np.nextafter(b - tol, -np.inf), b + 0.5 * tol, b * (1 + 2e-5),
+                np.full_like(b, np.inf), np.full_like(b, np.nan)]
+    for a in cols:
+        with np.errstate(invalid="ignore", over="ignore"):
+            got = oracle.isclose_f64(a, b)
+        want = torch.isclose(torch.from_numpy(a), torch.from_numpy(b), atol=1e-5).numpy()
+        assert np.array_equal(got, want)
+    # closed forms: the tolerance is 1e-5 + 1e-5 |b|, infinities equal themselves
+    assert oracle.isclose_f64(0.0, 1e-5) and not oracle.isclose_f64(0.0, 1.1e-5)
+    assert oracle.isclose_f64(1.0 + 2e-5, 1.0) and not oracle.isclose_f64(1.0 + 2.1e-5, 1.0)
+    assert oracle.isclose_f64(np.inf, np.inf) and not oracle.isclose_f64(np.inf, -np.inf)
+    assert not oracle.isclose_f64(np.nan, np.nan)
+
+
+def test_residual_filter_known_answers():
+    # tests/operators/test_unique_hash_join.py:251-272: #5 > #6 on the pairs
+    # (0,0), (1,1), (2,2) keeps (1,1)
+    left = {1: np.array([1, 2, 3]), 5: np.array([10, 20, 30])}
+    right = {2: np.array([1, 2, 3]), 6: np.array([15, 15, 35])}
+    lo, ro = oracle.residual_filter(left, right, [["#5", ">", "#6"]], [0, 1, 2], [0, 1, 2])
+    assert lo.tolist() == [1] and ro.tolist() == [1]
+    # Q21's residual NOT (l_suppkey#3 = l_suppkey#4), every pair of a 4 x 3
+    left = {3: np.array([7, 8, 7, 9])}
+    right = {4: np.array([7, 9, 1])}
+    lm, rm = np.repeat(np.arange(4), 3), np.tile(np.arange(3), 4)
+    q21 = ['NOT', ['l_suppkey#3', '=', 'l_suppkey#4']]
+    lo, ro = oracle.residual_filter(left, right, [q21], lm, rm)
+    want = [(i, j) for i in range(4) for j in range(3) if left[3][i] != right[4][j]]
+    assert list(zip(lo.tolist(), ro.tolist())) == want
+    # several conditions AND, pair order kept; an id on both sides reads left
+    both = {5: np.array([1, 2, 3])}
+    lo, ro = oracle.residual_filter(both, {5: np.array([9, 9, 9]), 6: np.array([0, 5, 1])},
+                                    [['#5', '>', '1'], ['#6', '<', '#5']],
+                                    [2, 1, 0, 2], [2, 2, 2, 0])
+    assert list(zip(lo.tolist(), ro.tolist())) == [(2, 2), (1, 2), (2, 0)]
+
+
+def test_residual_literal_rules():
+    """Closed forms of the literal rules: dates are days since 1990-01-01,
+    an integer literal wraps to the column dtype, a float literal under <=
+    / >= is shifted by 1e-5, and a one-character string is its code
+    point."""
+    i32 = {1: np.array([1898, 1899, 1900], dtype=np.int32)}
+    r = {2: np.zeros(3, np.int64)}
+    idx = np.arange(3)
+    assert oracle.residual_mask(i32, r, ['#1', '=', '1995-03-15'], idx, idx).tolist() == \
+        [False, True, False]
+    assert oracle.residual_mask(i32, r, ['#1', '>', str(2 ** 32 + 1899)], idx, idx).tolist() == \
+        [False, False, True]
+    f = {1: np.array([1.0, 1.0 + 0.9e-5, 1.0 + 1.1e-5])}
+    assert oracle.residual_mask(f, r, ['#1', '<=', '1.0'], idx, idx).tolist() == \
+        [True, True, False]
+    assert oracle.residual_mask(f, r, ['#1', '<', '1.0'], idx, idx).tolist() == \
+        [False, False, False]
+    s = {1: np.array([64, 65, 66], dtype=np.int8)}
+    assert oracle.residual_mask(s, r, ['#1', '=', 'A'], idx, idx).tolist() == \
+        [False, True, False]
+    u8 = {1: np.array([0, 100, 255], dtype=np.uint8)}
+    assert oracle.residual_mask(u8, r, ['#1', '<', '-1'], idx, idx).tolist() == \
+        [True, True, False]
+
+
+def test_residual_constants_mirror_the_header():
+    import re
+    from pageflow_amd.operators import residual
+    text = open(os.path.join(os.path.dirname(__file__), "..", "include",
+                             "pageflow_join.h")).read()
+    d = {k: v for k, v in re.findall(r"#define (PF_(?:RES|REL|COL)_\w+) \(?(\d+)", text)}
+    assert int(d["PF_RES_CMP"]) == residual.CMP and int(d["PF_RES_AND"]) == residual.AND
+    assert int(d["PF_RES_OR"]) == residual.OR and int(d["PF_RES_NOT"]) == residual.NOT
+    assert int(d["PF_RES_TRUE"]) == residual.TRUE
+    for name, rel in {"EQ": '=', "NE": '<>', "LT": '<', "LE": '<=', "GT": '>',
+                      "GE": '>='}.items():
+        assert int(d[f"PF_REL_{name}"]) == residual.REL[rel]
+    assert int(d["PF_REL_CLOSE"]) == residual.REL_CLOSE
+    assert re.search(r"PF_RES_F64 \(1LL << 16\)", text) and residual.F64 == 1 << 16
+    assert re.search(r"PF_RES_A_COL \(1LL << 17\)", text) and residual.A_COL == 1 << 17
+    assert re.search(r"PF_RES_B_COL \(1LL << 18\)", text) and residual.B_COL == 1 << 18
+    import torch
+    for name, dt in {"I64": torch.int64, "I32": torch.int32, "I16": torch.int16,
+                     "I8": torch.int8, "U8": torch.uint8, "F64": torch.float64}.items():
+        assert int(d[f"PF_COL_{name}"]) == residual.COL_TYPE[dt]
+    assert int(d["PF_RES_MAX_COLS"]) == residual.MAX_COLS
+    assert int(d["PF_RES_MAX_INS"]) == residual.MAX_INS
