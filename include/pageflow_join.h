@@ -187,11 +187,13 @@ int pf_radix_partition_shape(const void* keys, int32_t key_bytes, const int32_t*
  * sides): both sides radix-partitioned by the top `bits` bits of mix64(key)
  * (bits <= 12), each bucket joined against an LDS hash table of its build
  * rows.  multi = 0: a unique build (a probe row takes the lowest build row
- * with its key); multi = 1: every matching build row.  Two calls on one
- * workspace: prepare partitions, counts and waits (twice), leaving in
- * host_out (pinned int64[3]) [0] the pair count (-1: a build bucket is too
- * large for the multi path, [1] its rows; the caller takes another path),
- * [1] the largest build bucket, [2] the work items; write fills
+ * with its key); multi = 1: every matching build row, each build bucket
+ * sorted by (hash, row) so a key's rows are one run (in LDS up to 8192
+ * rows; a larger bucket by stable 8-bit LSD passes of the radix partition
+ * over its hashes in HBM, after one more wait).  Two calls on one
+ * workspace: prepare partitions, counts and waits, leaving in host_out
+ * (pinned int64[3]) [0] the pair count, [1] the largest build bucket, [2]
+ * the work items; write fills
  * out_probe/out_build (int64, at least the pair count rows) and gathers
  * npay (<= 2) int64 build payload columns by build row into pay_out.  Pairs
  * come in a deterministic order (bucket, then probe row in partition

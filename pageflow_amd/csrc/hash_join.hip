@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "pf_common.hpp"
 #include "../../include/pageflow_join.h"
@@ -56,6 +57,7 @@ constexpr int PCH_M = 65536;
 int pch_of(bool multi) { return multi ? PCH_M : PCH_U; }
 constexpr int MAX_PAY = 2;
 constexpr int kOnePassBits = 8;            // wider partitions take two passes
+constexpr int kSortDigit = 8;              // a bucket sorted in HBM: bits a pass
 
 // the widest digit one partition pass of partition_side takes
 int pass_bits(int bits) { return bits <= kOnePassBits ? bits : bits - bits / 2; }
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(SORT_NT) void hj_sort_buckets(uint64_t* __restrict_
   __shared__ uint64_t sh[SORT_CAP];
   __shared__ int32_t sr[SORT_CAP];
   const int64_t b0 = boff[blockIdx.x], n = boff[blockIdx.x + 1] - b0;
-  if (n <= 1) return;
+  if (n <= 1 || n > SORT_CAP) return;  // larger buckets are sorted in HBM
   int m = 2;
   while (m < n) m <<= 1;
   for (int i = threadIdx.x; i < m; i += SORT_NT) {
@@ -475,6 +477,7 @@ struct HjWs {
   uint64_t* th;        // the first pass's output of a two-pass partition
   int32_t* trow;
   int64_t* item_cnt;   // [max items] (counts, then scanned in place)
+  int64_t* sort_off;   // multi: a sort pass's 257 digit offsets
   int64_t* pay_part[MAX_PAY];  // build payloads in partition order
   void* part_ws;
   void* scan;
@@ -498,15 +501,19 @@ HjWs carve_hj(void* ws, int64_t nb, int64_t np, int bits, bool multi) {
   w.poff = cv.take<int64_t>(P + 1);
   w.item_off = cv.take<int64_t>(P + 1);
   w.item_bucket = cv.take<int32_t>(max_items(np, bits, multi));
-  const int64_t nt = bits > kOnePassBits ? (nb > np ? nb : np) : 1;
+  // the two-pass partition's first output; multi: also the ping-pong
+  // buffer of a build bucket sorted in HBM
+  const int64_t nt = bits > kOnePassBits ? (nb > np ? nb : np) : multi ? nb : 1;
   w.th = cv.take<uint64_t>(nt > 0 ? nt : 1);
   w.trow = cv.take<int32_t>(nt > 0 ? nt : 1);
   w.info = cv.take<int64_t>(4);
   w.total = cv.take<int64_t>(2);
   const int64_t mi = max_items(np, bits, multi);
   w.item_cnt = cv.take<int64_t>(mi);
+  w.sort_off = cv.take<int64_t>(multi ? 257 : 1);
   for (int c = 0; c < MAX_PAY; ++c) w.pay_part[c] = cv.take<int64_t>(nb > 0 ? nb : 1);
-  const size_t pws = radix_partition_ws(nb > np ? nb : np, pass_bits(bits));
+  size_t pws = radix_partition_ws(nb > np ? nb : np, pass_bits(bits));
+  if (multi && radix_partition_ws(nb, kSortDigit) > pws) pws = radix_partition_ws(nb, kSortDigit);
   w.part_ws = cv.take<char>(pws);
   w.scan = cv.take<char>(scan_i64_scratch(mi));
   w.bytes = cv.bytes();
@@ -533,6 +540,43 @@ int partition_side(const int64_t* keys, int64_t n, int bits, uint64_t* h, int32_
     return 1;
   hj_bucket_bounds<<<grid_for((1 << bits) + 1, 256), 256, 0, st>>>(h, n, bits, boff);
   return launch_check("hash_join bounds");
+}
+
+// Multi-match build buckets of more than SORT_CAP rows, sorted by (hash,
+// row) in HBM: a bucket's rows are in row order (the partition is stable),
+// so stable LSD passes over the hash bits below the top `bits` order them
+// by (hash, row), as the LDS sort orders the smaller buckets.  One wait for
+// the bucket bounds.
+int sort_big_buckets(const HjWs& w, int64_t nb, int bits, hipStream_t st) {
+  const int P = 1 << bits;
+  std::vector<int64_t> off(P + 1);
+  if (hip_check(hipMemcpyAsync(off.data(), w.boff, (P + 1) * sizeof(int64_t),
+                               hipMemcpyDeviceToHost, st), "hash_join bucket bounds"))
+    return 1;
+  if (wait_stream(st, "hash_join")) return 1;
+  size_t pws = radix_partition_ws(nb > 0 ? nb : 1, kSortDigit);
+  for (int b = 0; b < P; ++b) {
+    const int64_t n = off[b + 1] - off[b];
+    if (n <= SORT_CAP) continue;
+    uint64_t* k[2] = {w.bh + off[b], w.th};
+    int32_t* r[2] = {w.brow + off[b], w.trow};
+    int cur = 0;
+    for (int shift = 0; shift < 64 - bits; shift += kSortDigit) {
+      const int width = 64 - bits - shift < kSortDigit ? 64 - bits - shift : kSortDigit;
+      if (radix_partition(k[cur], 8, r[cur], n, width, shift, 2, k[cur ^ 1], r[cur ^ 1],
+                          w.sort_off, w.part_ws, pws, st, 4, -1))
+        return 1;
+      cur ^= 1;
+    }
+    if (cur == 1) {
+      if (hip_check(hipMemcpyAsync(k[0], k[1], n * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                                   st), "hash_join sort copy") ||
+          hip_check(hipMemcpyAsync(r[0], r[1], n * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                   st), "hash_join sort copy"))
+        return 1;
+    }
+  }
+  return 0;
 }
 
 HjParams params_of(const HjWs& w, int bits, bool multi, int64_t n_items) {
@@ -592,14 +636,8 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
   if (wait_stream(st, "hash_join")) return 1;
   const int64_t n_items = host_out[0];
   const int64_t largest = host_out[1];
-  if (multi && largest > SORT_CAP) {
-    // a build bucket too large to sort in LDS: the caller takes another path
-    host_out[0] = -1;
-    host_out[1] = largest;
-    host_out[2] = 0;
-    return ok();
-  }
   if (n_items > max_items(np, bits, multi != 0)) return fail("hash_join: work items overflow");
+  if (multi && largest > SORT_CAP && sort_big_buckets(w, nb, bits, st)) return 1;
   if (multi && nb > 0) {
     {
     KTimer _kt("hj_sort_buckets", st);

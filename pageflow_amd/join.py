@@ -58,9 +58,9 @@ class HashJoin:
         self.pairs = 0
 
     def prepare(self, build_keys: torch.Tensor, probe_keys: torch.Tensor) -> int:
-        """Partitions both sides and counts the pairs (two host waits).
-        Returns the pair count, or -1 when a build bucket is too large for
-        the multi path."""
+        """Partitions both sides and counts the pairs (two host waits; three
+        when a multi-match build bucket is larger than the LDS sort takes
+        and is sorted in HBM).  Returns the pair count."""
         _check_keys(build_keys, "build keys")
         _check_keys(probe_keys, "probe keys")
         if build_keys.numel() != self.nb or probe_keys.numel() != self.np:
@@ -103,15 +103,11 @@ class HashJoin:
 def hash_join(build_keys: torch.Tensor, probe_keys: torch.Tensor,
               payloads: Sequence[torch.Tensor] = (), unique: bool = True,
               bits: Optional[int] = None
-              ) -> Optional[Tuple[torch.Tensor, torch.Tensor, List[torch.Tensor]]]:
+              ) -> Tuple[torch.Tensor, torch.Tensor, List[torch.Tensor]]:
     """(probe rows, build rows, gathered build payloads) of an inner
-    equi-join of int64 keys, in the deterministic partition order; None when
-    the multi path cannot take a build bucket (too many rows of one
-    bucket)."""
+    equi-join of int64 keys, in the deterministic partition order."""
     plan = HashJoin(build_keys.numel(), probe_keys.numel(), unique, bits, build_keys.device)
     n = plan.prepare(build_keys, probe_keys)
-    if n < 0:
-        return None
     dev = build_keys.device
     out_p = torch.empty(n, dtype=torch.int64, device=dev)
     out_b = torch.empty(n, dtype=torch.int64, device=dev)

@@ -15,9 +15,9 @@ global indexes when asked).  The path taken, in order:
 3. otherwise every pair: here the partitioned multi-match LDS-table join
    (``pageflow_amd.join.hash_join``), which stands in for the reference's
    sort/search core (hashjoin.py:380-441) and yields the same pair multiset
-   (the reference's own tests compare these pairs as multisets); when a
-   build bucket is too large for its LDS sort, the sort/search program
-   itself (a device sort, two searches and the expand kernel).
+   (the reference's own tests compare these pairs as multisets), every
+   bucket in one pass; a build bucket too large for the LDS sort is sorted
+   in HBM by the library's radix partition first.
 
 The residual conditions on the pairs compile to one device program
 (``operators.residual``): a kernel evaluates every condition on every pair
@@ -272,9 +272,10 @@ def _gather_join_result(tensors_left, tensors_right, join_output_keys, return_gl
 
 def _sort_search_pairs(tensors_left, tensors_right, left, right, condition_list,
                        bucket_counts, device, name="join"):
-    """Every matching (left row, right row) pair, residuals applied.  The
-    multi-match LDS-table join; the sort/search program (device sort, two
-    searches, the expand kernel) when a bucket is too large for it."""
+    """Every matching (left row, right row) pair, residuals applied: the
+    multi-match partitioned join over all buckets in one pass (a build
+    bucket too large for the LDS sort is sorted in HBM first), then the
+    residual program over the pairs."""
     nL, nR = left.size(0), right.size(0)
     if bucket_counts is not None:
         lc, rc = bucket_counts
@@ -283,14 +284,7 @@ def _sort_search_pairs(tensors_left, tensors_right, left, right, condition_list,
     if nL == 0 or nR == 0:
         e = torch.empty(0, dtype=torch.int64, device=device)
         return e, e.clone()
-    res = _join.hash_join(left.contiguous(), right.contiguous(), unique=False)
-    if res is not None:
-        right_m, left_m, _ = res
-    else:
-        sk, perm = torch.sort(left)
-        lo = torch.searchsorted(sk, right)
-        length = torch.searchsorted(sk, right, right=True) - lo
-        left_m, right_m = join_core.expand(perm, lo, length)
+    right_m, left_m, _ = _join.hash_join(left.contiguous(), right.contiguous(), unique=False)
     if condition_list and left_m.numel() > INT_ROWS_MAX:
         raise RuntimeError(f"{name}: a core call of {nL} left and {nR} right rows makes "
                            f"{left_m.numel()} pairs, over the {INT_ROWS_MAX} its residual "

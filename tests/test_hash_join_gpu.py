@@ -88,9 +88,28 @@ def test_oversized_build_buckets_take_chunked_tables():
     _check(bk, rng.integers(0, 1600, 50_000), False, bits=0)
 
 
-def test_multi_refuses_a_bucket_past_the_sort_capacity():
-    bk = torch.arange(20_000, dtype=torch.int64, device="cuda")
-    assert hash_join(bk, bk.clone(), unique=False, bits=0) is None
+@pytest.mark.parametrize("bits,nb,nkeys", [(0, 8192, 3000), (0, 8193, 3000), (0, 20_000, 20_000),
+                                           (0, 50_000, 3000), (3, 120_000, 40_000)])
+def test_multi_sorts_buckets_past_the_lds_capacity_in_hbm(bits, nb, nkeys):
+    # buckets of more than 8192 build rows are sorted by (hash, row) in HBM
+    # (all 64 hash bits at bits 0), the smaller ones in LDS
+    rng = np.random.default_rng(nb + bits)
+    bk = rng.integers(0, nkeys, nb)
+    pk = rng.integers(-50, nkeys + 50, 200_000)
+    _check(bk, pk, False, bits)
+
+
+def test_multi_hot_key_bucket_sorted_in_hbm_is_deterministic():
+    rng = np.random.default_rng(8)
+    bk = np.concatenate([np.full(12_000, 77, np.int64), rng.integers(0, 30_000, 30_000)])
+    rng.shuffle(bk)
+    pk = np.where(rng.random(300_000) < 0.0004, 77, rng.integers(0, 31_000, 300_000))
+    # 9 bits: 55 hash bits sorted in 7 passes, the last one's output copied back
+    _check(bk, pk, False, bits=9)
+    b, p = torch.as_tensor(bk).cuda(), torch.as_tensor(pk).cuda()
+    x = hash_join(b, p, unique=False, bits=9)
+    y = hash_join(b, p, unique=False, bits=9)
+    assert torch.equal(x[0], y[0]) and torch.equal(x[1], y[1])
 
 
 def test_skewed_probe_and_duplicate_heavy_build():
