@@ -422,18 +422,17 @@ struct RpWs {
 
 // Shapes of the <= 8-bit scatter (threads, rows a thread per tile); the
 // default is the measured best on MI355X, a caller (the autotuner) may pick
-// another.
+// another.  512 x 16 tiles give each bucket twice the rows a tile (longer
+// runs per write) but take 96 KB of LDS, one workgroup a CU: faster from
+// 64M rows up (6-bit passes over 640M rows: 4.18 -> 4.00 ms a pass; 400M:
+// 2.57 -> 2.43 ms), slower on a few hundred chunks (10M rows: 0.07 -> 0.10
+// ms), where 512 x 8 tiles keep several workgroups a CU.
 struct Shape8 {
   int nt, it;
 };
 constexpr Shape8 kShapes8[4] = {{256, 8}, {256, 16}, {512, 8}, {512, 16}};
-constexpr int kDefaultShape8 = 2;
-
-int tile_of(int bits, int shape) {
-  if (bits > 8) return RpShape<12>::NT * RpShape<12>::IT;
-  const Shape8 sh = kShapes8[shape < 0 || shape > 3 ? kDefaultShape8 : shape];
-  return sh.nt * sh.it;
-}
+constexpr int64_t kLargeTileRows = 64LL << 20;  // rows from which 512 x 16 tiles win
+int default_shape8(int64_t n) { return n >= kLargeTileRows ? 3 : 2; }
 
 // The largest tile of any shape fixes the chunk rows, so one workspace size
 // serves every shape.
@@ -486,7 +485,7 @@ int rp_launch(const K* keys, const int32_t* vals, int64_t n, int bits, int shift
       rp_scatter<K, KO, MODE, 12><<<(unsigned)nchunks, RpShape<12>::NT, 0, st>>>(
           keys, vals, n, crows, bits, shift, nchunks, r.offs, kout, vout);
     } else {
-      switch (shape < 0 || shape > 3 ? kDefaultShape8 : shape) {
+      switch (shape < 0 || shape > 3 ? default_shape8(n) : shape) {
         case 0: launch8<K, KO, MODE, 256, 8>(keys, vals, n, crows, bits, shift, nchunks, r.offs,
                                              kout, vout, st); break;
         case 1: launch8<K, KO, MODE, 256, 16>(keys, vals, n, crows, bits, shift, nchunks,

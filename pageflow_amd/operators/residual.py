@@ -265,16 +265,6 @@ def compile_conditions(tensors_left: Dict, tensors_right: Dict,
     return p
 
 
-_host: Dict[int, torch.Tensor] = {}
-
-
-def _host_buffer(dev: torch.device) -> torch.Tensor:
-    i = dev.index if dev.index is not None else torch.cuda.current_device()
-    if i not in _host:
-        _host[i] = torch.empty(2, dtype=torch.int64, pin_memory=True)
-    return _host[i]
-
-
 def _count(prog: Program, left_m: torch.Tensor, right_m: torch.Tensor,
            mask: Optional[torch.Tensor]):
     _lib.require_cuda(left_m, "the left pair rows")
@@ -285,10 +275,15 @@ def _count(prog: Program, left_m: torch.Tensor, right_m: torch.Tensor,
         raise ValueError("residual: the pair rows are two 1-D tensors of one length")
     n = left_m.numel()
     dev = left_m.device
+    if right_m.device != dev:
+        raise ValueError("residual: the pair rows are on two devices")
+    for t in prog.cols:
+        if t.device != dev:
+            raise ValueError(f"residual: a column on {t.device}, the pairs on {dev}")
     lib = _lib.load()
     ws_bytes = lib.pf_pair_filter_workspace_bytes(n)
     ws = _lib.workspace(ws_bytes, dev)
-    host = _host_buffer(dev)
+    host = torch.empty(2, dtype=torch.int64, pin_memory=True)  # per call: thread-safe
     flat = [w for ins in prog.ins for w in ins]
     with torch.cuda.device(dev):
         _lib.check(lib.pf_pair_filter_count(
