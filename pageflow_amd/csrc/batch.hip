@@ -346,8 +346,8 @@ __global__ __launch_bounds__(NT) void bj_compact(const int64_t* __restrict__ p_o
       }
       if (pr >= 0) {
         const int64_t pos = o + pre + mbcnt(bal);
-        stream_store(&out_probe[pos], rmap ? (int64_t)rmap[r] : r00 + r);
-        stream_store(&out_build[pos], (int64_t)pr);
+        out_probe[pos] = rmap ? (int64_t)rmap[r] : r00 + r;
+        out_build[pos] = (int64_t)pr;
       }
       o += tot;
       __syncthreads();

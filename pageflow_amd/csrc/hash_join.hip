@@ -387,10 +387,10 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
             }
             const int li = s_hi[wb + lo] + (j - s_ex[wb + lo]);
             const int64_t o = base + j;
-            stream_store(&p.out_probe[o], (int64_t)s_pr[wb + lo]);
-            stream_store(&p.out_build[o], (int64_t)rows[li]);
-            if (has_pay) stream_store(&p.pay_out[0][o], pay0[li]);
-            if (p.npay > 1) stream_store(&p.pay_out[1][o], p.pay_src[1][cb + li]);
+            p.out_probe[o] = (int64_t)s_pr[wb + lo];
+            p.out_build[o] = (int64_t)rows[li];
+            if (has_pay) p.pay_out[0][o] = pay0[li];
+            if (p.npay > 1) p.pay_out[1][o] = p.pay_src[1][cb + li];
           }
           wave_lds_sync();  // s_* are rewritten for the next slot
         }
@@ -399,10 +399,10 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
         for (int k = 0; k < HJ_IT; ++k) {
           if (m[k] == 0) continue;
           const int64_t o = pos + cnt[k * HJ_W + w] + inw[k];
-          stream_store(&p.out_probe[o], (int64_t)pr[k]);
-          stream_store(&p.out_build[o], (int64_t)rows[hidx[k]]);
-          if (has_pay) stream_store(&p.pay_out[0][o], pay0[hidx[k]]);
-          if (p.npay > 1) stream_store(&p.pay_out[1][o], p.pay_src[1][cb + hidx[k]]);
+          p.out_probe[o] = (int64_t)pr[k];
+          p.out_build[o] = (int64_t)rows[hidx[k]];
+          if (has_pay) p.pay_out[0][o] = pay0[hidx[k]];
+          if (p.npay > 1) p.pay_out[1][o] = p.pay_src[1][cb + hidx[k]];
         }
       }
       // the round's last (slot, wave) total, from the wave that holds it

@@ -68,10 +68,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// A store of an output no kernel of the call reads again: marked
-// nontemporal, so the stream does not displace the staged data in L2.
-template <typename T>
-__device__ __forceinline__ void stream_store(T* p, T v) { __builtin_nontemporal_store(v, p); }
 
 // Inclusive prefix sum across the 64 lanes of a wave.
 template <typename T>
