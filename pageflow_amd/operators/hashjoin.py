@@ -29,7 +29,7 @@ kernels.
 """
 from __future__ import annotations
 
-from typing import Literal, Optional
+from typing import Literal
 
 import torch
 
