@@ -256,3 +256,30 @@ def test_c_abi_validates_programs():
     assert _raw_count(deep) != 0 and "deeper" in _last_error()
     ok = [(R.TRUE, 0, 0)] * 32 + [(R.AND, 0, 0)] * 31
     assert _raw_count(ok) == 0
+
+
+def test_sixteen_columns_and_a_deep_program():
+    # the column limit: 16 slots (8 a side, every dtype), and 35 program steps
+    rng = np.random.default_rng(21)
+    dts = [np.int64, np.int32, np.int16, np.int8, np.uint8, np.float64, np.int64, np.int32]
+    left = {2 * j + 1: rng.integers(0, 4, 900).astype(dt) for j, dt in enumerate(dts)}
+    right = {2 * j + 2: rng.integers(0, 4, 700).astype(dt) for j, dt in enumerate(dts)}
+    right[12] = right[12] + 0.5
+    ops = ['=', '<>', '<', '<=', '>', '>=']
+    conds = []
+    for j in range(8):
+        a, b = 2 * j + 1, 2 * ((j + 3) % 8) + 2
+        op = ops[j % 6]
+        if dts[j] == np.float64 and op == '=':
+            op = '<>'
+        inner = [f'l#{a}', op, f'r#{b}']
+        conds.append(['NOT', [inner, 'OR', [f'l#{a}', '>=', '2']]] if j % 2 else
+                     [inner, 'AND', ['isnotnull', f'r#{b}']])
+    n = 123_457
+    lm, rm = rng.integers(0, 900, n), rng.integers(0, 700, n)
+    prog = residual.compile_conditions(_dev(left), _dev(right), conds)
+    assert len(prog.cols) == 16 and len(prog.ins) <= residual.MAX_INS
+    gl, gr = residual.filter_pairs(prog, torch.from_numpy(lm).cuda(), torch.from_numpy(rm).cuda())
+    wl, wr = oracle.residual_filter(left, right, conds, lm, rm)
+    assert gl.numel() > 0
+    assert np.array_equal(gl.cpu().numpy(), wl) and np.array_equal(gr.cpu().numpy(), wr)
