@@ -8,7 +8,7 @@
 // column, then the int64 gidx = goff + pre-mask row) starts every non-empty
 // bucket on a whole lcm(row bytes, 512)-byte unit, padding rows unwritten.
 //
-// gfx950 design: a 2048-row tile per 512-thread workgroup (8 wave64s, four
+// gfx950 design: a 4096-row tile per 1024-thread workgroup (16 wave64s, four
 // 64-row rounds per wave).  The stable rank inside a tile comes from
 // ballots (one per bucket bit plus one for validity) and mbcnt, with
 // per-wave running counts in LDS as u16; the tile is staged in LDS in
@@ -40,7 +40,8 @@ constexpr int MAXC = 6;     // data columns a call
 constexpr int MAXCLS = 4;   // row-width classes, the gidx's 8 B included
 constexpr int BMAX = 512;   // buckets
 constexpr int LOGBMAX = 9;
-constexpr int NT = 512;     // threads a tile
+constexpr int NT = 1024;    // threads a tile (4096-row tiles: 3.59 -> 3.31 ms
+                            // against 512, aggregation pass 1 into 256 buckets)
 constexpr int NW = NT / kWave;
 constexpr int IT = 4;       // rounds a wave
 constexpr int TILE = NT * IT;
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(NT) void ps_hist(Params p) {
   const int64_t seg = blk * TILE + (int64_t)w * WROWS;
   bool valid[IT];
   load_valid<SRC>(p, seg, valid);
-  int jl[IT] = {0, 0, 0, 0};
+  int jl[IT] = {};
   int64_t coff = 0;
   if (!SRC && p.key_comp) {
     tile_compact(valid, jl, wv);  // its barrier also ends the zeroing
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(NT) void ps_scatter(Params p) {
 
   bool valid[IT];
   load_valid<SRC>(p, seg, valid);
-  int jl[IT] = {0, 0, 0, 0};
+  int jl[IT] = {};
   int64_t coff = 0;
   if (!SRC && p.needs_j) {
     tile_compact(valid, jl, wv);
