@@ -139,14 +139,19 @@ __global__ __launch_bounds__(1024) void hj_items(const int64_t* __restrict__ bof
 // ---------------------------------------------------------- LDS sort --
 
 // Block b sorts build bucket b's (hash, row) pairs by hash, then row
-// (bitonic network in LDS; the bucket holds at most SORT_CAP rows).
+// (bitonic network in LDS; the bucket holds at most SORT_CAP rows).  The LDS
+// is sized at launch for the largest bucket sorted here (cap rows, a power
+// of two): with buckets of a few thousand rows, several workgroups share a
+// CU where a fixed SORT_CAP-row allocation would leave one.
 __global__ __launch_bounds__(SORT_NT) void hj_sort_buckets(uint64_t* __restrict__ bh,
                                                            int32_t* __restrict__ brow,
-                                                           const int64_t* __restrict__ boff) {
-  __shared__ uint64_t sh[SORT_CAP];
-  __shared__ int32_t sr[SORT_CAP];
+                                                           const int64_t* __restrict__ boff,
+                                                           int cap) {
+  extern __shared__ uint64_t sort_lds[];
+  uint64_t* sh = sort_lds;
+  int32_t* sr = (int32_t*)(sort_lds + cap);
   const int64_t b0 = boff[blockIdx.x], n = boff[blockIdx.x + 1] - b0;
-  if (n <= 1 || n > SORT_CAP) return;  // larger buckets are sorted in HBM
+  if (n <= 1 || n > cap) return;  // larger buckets are sorted in HBM
   int m = 2;
   while (m < n) m <<= 1;
   for (int i = threadIdx.x; i < m; i += SORT_NT) {
@@ -641,7 +646,15 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
   if (multi && nb > 0) {
     {
     KTimer _kt("hj_sort_buckets", st);
-    hj_sort_buckets<<<(unsigned)P, SORT_NT, 0, st>>>(w.bh, w.brow, w.boff);
+    int cap = 2;  // the largest bucket sorted in LDS, to a power of two
+    while (cap < largest && cap < SORT_CAP) cap <<= 1;
+    const size_t lds = (size_t)cap * (sizeof(uint64_t) + sizeof(int32_t));
+    if (hip_check(hipFuncSetAttribute((const void*)hj_sort_buckets,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(SORT_CAP * (sizeof(uint64_t) + sizeof(int32_t)))),
+                  "hash_join sort LDS"))
+      return 1;
+    hj_sort_buckets<<<(unsigned)P, SORT_NT, lds, st>>>(w.bh, w.brow, w.boff, cap);
   }
     if (launch_check("hash_join sort")) return 1;
   }
