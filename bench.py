@@ -78,6 +78,20 @@ def cpu_baseline(build_cpu, probe_cpu, sample_rows: int):
                       f"payload gather, one thread, median of 3 ({t:.2f} s)"}
 
 
+def cpu_sample(fn, units: float, unit: str, what: str, reps: int = 3):
+    """A tier's CPU baseline: the oracle on a bounded sample of the tier's
+    own inputs, one thread, median of reps runs."""
+    fn()  # warm (the oracle's C library loads on first use)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
+    return {"value": units / t, "unit": unit, "cores": 1, "kind": "port",
+            "sample": f"{what}; median of {reps} ({t:.2f} s)"}
+
+
 def bench_partition(args, dev) -> None:
     """The standalone radix partition of the large join's 640M probe keys
     (int64) with int32 row ids into 2^bits buckets, for each bits given.
@@ -216,6 +230,15 @@ def bench_agg_pass1(args, dev, h_all):
     torch.cuda.synchronize()
     torch_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     sc_gbps = 44.0 * n / (sc * 1e-3) / 1e9  # the scatter: ids + 3 outputs in, 3 outputs out
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        m = 4_000_000
+        cols = [t[:m].cpu().numpy() for t in (h, v64, v32)]
+        bk = bucket[:m].cpu().numpy()
+        cpu = cpu_sample(lambda: oracle.partition_outputs(cols, bk, B), m, "rows/s",
+                         f"the first {m} rows of the same tick, the oracle's padded "
+                         f"partition of the three outputs (numpy stable sort)")
     log(f"aggregation pass 1: {n} rows into {B} buckets {ms:.2f} ms ({gbps:.0f} GB/s), "
         f"ps_scatter {sc:.2f} ms ({sc_gbps:.0f} GB/s); the reference's steps in torch "
         f"{torch_ms:.2f} ms")
@@ -223,7 +246,8 @@ def bench_agg_pass1(args, dev, h_all):
             "torch_reference_steps_ms": torch_ms, "speedup_vs_torch_steps": torch_ms / ms,
             "frac": gbps / HBM_PEAK_GBPS, "ps_scatter_ms": sc, "ps_scatter_GBps": sc_gbps,
             "ps_scatter_frac": sc_gbps / HBM_PEAK_GBPS,
-            "kernels_ms": {k: v[0] / args.steps for k, v in prof.items()}}
+            "kernels_ms": {k: v[0] / args.steps for k, v in prof.items()},
+            "cpu_baseline": cpu}
 
 
 def bench_batch(args, dev, emit: bool = True):
@@ -412,6 +436,17 @@ def bench_skew(args, dev, emit: bool = True):
     log(f"skew: inner {t_inner * 1e3:.2f} ms ({pairs} pairs), semi {t_semi * 1e3:.2f} ms, "
         f"anti {t_anti * 1e3:.2f} ms")
     res["residual"] = bench_residual(args, dev, ob, op, nb, np_, timed)
+    if not args.no_cpu:
+        import oracle
+        bk = build[:2_500_000].cpu().numpy()
+        ps = probe[:2_000_000].cpu().numpy()
+        res["cpu_baseline"] = cpu_sample(
+            lambda: oracle.sort_search_pairs(bk, ps), bk.shape[0] + ps.shape[0], "tuples/s",
+            f"the first {bk.shape[0]} build and {ps.shape[0]} probe rows joined; the "
+            f"oracle's sort/search inner join (numpy sort, two searchsorted, expand)")
+        res["semi_cpu_baseline"] = cpu_sample(
+            lambda: oracle.semi_anti(bk, ps, False), bk.shape[0] + ps.shape[0], "tuples/s",
+            f"the same sample; the oracle's C semi-join hash set")
     if emit:
         line = {"metric": "skewed multi-match join tuples/s", "value": (nb + np_) / t_inner,
                 "unit": "tuples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -466,11 +501,22 @@ def bench_residual(args, dev, ob, op, nb: int, np_: int, timed):
     log(f"residual: {n} pairs -> {kept}, fused {t_fused * 1e3:.2f} ms, "
         f"torch steps {t_torch * 1e3:.2f} ms")
     kms = {k: v[0] / max(v[1], 1) for k, v in prof.items()}
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        m = 10_000_000
+        lm, rm = ob[:m].cpu().numpy(), op[:m].cpu().numpy()
+        L, R = {11: bpay.cpu().numpy()}, {12: ppay.cpu().numpy()}
+        cond = [['b_pay#11', '<', 'p_pay#12']]
+        cpu = cpu_sample(lambda: oracle.residual_filter(L, R, cond, lm, rm), m, "pairs/s",
+                         f"the first {m} pairs; the oracle's residual evaluation (numpy "
+                         f"gathers, compare, nonzero, two selects)")
     b_eval = 32.0 * n + n / 8.0
     b_comp = n / 8.0 + 32.0 * kept
     out = {"pairs": n, "kept": kept, "fused_ms": t_fused * 1e3, "torch_steps_ms": t_torch * 1e3,
            "speedup": t_torch / t_fused, "pairs_per_s": n / t_fused,
-           "fused_GBps": (b_eval + b_comp) / t_fused / 1e9, "kernels_ms": kms}
+           "fused_GBps": (b_eval + b_comp) / t_fused / 1e9, "kernels_ms": kms,
+           "cpu_baseline": cpu}
     if "rf_eval" in kms:
         out["rf_eval_GBps"] = b_eval / (kms["rf_eval"] * 1e-3) / 1e9
         out["rf_eval_frac"] = out["rf_eval_GBps"] / HBM_PEAK_GBPS
@@ -635,11 +681,21 @@ def bench_uva(args, dev, emit: bool = True):
     g_link = g_lines * 128.0 / t_gather / 1e9
     m_link = m_lines * 128.0 / t_mask / 1e9
     kern = {k: {"total_ms": v[0] / steps, "launches": v[1] / steps} for k, v in prof.items()}
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        m = 8_000_000
+        hcol = [torch.cat(parts).numpy().reshape(-1, 1)]  # one host part
+        sidx = idx[:m].cpu().numpy()
+        cpu = cpu_sample(lambda: oracle.zero_copy_rearrange(hcol, 0, sidx, 0), m, "rows/s",
+                         f"the first {m} of the same indices over the same column in host "
+                         f"memory; the oracle's gather (numpy), read by the host")
     res = {"column_rows": n, "gather_rows": n_out, "null_every": 100, "mask_kept": kept,
            "gather_ms": t_gather * 1e3, "gather_rows_per_s": n_out / t_gather,
            "gather_link_GBps": g_gbps, "mask_ms": t_mask * 1e3, "mask_link_GBps": m_gbps,
            "h2d_copy_GBps": h2d, "gather_lines_GBps": g_link, "mask_lines_GBps": m_link,
            "gather_frac_of_h2d": g_link / h2d, "mask_frac_of_h2d": m_link / h2d,
+           "cpu_baseline": cpu,
            "kernels_ms": kern}
     log(f"uva: gather {n_out} rows {t_gather * 1e3:.2f} ms ({g_gbps:.1f} GB/s of rows, "
         f"{g_link:.1f} GB/s of 128-B host lines = {g_link / h2d:.2f} of a {h2d:.1f} GB/s H2D "

@@ -345,7 +345,7 @@ def zero_copy_rearrange(parts: Sequence[np.ndarray], first_row: int, rearrange,
     out = np.empty((idx.shape[0], width), dtype=dtype)
     # segments back to back: row-space row first_row + k is row k of their
     # concatenation, each segment holding [start, start + len)
-    column = np.concatenate(parts)
+    column = parts[0] if len(parts) == 1 else np.concatenate(parts)
     rows = row_start + idx
     valid = idx != NULL
     outside = valid & ((rows < first_row) | (rows >= first_row + column.shape[0]))
