@@ -34,16 +34,11 @@ from .partitions import JOIN_B_MAX as _B_MAX
 _MAX_COLS = 6
 _MAX_WIDTHS = 4
 
-_host: Optional[torch.Tensor] = None
-
-
 def _host_buffer() -> torch.Tensor:
-    """The process's pinned read-back buffer (counts, padded totals, -1,
-    out-of-range bucket ids)."""
-    global _host
-    if _host is None:
-        _host = torch.empty(_B_MAX + _MAX_WIDTHS + 2, dtype=torch.int64, pin_memory=True)
-    return _host
+    """A call's pinned read-back buffer (counts, padded totals, -1,
+    out-of-range bucket ids), from torch's caching host allocator, so
+    concurrent calls do not share it."""
+    return torch.empty(_B_MAX + _MAX_WIDTHS + 2, dtype=torch.int64, pin_memory=True)
 
 
 def _mix64(h: torch.Tensor) -> torch.Tensor:
