@@ -278,14 +278,20 @@ int pf_zero_copy_rearrange(const int64_t* seg_ptrs, const int64_t* seg_rows, int
   const int64_t* idx = rearrange;
   const int32_t* pos = nullptr;
   const int bits = span_bits(row_stop - row_start);
-  if (n >= ZC_SORT_MIN && bits > 0) {
-    // stable LSD passes of 8 bits: the indices ascending, each with its
-    // output position
+  // rows sharing a 128-B host line need no order among themselves: the sort
+  // skips the index bits below the line (4 for 8-B rows, one pass of four
+  // for a 2^28-row span)
+  const int64_t row_bytes = (int64_t)elem_bytes * width;
+  int low = 0;
+  while (low < bits && (row_bytes << (low + 1)) <= 128) ++low;
+  if (n >= ZC_SORT_MIN && bits > low) {
+    // stable LSD passes of 8 bits: the indices ascending by line, each with
+    // its output position
     const int64_t* kin = rearrange;
     const int32_t* vin = nullptr;
     int64_t* kout = z.k0;
     int32_t* vout = z.v0;
-    for (int shift = 0; shift < bits; shift += 8) {
+    for (int shift = low; shift < bits; shift += 8) {
       const int b = bits - shift < 8 ? bits - shift : 8;
       if (radix_partition(kin, 8, vin, n, b, shift, kRaw, kout, vout, z.offs, z.rp, z.rp_bytes,
                           st, kWhoSort, -1))
