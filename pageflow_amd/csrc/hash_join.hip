@@ -58,6 +58,7 @@ int pch_of(bool multi) { return multi ? PCH_M : PCH_U; }
 constexpr int MAX_PAY = 2;
 constexpr int kOnePassBits = 8;            // wider partitions take two passes
 constexpr int kSortDigit = 8;              // a bucket sorted in HBM: bits a pass
+constexpr int kBigBucketsGlobal = 32;      // more buckets past SORT_CAP: one global sort
 
 // the widest digit one partition pass of partition_side takes
 int pass_bits(int bits) { return bits <= kOnePassBits ? bits : bits - bits / 2; }
@@ -552,7 +553,7 @@ int partition_side(const int64_t* keys, int64_t n, int bits, uint64_t* h, int32_
 // so stable LSD passes over the hash bits below the top `bits` order them
 // by (hash, row), as the LDS sort orders the smaller buckets.  One wait for
 // the bucket bounds.
-int sort_big_buckets(const HjWs& w, int64_t nb, int bits, hipStream_t st) {
+int sort_big_buckets(const HjWs& w, int64_t nb, int bits, hipStream_t st, bool* all_sorted) {
   const int P = 1 << bits;
   std::vector<int64_t> off(P + 1);
   if (hip_check(hipMemcpyAsync(off.data(), w.boff, (P + 1) * sizeof(int64_t),
@@ -560,6 +561,23 @@ int sort_big_buckets(const HjWs& w, int64_t nb, int bits, hipStream_t st) {
     return 1;
   if (wait_stream(st, "hash_join")) return 1;
   size_t pws = radix_partition_ws(nb > 0 ? nb : 1, kSortDigit);
+  int big = 0;
+  for (int b = 0; b < P; ++b) big += off[b + 1] - off[b] > SORT_CAP;
+  if (big > kBigBucketsGlobal) {
+    // most buckets large: one stable LSD sort of the whole side by all 64
+    // hash bits (8 passes, ending in place) leaves every bucket in its range
+    // in (hash, row) order, in far fewer launches than a sort a bucket
+    uint64_t* k[2] = {w.bh, w.th};
+    int32_t* r[2] = {w.brow, w.trow};
+    for (int pass = 0; pass < 64 / kSortDigit; ++pass) {
+      const int c = pass & 1;
+      if (radix_partition(k[c], 8, r[c], nb, kSortDigit, pass * kSortDigit, 2, k[c ^ 1],
+                          r[c ^ 1], w.sort_off, w.part_ws, pws, st, 4, -1))
+        return 1;
+    }
+    *all_sorted = true;
+    return 0;
+  }
   for (int b = 0; b < P; ++b) {
     const int64_t n = off[b + 1] - off[b];
     if (n <= SORT_CAP) continue;
@@ -642,8 +660,9 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
   const int64_t n_items = host_out[0];
   const int64_t largest = host_out[1];
   if (n_items > max_items(np, bits, multi != 0)) return fail("hash_join: work items overflow");
-  if (multi && largest > SORT_CAP && sort_big_buckets(w, nb, bits, st)) return 1;
-  if (multi && nb > 0) {
+  bool all_sorted = false;  // every bucket sorted in HBM already
+  if (multi && largest > SORT_CAP && sort_big_buckets(w, nb, bits, st, &all_sorted)) return 1;
+  if (multi && nb > 0 && !all_sorted) {
     {
     KTimer _kt("hj_sort_buckets", st);
     int cap = 2;  // the largest bucket sorted in LDS, to a power of two

@@ -99,6 +99,15 @@ def test_multi_sorts_buckets_past_the_lds_capacity_in_hbm(bits, nb, nkeys):
     _check(bk, pk, False, bits)
 
 
+def test_multi_many_large_buckets_take_one_global_sort():
+    # 64 buckets of ~19K build rows, all past the LDS sort: one sort of the
+    # whole side by every hash bit instead of a sort a bucket
+    rng = np.random.default_rng(12)
+    bk = rng.integers(0, 400_000, 1_200_000)
+    pk = rng.integers(-100, 400_100, 300_000)
+    _check(bk, pk, False, bits=6)
+
+
 def test_multi_hot_key_bucket_sorted_in_hbm_is_deterministic():
     rng = np.random.default_rng(8)
     bk = np.concatenate([np.full(12_000, 77, np.int64), rng.integers(0, 30_000, 30_000)])
