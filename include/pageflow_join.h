@@ -170,7 +170,9 @@ int pf_radix_partition(const void* keys, int32_t key_bytes, const int32_t* vals,
                        size_t workspace_bytes, void* stream);
 /* The same with the scatter's tile shape chosen by the caller (the
  * autotuner): shape 0..3 = (256 threads x 8 rows, 256 x 16, 512 x 8,
- * 512 x 16) a tile for digits of up to 8 bits; -1 the default.  Above 8
+ * 512 x 16), 5 = 1024 x 8, a tile for digits of up to 8 bits; -1 the
+ * default (from 64M rows 1024 x 8 up to 6-bit digits and 512 x 16 above,
+ * 512 x 8 below 64M rows).  Above 8
  * bits the partition runs as two stable passes (the low half of the digit,
  * then the high half, each with that tile shape) and a bucket-bounds
  * kernel, unless shape is 4: one pass of the wide-digit scatter. */

@@ -420,19 +420,24 @@ struct RpWs {
   size_t bytes;
 };
 
-// Shapes of the <= 8-bit scatter (threads, rows a thread per tile); the
-// default is the measured best on MI355X, a caller (the autotuner) may pick
-// another.  512 x 16 tiles give each bucket twice the rows a tile (longer
-// runs per write) but take 96 KB of LDS, one workgroup a CU: faster from
-// 64M rows up (6-bit passes over 640M rows: 4.18 -> 4.00 ms a pass; 400M:
-// 2.57 -> 2.43 ms), slower on a few hundred chunks (10M rows: 0.07 -> 0.10
-// ms), where 512 x 8 tiles keep several workgroups a CU.
-struct Shape8 {
-  int nt, it;
-};
-constexpr Shape8 kShapes8[4] = {{256, 8}, {256, 16}, {512, 8}, {512, 16}};
-constexpr int64_t kLargeTileRows = 64LL << 20;  // rows from which 512 x 16 tiles win
-int default_shape8(int64_t n) { return n >= kLargeTileRows ? 3 : 2; }
+// Shapes of the <= 8-bit scatter (threads, rows a thread per tile): 0..3
+// and 5; the default is the measured best on MI355X, a caller (the
+// autotuner) may pick another.  8192-row tiles give each bucket twice the
+// rows a tile of 512 x 8 (longer runs per write) but take 96-114 KB of LDS,
+// one workgroup a CU: faster from 64M rows up, slower on a few hundred
+// chunks (10M rows: 0.07 -> 0.10 ms), where 512 x 8 tiles keep several
+// workgroups a CU.  Of the 8192-row tiles, 1024 x 8 (16 waves to hide the
+// loads) beats 512 x 16: 6-bit passes over 640M rows 4.18 (512 x 8) ->
+// 4.00 (512 x 16) -> 3.71 ms (1024 x 8); 400M: 2.57 -> 2.43 -> 2.25 ms.
+// shape: 0 = 256 x 8, 1 = 256 x 16, 2 = 512 x 8, 3 = 512 x 16, 5 = 1024 x 8
+constexpr int kShape1k = 5;                     // 1024 threads x 8 rows
+constexpr int64_t kLargeTileRows = 64LL << 20;  // rows from which 8192-row tiles win
+// up to 6-bit digits 1024 x 8; 7-8 bits 512 x 16, whose waves see twice the
+// rows a bucket (8 bits over 640M rows: 5.05 ms against 5.15 for 1024 x 8)
+int default_shape8(int64_t n, int bits) {
+  return n < kLargeTileRows ? 2 : bits <= 6 ? kShape1k : 3;
+}
+bool shape8_valid(int shape) { return (shape >= 0 && shape <= 3) || shape == kShape1k; }
 
 // The largest tile of any shape fixes the chunk rows, so one workspace size
 // serves every shape.
@@ -485,14 +490,16 @@ int rp_launch(const K* keys, const int32_t* vals, int64_t n, int bits, int shift
       rp_scatter<K, KO, MODE, 12><<<(unsigned)nchunks, RpShape<12>::NT, 0, st>>>(
           keys, vals, n, crows, bits, shift, nchunks, r.offs, kout, vout);
     } else {
-      switch (shape < 0 || shape > 3 ? default_shape8(n) : shape) {
+      switch (shape8_valid(shape) ? shape : default_shape8(n, bits)) {
         case 0: launch8<K, KO, MODE, 256, 8>(keys, vals, n, crows, bits, shift, nchunks, r.offs,
                                              kout, vout, st); break;
         case 1: launch8<K, KO, MODE, 256, 16>(keys, vals, n, crows, bits, shift, nchunks,
                                               r.offs, kout, vout, st); break;
         case 2: launch8<K, KO, MODE, 512, 8>(keys, vals, n, crows, bits, shift, nchunks, r.offs,
                                              kout, vout, st); break;
-        default: launch8<K, KO, MODE, 512, 16>(keys, vals, n, crows, bits, shift, nchunks,
+        case 3: launch8<K, KO, MODE, 512, 16>(keys, vals, n, crows, bits, shift, nchunks,
+                                              r.offs, kout, vout, st); break;
+        default: launch8<K, KO, MODE, 1024, 8>(keys, vals, n, crows, bits, shift, nchunks,
                                                r.offs, kout, vout, st); break;
       }
     }
@@ -587,7 +594,7 @@ int two_pass(const void* keys, int key_bytes, const int32_t* vals, int64_t n, in
   const size_t hb = wsb - align256((size_t)n * 8) - align256((size_t)n * 4);
   void* tk = base + hb;
   int32_t* tv = (int32_t*)(base + hb + align256((size_t)n * 8));
-  const int sh = shape >= 0 && shape < kShapeOnePass ? shape : -1;
+  const int sh = shape8_valid(shape) ? shape : -1;
   if (mode == RP_RAW) {
     if (radix_partition(keys, key_bytes, vals, n, lo, shift, RP_RAW, tk, tv, offs, ws, hb, st,
                         who, sh) ||

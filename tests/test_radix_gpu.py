@@ -76,7 +76,7 @@ def test_large_partition_matches_oracle():
     _check(_keys(20_000_000, torch.int64, 11), 10, radix.HASH_KEY)
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 5])
 @pytest.mark.parametrize("bits", [3, 8])
 def test_every_scatter_shape(shape, bits):
     keys = _keys(700_001, torch.int64, 20 + shape)
