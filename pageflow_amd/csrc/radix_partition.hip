@@ -632,7 +632,8 @@ int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
                              void* scan_ws, int32_t* kout, int32_t* vout, hipStream_t st,
                              const char* hist_name, const char* scatter_name) {
   if (nseg <= 0 || nblocks <= 0) return 0;
-  static_assert(kSegChunkRows % (512 * 16) == 0, "a segment chunk is whole scatter tiles");
+  // 1024 x 8 tiles: 7.24 -> 6.66 ms against 512 x 16 on the batch's slice partitions
+  static_assert(kSegChunkRows % (1024 * 8) == 0, "a segment chunk is whole scatter tiles");
   {
     KTimer _kt(hist_name, st);
     rp_hist_seg<<<(unsigned)nblocks, RpShape<8>::NT, 0, st>>>(keys, d_segs, nseg, hist);
@@ -641,7 +642,7 @@ int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
   if (scan_i32_to_i64_excl(hist, offs, hist_entries, nullptr, scan_ws, st)) return 1;
   {
     KTimer _kt(scatter_name, st);
-    rp_scatter8_seg<512, 16><<<(unsigned)nblocks, 512, 0, st>>>(keys, d_segs, nseg, offs, kout,
+    rp_scatter8_seg<1024, 8><<<(unsigned)nblocks, 1024, 0, st>>>(keys, d_segs, nseg, offs, kout,
                                                                vout);
   }
   return launch_check("segmented partition scatter");
