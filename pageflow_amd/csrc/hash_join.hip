@@ -11,7 +11,9 @@
 // Work items are (bucket, probe chunk: 256K rows unique, 64K multi).  A
 // workgroup loads its bucket's build hashes into LDS (a chunk at a time when
 // the bucket is larger than the LDS table), builds an open-addressing table
-// over them (u32 slots, LDS CAS), and streams the probe chunk against it.
+// over them (16384 u16 slots, two to a word, LDS CAS: at most 5120 rows a
+// chunk keeps the load factor near 0.3 and the waves' longest probe chains
+// short), and streams the probe chunk against it.
 //   unique   each hash's lowest build row answers (a proven-unique build)
 //   multi    the build bucket is first sorted by (hash, row) in LDS
 //            (hj_sort_buckets), so a key's rows are one run; the table maps
@@ -43,9 +45,9 @@ constexpr int HJ_W = HJ_NT / kWave;
 constexpr int HJ_IT = 12;                  // write pass: probe rows a thread per round
 constexpr int HJ_ROUND = HJ_NT * HJ_IT;
 constexpr int CAP_U = 5120;                // unique: build rows an LDS chunk
-constexpr int TS_U = 8192;                 // unique: table slots
+constexpr int TS_U = 16384;                // unique: table slots (u16 each)
 constexpr int CAP_M = 4096;                // multi: build rows an LDS chunk
-constexpr int TS_M = 8192;
+constexpr int TS_M = 16384;
 constexpr int SORT_CAP = 8192;             // multi: largest build bucket sorted in LDS
 constexpr int SORT_NT = 512;
 // probe rows a work item: a multi-match item writes up to its rows times
@@ -219,9 +221,11 @@ struct Lds<true> {
 };
 
 // The LDS table's entry of hash h: its first row in the chunk, or -1.
-// Slots hold row + 1 (0 = empty), linear probing from the hash's low bits.
+// Slots are u16 holding row + 1 (0 = empty), two to a 32-bit word, linear
+// probing from the hash's low bits.
+static_assert(CAP_U < 65536 && CAP_M < 65536, "row + 1 fits a u16 slot");
 template <int TS>
-__device__ __forceinline__ int lds_find(const uint64_t* keys, const uint32_t* slot, uint64_t h) {
+__device__ __forceinline__ int lds_find(const uint64_t* keys, const uint16_t* slot, uint64_t h) {
   uint32_t s = (uint32_t)h & (TS - 1);
   uint32_t v = slot[s];
   while (v != 0u) {
@@ -230,6 +234,36 @@ __device__ __forceinline__ int lds_find(const uint64_t* keys, const uint32_t* sl
     v = slot[s];
   }
   return -1;
+}
+
+// Inserts row i (hash h) into the u16 table held in 32-bit words: CAS on the
+// slot's word; a slot holding the same hash keeps the lower row.
+template <int TS>
+__device__ __forceinline__ void lds_insert(const uint64_t* keys, uint32_t* word, uint64_t h, int i) {
+  const uint32_t me = (uint32_t)(i + 1);
+  uint32_t s = (uint32_t)h & (TS - 1);
+  for (;;) {
+    uint32_t* w = word + (s >> 1);
+    const int sh = (int)(s & 1u) * 16;
+    uint32_t old = atomicCAS(w, 0xffffffffu, 0xffffffffu);  // a read of the word (never all ones)
+    uint32_t cur = (old >> sh) & 0xffffu;
+    while (cur == 0u) {
+      const uint32_t prev = atomicCAS(w, old, old | (me << sh));
+      if (prev == old) return;
+      old = prev;
+      cur = (old >> sh) & 0xffffu;
+    }
+    if (keys[cur - 1] == h) {  // a repeated hash (unique mode): lowest row
+      while (cur > me) {
+        const uint32_t prev = atomicCAS(w, old, (old & ~(0xffffu << sh)) | (me << sh));
+        if (prev == old) return;
+        old = prev;
+        cur = (old >> sh) & 0xffffu;
+      }
+      return;
+    }
+    s = (s + 1) & (TS - 1);
+  }
 }
 
 // MODE 0: count pairs per item; MODE 1: write them.  Probe rows are taken
@@ -248,7 +282,8 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
   constexpr int IT = W1 ? HJ_IT : 8;
   constexpr int ROUND = HJ_NT * IT;
   __shared__ uint64_t keys[CAP];
-  __shared__ uint32_t slot[TS];
+  __shared__ uint32_t slotw[TS / 2];
+  const uint16_t* slot = reinterpret_cast<const uint16_t*>(slotw);
   __shared__ int32_t rows[W1 ? CAP : 1];
   __shared__ int64_t pay0[W1 ? CAP : 1];
   __shared__ uint16_t glen[MULTI ? CAP : 1];
@@ -265,7 +300,7 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
   for (int64_t cb = it.bs; cb < it.be; cb += CAP) {
     const int nloc = (int)(it.be - cb < CAP ? it.be - cb : CAP);
     __syncthreads();  // the previous chunk's lookups are done
-    for (int s = tid; s < TS; s += HJ_NT) slot[s] = 0;
+    for (int s = tid; s < TS / 2; s += HJ_NT) slotw[s] = 0;
     for (int i = tid; i < nloc; i += HJ_NT) {
       keys[i] = p.bh[cb + i];
       if (W1) rows[i] = p.brow[cb + i];
@@ -280,16 +315,7 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
         while (i + len < nloc && keys[i + len] == h) ++len;
         glen[i] = (uint16_t)(len > 65535 ? 65535 : len);
       }
-      uint32_t s = (uint32_t)h & (TS - 1);
-      for (;;) {
-        const uint32_t cur = atomicCAS(&slot[s], 0u, (uint32_t)(i + 1));
-        if (cur == 0) break;
-        if (keys[cur - 1] == h) {  // a repeated hash (unique mode): lowest row
-          atomicMin(&slot[s], (uint32_t)(i + 1));
-          break;
-        }
-        s = (s + 1) & (TS - 1);
-      }
+      lds_insert<TS>(keys, slotw, h, i);
     }
     __syncthreads();
 

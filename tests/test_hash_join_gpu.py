@@ -149,3 +149,22 @@ def test_plan_reuse():
         plan.write(op, ob)
         got = _sorted_pairs(op, ob)
         assert np.array_equal(got, oracle.join_pairs_sorted(bk.cpu().numpy(), pk.cpu().numpy()))
+
+
+@pytest.mark.parametrize("bits", [4, 8])
+def test_unique_path_on_repeated_build_keys_answers_each_keys_lowest_row(bits):
+    # The unique path's LDS table keeps the lower build row of a repeated hash
+    # (a CAS on the u16 slot's 32-bit word, racing with inserts into the
+    # word's other slot).  Every bucket here fits one LDS chunk, so each probe
+    # row of a present key pairs with exactly that key's first build row.
+    rng = np.random.default_rng(21 + bits)
+    bk = rng.integers(0, 3000, 20_000).astype(np.int64)
+    pk = rng.integers(-50, 3100, 300_000).astype(np.int64)
+    pi, bi, _ = hash_join(torch.as_tensor(bk).cuda(), torch.as_tensor(pk).cuda(), [],
+                          unique=True, bits=bits)
+    keys, first = np.unique(bk, return_index=True)
+    pos = np.searchsorted(keys, pk)
+    hit = (pos < keys.size) & (keys[np.minimum(pos, keys.size - 1)] == pk)
+    want = np.stack([first[pos[hit]], np.nonzero(hit)[0]], 1)
+    want = want[np.lexsort((want[:, 1], want[:, 0]))]
+    assert np.array_equal(_sorted_pairs(pi, bi), want)
