@@ -168,3 +168,16 @@ def test_unique_path_on_repeated_build_keys_answers_each_keys_lowest_row(bits):
     want = np.stack([first[pos[hit]], np.nonzero(hit)[0]], 1)
     want = want[np.lexsort((want[:, 1], want[:, 0]))]
     assert np.array_equal(_sorted_pairs(pi, bi), want)
+
+
+@pytest.mark.parametrize("nb,unique", [(5120, True), (5121, True), (10_240, True),
+                                       (4096, False), (4097, False)])
+def test_build_bucket_at_the_lds_chunk_capacity(nb, unique):
+    # One bucket (bits 0) of exactly an LDS chunk's rows (5120 unique, 4096
+    # multi), one row past it (a second chunk of one row) and two full chunks:
+    # the u16 table at its fullest, and the chunk boundary.
+    rng = np.random.default_rng(nb)
+    bk = (rng.permutation(nb).astype(np.int64) * 3 + 1 if unique
+          else rng.integers(0, nb // 2, nb).astype(np.int64))
+    pk = rng.integers(-10, 3 * nb + 10, 250_000).astype(np.int64)
+    _check(bk, pk, unique, bits=0)
