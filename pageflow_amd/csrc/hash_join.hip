@@ -408,8 +408,9 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
           const int32_t T = (int32_t)__shfl(inw[k] + m[k], kWave - 1, kWave);
           wave_lds_sync();  // a wave reads only its own lanes' entries
           const int64_t base = pos + cnt[k * HJ_W + w];
+          int lo0 = 0;  // the previous window's last source lane: sources only grow
           for (int j = lane; j < T; j += kWave) {
-            int lo = 0, hi = kWave;  // the last lane whose run starts at or before j
+            int lo = lo0, hi = kWave;  // the last lane whose run starts at or before j
             while (hi - lo > 1) {
               const int mid = (lo + hi) >> 1;
               if (s_ex[wb + mid] <= j)
@@ -423,6 +424,8 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
             p.out_build[o] = (int64_t)rows[li];
             if (has_pay) p.pay_out[0][o] = pay0[li];
             if (p.npay > 1) p.pay_out[1][o] = p.pay_src[1][cb + li];
+            // every lane is active here but in the last window, whose value is unused
+            lo0 = __shfl(lo, kWave - 1, kWave);
           }
           wave_lds_sync();  // s_* are rewritten for the next slot
         }
