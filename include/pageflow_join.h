@@ -114,6 +114,24 @@ int pf_join_unique_hash(const int64_t* b0, const int64_t* b1, int64_t nb, const 
                         int64_t* probe_idx, int64_t* host_out, void* workspace,
                         size_t workspace_bytes, void* stream);
 
+/* pf_join_unique_hash with no workspace and no allocation of any kind.  It has
+ * no counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The table, partner and scan scratch live in one
+ * fixed device array of PF_NOALLOC_SCRATCH_BYTES that each device holds from
+ * the first call on it until the process ends; a call whose
+ * pf_join_unique_hash_workspace_bytes(nb, np, nkeys) exceeds it fails with an
+ * error and launches nothing.  The caller still passes the inputs, the
+ * outputs (int64[np] each) and host_out, as for pf_join_unique_hash.  Every
+ * call on a device shares that device's array: calls that overlap on one
+ * device (other threads, other streams) race and must be serialised by the
+ * caller.  Each call initialises all the scratch it reads.  Waits once. */
+#define PF_NOALLOC_SCRATCH_BYTES (64u << 20)
+int pf_join_unique_hash_noalloc(const int64_t* b0, const int64_t* b1, int64_t nb,
+                                const int64_t* p0, const int64_t* p1, int64_t np,
+                                int32_t nkeys, int64_t* build_idx, int64_t* probe_idx,
+                                int64_t* host_out, void* stream);
+
 /* ------------------------------------------------------- semi / anti --
  * Replaces tqpkernels.join_semi_anti (operators/kernels/join_core.cu:358-400;
  * Python caller operators/join_core.py:138-139).

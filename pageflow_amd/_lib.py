@@ -38,6 +38,7 @@ _SIGNATURES = {
     "pf_partition_by_bucket": (ctypes.c_int, [PP, PP, P, I32, P, I64, I32, P, P, SZ, P]),
     "pf_join_unique_hash_workspace_bytes": (SZ, [I64, I64, I32]),
     "pf_join_unique_hash": (ctypes.c_int, [P, P, I64, P, P, I64, I32, P, P, P, P, SZ, P]),
+    "pf_join_unique_hash_noalloc": (ctypes.c_int, [P, P, I64, P, P, I64, I32, P, P, P, P]),
     "pf_join_semi_anti_workspace_bytes": (SZ, [I64, I64, I32]),
     "pf_join_semi_anti": (ctypes.c_int, [P, I64, P, I64, P, P, I32, I32, P, P, P, SZ, P]),
     "pf_semi_anti_mask_workspace_bytes": (SZ, [I64, I64, I32]),

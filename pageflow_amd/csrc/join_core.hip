@@ -20,6 +20,7 @@
 // device scan, ballot-ranked writes), so the order is deterministic; no CUB.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -733,21 +734,10 @@ int rf_validate(const RfProgram& P, int n_cols) {
   return 0;
 }
 
-}  // namespace
-}  // namespace pf
-
-extern "C" {
-
-size_t pf_join_unique_hash_workspace_bytes(int64_t nb, int64_t np, int32_t nkeys) {
-  return pf::carve_unique(nullptr, nb, np, nkeys == 2).bytes;
-}
-
-int pf_join_unique_hash(const int64_t* b0, const int64_t* b1, int64_t nb, const int64_t* p0,
-                        const int64_t* p1, int64_t np, int32_t nkeys, int64_t* build_idx,
-                        int64_t* probe_idx, int64_t* host_out, void* workspace,
-                        size_t workspace_bytes, void* stream) {
-  using namespace pf;
-  hipStream_t st = (hipStream_t)stream;
+int unique_hash_run(const int64_t* b0, const int64_t* b1, int64_t nb, const int64_t* p0,
+                    const int64_t* p1, int64_t np, int32_t nkeys, int64_t* build_idx,
+                    int64_t* probe_idx, int64_t* host_out, void* workspace,
+                    size_t workspace_bytes, hipStream_t st) {
   if (nkeys != 1 && nkeys != 2) return fail("join_core: one or two aligned build/probe keys");
   if (nb <= 0 || np <= 0) return fail("join_core: unique hash needs rows on both sides");
   if (nb > MAX_ROWS || np > MAX_ROWS)
@@ -786,6 +776,72 @@ int pf_join_unique_hash(const int64_t* b0, const int64_t* b1, int64_t nb, const 
   if (launch_check("join_unique_hash")) return 1;
   if (wait_stream(st, "join_unique_hash")) return 1;
   return ok();
+}
+
+// The allocation-free unique-hash call's scratch: one fixed array a device
+// holds from its first use until the process ends.  Host code never passes
+// `scratch` itself to a kernel (on the host it names a stand-in, not device
+// memory): each device's address comes from hipGetSymbolAddress, taken once
+// per device with that device current.
+__device__ __align__(256) unsigned char scratch[PF_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`.
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  if (hip_check(hipGetDevice(&dev), "join_unique_hash_noalloc")) return 1;
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail("join_unique_hash_noalloc: device " + std::to_string(dev) + " out of range");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                  "join_unique_hash_noalloc: scratch address"))
+      return 1;
+    if (!p) return fail("join_unique_hash_noalloc: no scratch address on this device");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+}  // namespace pf
+
+extern "C" {
+
+size_t pf_join_unique_hash_workspace_bytes(int64_t nb, int64_t np, int32_t nkeys) {
+  return pf::carve_unique(nullptr, nb, np, nkeys == 2).bytes;
+}
+
+int pf_join_unique_hash(const int64_t* b0, const int64_t* b1, int64_t nb, const int64_t* p0,
+                        const int64_t* p1, int64_t np, int32_t nkeys, int64_t* build_idx,
+                        int64_t* probe_idx, int64_t* host_out, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+  return pf::unique_hash_run(b0, b1, nb, p0, p1, np, nkeys, build_idx, probe_idx, host_out,
+                             workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int pf_join_unique_hash_noalloc(const int64_t* b0, const int64_t* b1, int64_t nb,
+                                const int64_t* p0, const int64_t* p1, int64_t np,
+                                int32_t nkeys, int64_t* build_idx, int64_t* probe_idx,
+                                int64_t* host_out, void* stream) {
+  using namespace pf;
+  if (nb > 0 && np > 0 && (nkeys == 1 || nkeys == 2)) {
+    const size_t need = carve_unique(nullptr, nb, np, nkeys == 2).bytes;
+    if (need > (size_t)PF_NOALLOC_SCRATCH_BYTES)
+      return fail("join_unique_hash_noalloc: " + std::to_string(nb) + " build and " +
+                  std::to_string(np) + " probe rows need " + std::to_string(need) +
+                  " bytes of scratch, over the fixed " +
+                  std::to_string((size_t)PF_NOALLOC_SCRATCH_BYTES));
+  }
+  void* ws = nullptr;
+  if (noalloc_scratch(&ws)) return 1;
+  return unique_hash_run(b0, b1, nb, p0, p1, np, nkeys, build_idx, probe_idx, host_out, ws,
+                         (size_t)PF_NOALLOC_SCRATCH_BYTES, (hipStream_t)stream);
 }
 
 size_t pf_join_semi_anti_workspace_bytes(int64_t nb, int64_t np, int32_t op) {
