@@ -175,8 +175,8 @@ int pf_join_expand_write(const int64_t* sorted_ids, const int64_t* start, int64_
  * (pf_partition_sort, B <= 512, padded, int64 keys) generalised for the
  * batched and large-join tiers to 2^12 buckets, int32 or int64 keys and
  * int32 row ids, unpadded.  Stable: rows keep their input order in a bucket.
- * keys: n keys of key_bytes (8 or 4); vals: int32 row ids (NULL: the row
- * index).  mode 0: digit = the `bits` bits of mix64(key) below its top
+ * keys: n < 2^31 keys of key_bytes (8 or 4); vals: int32 row ids (NULL: the
+ * row index).  mode 0: digit = the `bits` bits of mix64(key) below its top
  * `shift` bits, keys_out gets the keys; mode 1: the same digit, keys_out
  * gets mix64(key) as int64; mode 2: digit = bits [shift, shift + bits) of
  * the unsigned key.  bucket_offsets: device int64[2^bits + 1], each bucket's

@@ -19,6 +19,12 @@ Parity pinning (what checks each function besides the formula itself):
 * ``partition``: the reference test's stable-sort definition
   (``tests/operators/test_partition_sort.py:106-118``) and the padded layout
   invariants (``:93-103``), plus brute force per bucket.
+* ``radix_partition``: each row's digit computed with Python ints from the
+  golden hashes (hashed mode, every shift) or from the key read as an
+  unsigned integer of its width (raw mode, int32 and int64), and the buckets
+  filled by a brute-force scan in input order; at shift 0 it is the
+  reference's stable sort by ``bucket_of``
+  (``tests/operators/test_partition_sort.py:106-118``).
 * ``unique_hash_pairs``: the known answers of
   ``tests/operators/test_unique_hash_join.py:86-151`` and a brute-force
   dense comparison.

@@ -661,7 +661,10 @@ int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_
                     int who, int shape) {
   if (key_bytes != 8 && key_bytes != 4) return fail("radix_partition: keys are int64 or int32");
   if (bits < 0 || bits > 12) return fail("radix_partition: bits must be in [0, 12]");
-  if (n < 0 || n >= (1LL << 32)) return fail("radix_partition: n out of range");
+  // row ids are int32: the iota ids of a call without vals, and the
+  // partitioned ids of every caller, stay below 2^31
+  if (n < 0 || n >= (1LL << 31))
+    return fail("radix_partition: n must be below 2^31 (row ids are int32)");
   if (mode < RP_HASH_KEY || mode > RP_RAW) return fail("radix_partition: unknown mode");
   const int kbits = mode == RP_RAW ? key_bytes * 8 : 64;
   if (shift < 0 || shift + bits > kbits) return fail("radix_partition: shift + bits too large");

@@ -11,6 +11,8 @@ from . import _lib
 
 HASH_KEY, HASH_OUT, RAW = 0, 1, 2
 MAX_BITS = 12
+# Row ids are int32 (the iota ids written when ``vals`` is None included).
+MAX_ROWS = (1 << 31) - 1
 
 
 def workspace_bytes(n: int, bits: int) -> int:
@@ -32,6 +34,8 @@ def radix_partition(keys: torch.Tensor, bits: int, vals: Optional[torch.Tensor] 
         raise ValueError(f"radix_partition: bits must be in [0, {MAX_BITS}]")
     keys = keys.contiguous()
     n = keys.numel()
+    if n > MAX_ROWS:
+        raise ValueError(f"radix_partition: {n} rows; row ids are int32, so at most {MAX_ROWS}")
     dev = keys.device
     if vals is not None:
         if vals.dtype != torch.int32 or vals.numel() != n:

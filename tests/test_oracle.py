@@ -59,6 +59,84 @@ def test_buckets_spread_a_linear_hash():
     assert counts.min() > 0.75 * 4096 and counts.max() < 1.25 * 4096
 
 
+# ------------------------------------------------------- radix partition --
+#
+# oracle.radix_partition is the checker of the library's radix partition.  It
+# is pinned here by an independent computation: the digit of each row from
+# Python ints (the hash taken from the golden vectors, not recomputed), and
+# the bucket contents by a brute-force scan in input order, which is the
+# reference's stable-sort-by-bucket definition of a partition
+# (tests/operators/test_partition_sort.py:106-118, buckets from
+# tests/mix64.py:30-36).
+
+def _golden():
+    d = np.load(os.path.join(HERE, "golden", "mix64_golden.npz"))
+    keys = d["keys"].astype(np.int64)
+    hashes = [int(h) for h in d["mix64"].astype(np.uint64)]
+    return keys, hashes
+
+
+def _brute_partition(keys, vals, digits, nbuckets):
+    """Bucket by bucket, the rows with that digit in input order."""
+    out_k, out_v, counts = [], [], []
+    for b in range(nbuckets):
+        rows = [i for i in range(len(digits)) if digits[i] == b]
+        out_k += [keys[i] for i in rows]
+        out_v += [vals[i] for i in rows]
+        counts.append(len(rows))
+    return out_k, out_v, counts
+
+
+@pytest.mark.parametrize("bits,shift", [(0, 0), (1, 0), (4, 0), (8, 0), (12, 0), (3, 5),
+                                        (6, 6), (8, 4), (4, 60), (1, 63), (12, 52)])
+def test_radix_partition_hashed_digits_by_python_ints(bits, shift):
+    keys, hashes = _golden()
+    rng = np.random.default_rng(bits * 64 + shift)
+    order = rng.permutation(len(keys))
+    keys = np.concatenate([keys[order], keys[order[:300]]])     # repeated keys too
+    hashes = [hashes[i] for i in order] + [hashes[i] for i in order[:300]]
+    vals = np.arange(len(keys), dtype=np.int32)[::-1].copy()
+    # the digit: the `bits` bits of the hash just below its top `shift` bits
+    digits = [(h >> (64 - shift - bits)) & ((1 << bits) - 1) for h in hashes]
+    want_k, want_v, want_c = _brute_partition(keys.tolist(), vals.tolist(), digits, 1 << bits)
+    got_k, got_v, got_c = oracle.radix_partition(keys, vals, bits, shift, hashed=True)
+    assert got_k.tolist() == want_k and got_v.tolist() == want_v
+    assert got_c.tolist() == want_c
+
+
+@pytest.mark.parametrize("bits", [1, 4, 8, 9])
+def test_radix_partition_at_shift_zero_is_the_reference_bucket_sort(bits):
+    # bucket_of is pinned by the golden vectors; at shift 0 the partition is
+    # the reference's stable sort by it
+    keys, _ = _golden()
+    keys = np.concatenate([keys, keys[:123]])
+    vals = np.arange(len(keys), dtype=np.int32)
+    buckets = oracle.bucket_of(keys, 1 << bits)
+    order = np.argsort(buckets, kind="stable")
+    got_k, got_v, got_c = oracle.radix_partition(keys, vals, bits, 0, hashed=True)
+    assert np.array_equal(got_k, keys[order]) and np.array_equal(got_v, vals[order])
+    assert np.array_equal(got_c, np.bincount(buckets, minlength=1 << bits))
+
+
+@pytest.mark.parametrize("dtype", [np.int64, np.int32])
+@pytest.mark.parametrize("bits,shift", [(1, 0), (8, 0), (8, 8), (4, 28), (12, 20), (8, 24),
+                                        (8, 28), (12, 26)])
+def test_radix_partition_raw_digits_by_python_ints(dtype, bits, shift):
+    rng = np.random.default_rng(bits + 100 * shift)
+    info = np.iinfo(dtype)
+    keys = rng.integers(info.min, info.max, 3000, dtype=dtype, endpoint=True)
+    keys[:6] = [info.min, info.max, -1, 0, 1, -2]
+    vals = rng.permutation(len(keys)).astype(np.int32)
+    width = 64 if dtype == np.int64 else 32
+    # raw mode reads the key as an unsigned integer of its own width
+    digits = [((int(k) % (1 << width)) >> shift) & ((1 << bits) - 1) for k in keys]
+    want_k, want_v, want_c = _brute_partition(keys.tolist(), vals.tolist(), digits, 1 << bits)
+    got_k, got_v, got_c = oracle.radix_partition(keys, vals, bits, shift, hashed=False)
+    assert got_k.dtype == dtype
+    assert got_k.tolist() == want_k and got_v.tolist() == want_v
+    assert got_c.tolist() == want_c
+
+
 # ------------------------------------------------------------ unique hash --
 
 def _first_slots(keys, cap):
