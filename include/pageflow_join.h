@@ -58,6 +58,26 @@ int pf_mix64(const int64_t* keys, int64_t n, int64_t* out, void* stream);
  * Replaces operators/aggregate.py:107-116 (_bucket_ids). */
 int pf_bucket_ids(const int64_t* keys, int64_t n, int32_t B, int32_t* out, void* stream);
 
+/* A join key column of any scalar type as int64 join keys, for the pair path
+ * (replaces the dtype handling the reference's sort/search core inherits
+ * from torch.sort/searchsorted, operators/hashjoin.py:408-421).  key_type:
+ * PF_KEY_*.  Integer keys become their int64 value.  Float keys, and
+ * integer keys when as_float is set (an integer side joined against a float
+ * side compares in float64), become the float64 bit pattern with -0.0
+ * folded into +0.0, so equal values have equal images; NaN, which equals
+ * nothing, becomes a NaN pattern of its own per side (side 0 build, 1
+ * probe), so NaNs never match.  No host wait. */
+#define PF_KEY_I64 0
+#define PF_KEY_I32 1
+#define PF_KEY_I16 2
+#define PF_KEY_I8 3
+#define PF_KEY_U8 4
+#define PF_KEY_F64 5
+#define PF_KEY_F32 6
+#define PF_KEY_BOOL 7
+int pf_join_key_bits(const void* keys, int32_t key_type, int64_t n, int32_t as_float,
+                     int32_t side, int64_t* out, void* stream);
+
 /* ---------------------------------------------------- partition_sort --
  * Replaces tqpkernels.partition_sort (operators/kernels/partition_sort.cu:610-613;
  * its Python caller operators/partition_sort.py:125-127).
@@ -231,11 +251,14 @@ int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int6
  * Replaces the payload gather of operators/hashjoin.py:359-377
  * (torch.index_select per output column) and conversion.py:39-50
  * (index_with_null): out[i] = src[idx[i]] for rows of row_elems elements of
- * elem_bytes (1, 2, 4, 8); where idx[i] is the null sentinel (-64), or src
- * has no row, every element of the row is -64.  An index outside [0, nsrc)
- * also writes the null row and sets *bad (optional, device).  No wait. */
+ * elem_bytes (1, 2, 4, 8), copied as bits, so any element type gathers;
+ * where idx[i] is the null sentinel (-64), or src has no row, every element
+ * of the row is null_bits (its low elem_bytes bytes): the bits of -64 in the
+ * column's type, -64.0 for a float column.  An index outside [0, nsrc) also
+ * writes the null row and sets *bad (optional, device).  No wait. */
 int pf_gather_rows(const void* src, int64_t nsrc, int32_t elem_bytes, int64_t row_elems,
-                   const int64_t* idx, int64_t n, void* out, int64_t* bad, void* stream);
+                   const int64_t* idx, int64_t n, void* out, int64_t* bad, int64_t null_bits,
+                   void* stream);
 
 /* The join-type tails (operators/hashjoin.py:293-352, the bool scatter of
  * matched rows and its nonzero): pf_mark_rows sets flags[idx[i]] = value;

@@ -31,6 +31,7 @@ _SIGNATURES = {
     "pf_profile_read": (I32, [ctypes.c_char_p, I64, P, P, I32]),
     "pf_mix64": (ctypes.c_int, [P, I64, P, P]),
     "pf_bucket_ids": (ctypes.c_int, [P, I64, I32, P, P]),
+    "pf_join_key_bits": (ctypes.c_int, [P, I32, I64, I32, I32, P, P]),
     "pf_partition_sort_workspace_bytes": (SZ, [I64, I64, I32, I32, I32, I32]),
     "pf_partition_sort": (ctypes.c_int, [PP, PP, P, P, I32, I32, P, I64, I64, I64, I32, I32,
                                          P, P, P, SZ, P]),
@@ -50,7 +51,7 @@ _SIGNATURES = {
     "pf_radix_partition": (ctypes.c_int, [P, I32, P, I64, I32, I32, I32, P, P, P, P, SZ, P]),
     "pf_radix_partition_shape": (ctypes.c_int, [P, I32, P, I64, I32, I32, I32, P, P, P, P, SZ,
                                                 I32, P]),
-    "pf_gather_rows": (ctypes.c_int, [P, I64, I32, I64, P, I64, P, P, P]),
+    "pf_gather_rows": (ctypes.c_int, [P, I64, I32, I64, P, I64, P, P, I64, P]),
     "pf_mark_rows": (ctypes.c_int, [P, I64, P, I64, I32, P]),
     "pf_select_rows_workspace_bytes": (SZ, [I64]),
     "pf_select_rows": (ctypes.c_int, [P, I64, I32, P, P, P, SZ, P]),

@@ -274,6 +274,40 @@ __global__ void k_bucket_ids(const int64_t* __restrict__ in, int64_t n, int logB
     out[i] = logB ? (int32_t)(mix64((uint64_t)in[i]) >> (64 - logB)) : 0;
 }
 
+// Source types of pf_join_key_bits (pageflow_join.h PF_KEY_*).
+enum KeyType : int { KT_I64 = 0, KT_I32 = 1, KT_I16 = 2, KT_I8 = 3, KT_U8 = 4, KT_F64 = 5,
+                     KT_F32 = 6, KT_BOOL = 7 };
+
+template <typename T>
+__device__ __forceinline__ double key_as_f64(T v) { return (double)v; }
+
+// The int64 image of a join key under which two keys are equal exactly when
+// the reference's comparison finds them equal.  Integers: the value itself.
+// Floats (and integers joined against floats, compared as float64): the bit
+// pattern of the value with -0.0 folded into +0.0; a NaN, equal to nothing,
+// becomes one NaN pattern per side, so the two sides' NaNs never meet.
+template <typename T>
+__global__ void k_key_bits(const T* __restrict__ in, int64_t n, bool as_float,
+                           int64_t nan_bits, int64_t* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    const T v = in[i];
+    if (!as_float) {
+      out[i] = (int64_t)v;
+      continue;
+    }
+    double d = key_as_f64(v);
+    int64_t b;
+    if (d != d) {
+      b = nan_bits;
+    } else {
+      if (d == 0.0) d = 0.0;  // -0.0 -> +0.0
+      b = __double_as_longlong(d);
+    }
+    out[i] = b;
+  }
+}
+
 unsigned stream_grid(int64_t n) {
   int64_t g = (n + 255) / 256;
   if (g > 16384) g = 16384;
@@ -323,6 +357,32 @@ int pf_mix64(const int64_t* keys, int64_t n, int64_t* out, void* stream) {
   pf::k_mix64<<<pf::stream_grid(n), 256, 0, (hipStream_t)stream>>>(keys, n, out);
   if (pf::launch_check("pf_mix64")) return 1;
   return pf::ok();
+}
+
+int pf_join_key_bits(const void* keys, int32_t key_type, int64_t n, int32_t as_float,
+                     int32_t side, int64_t* out, void* stream) {
+  using namespace pf;
+  if (n < 0) return fail("join_key_bits: negative row count");
+  if (side != 0 && side != 1) return fail("join_key_bits: side is 0 (build) or 1 (probe)");
+  const bool fl = as_float != 0 || key_type == KT_F64 || key_type == KT_F32;
+  // quiet NaN patterns no canonical value takes: payload 1 (build), 2 (probe)
+  const int64_t nan_bits = (int64_t)0x7ff8000000000001ULL + side;
+  if (n == 0) return ok();
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned g = stream_grid(n);
+  switch (key_type) {
+    case KT_I64: k_key_bits<int64_t><<<g, 256, 0, st>>>((const int64_t*)keys, n, fl, nan_bits, out); break;
+    case KT_I32: k_key_bits<int32_t><<<g, 256, 0, st>>>((const int32_t*)keys, n, fl, nan_bits, out); break;
+    case KT_I16: k_key_bits<int16_t><<<g, 256, 0, st>>>((const int16_t*)keys, n, fl, nan_bits, out); break;
+    case KT_I8: k_key_bits<int8_t><<<g, 256, 0, st>>>((const int8_t*)keys, n, fl, nan_bits, out); break;
+    case KT_U8:
+    case KT_BOOL: k_key_bits<uint8_t><<<g, 256, 0, st>>>((const uint8_t*)keys, n, fl, nan_bits, out); break;
+    case KT_F64: k_key_bits<double><<<g, 256, 0, st>>>((const double*)keys, n, true, nan_bits, out); break;
+    case KT_F32: k_key_bits<float><<<g, 256, 0, st>>>((const float*)keys, n, true, nan_bits, out); break;
+    default: return fail("join_key_bits: unknown key type " + std::to_string(key_type));
+  }
+  if (launch_check("pf_join_key_bits")) return 1;
+  return ok();
 }
 
 int pf_bucket_ids(const int64_t* keys, int64_t n, int32_t B, int32_t* out, void* stream) {

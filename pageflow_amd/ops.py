@@ -21,25 +21,65 @@ def _host_buffer() -> torch.Tensor:
     return _host
 
 
-def gather_rows(src: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
+def _null_bits(dtype: torch.dtype) -> int:
+    """The bits of the null value -64 in ``dtype`` (conversion.py:39-50
+    fills a null row with -64 in the column's own type), as an int64."""
+    return int(torch.tensor([NULL], dtype=dtype).view(
+        {1: torch.int8, 2: torch.int16, 4: torch.int32, 8: torch.int64}[
+            torch.tensor([], dtype=dtype).element_size()])[0])
+
+
+def gather_rows(src: torch.Tensor, idx: torch.Tensor, check: bool = False) -> torch.Tensor:
     """``src``'s rows at ``idx`` (int64, 1-D), a null row (every element
-    -64) where ``idx`` is -64 or ``src`` has no row: index_select and
-    index_with_null (conversion.py:39-50) in one kernel."""
+    -64 in ``src``'s type) where ``idx`` is -64 or ``src`` has no row:
+    index_select and index_with_null (conversion.py:39-50) in one kernel.
+    Rows are copied as bits, so any 1/2/4/8-byte element type works.  With
+    ``check``, an index outside the rows (other than -64) raises
+    RuntimeError after the gather (one wait)."""
     _lib.require_cuda(src, "gather source")
     _lib.require_cuda(idx, "gather index")
     if src.element_size() not in (1, 2, 4, 8) or src.dim() not in (1, 2):
         raise ValueError("gather_rows: 1-D or 2-D rows of 1/2/4/8-byte elements")
-    if src.is_floating_point() or src.dtype == torch.bool:
-        raise ValueError("gather_rows: integer columns (the join path is integer)")
+    if src.dtype == torch.bool:
+        raise ValueError("gather_rows: a bool column has no null value")
     src = src.contiguous()
     idx = idx.contiguous().to(torch.int64)
     row_elems = src.shape[1] if src.dim() == 2 else 1
     out = torch.empty((idx.numel(),) + tuple(src.shape[1:]), dtype=src.dtype, device=src.device)
     lib = _lib.load()
+    bad = torch.zeros(1, dtype=torch.int64, device=src.device) if check else None
     with torch.cuda.device(src.device):
         _lib.check(lib.pf_gather_rows(_lib.ptr(src), src.shape[0], src.element_size(), row_elems,
-                                      _lib.ptr(idx), idx.numel(), _lib.ptr(out), None,
-                                      _lib.stream_of(src.device)))
+                                      _lib.ptr(idx), idx.numel(), _lib.ptr(out), _lib.ptr(bad),
+                                      _null_bits(src.dtype), _lib.stream_of(src.device)))
+    if check and int(bad.item()):
+        raise RuntimeError(f"gather_rows: an index lies outside the {src.shape[0]} rows")
+    return out
+
+
+# pageflow_join.h PF_KEY_*
+_KEY_TYPE = {torch.int64: 0, torch.int32: 1, torch.int16: 2, torch.int8: 3, torch.uint8: 4,
+             torch.float64: 5, torch.float32: 6, torch.bool: 7}
+
+
+def join_key_bits(key: torch.Tensor, side: int, as_float: bool = False) -> torch.Tensor:
+    """A key column of any scalar type as int64 join keys
+    (``pf_join_key_bits``): integers by value; floats, and integers when
+    ``as_float`` (joined against a float side), by float64 bit pattern with
+    -0.0 folded into +0.0 and NaN made unmatched (side 0 build, 1 probe)."""
+    _lib.require_cuda(key, "a join key")
+    if key.dtype not in _KEY_TYPE:
+        raise ValueError(f"join key of {key.dtype}: keys are integer, bool or float")
+    if key.dim() == 2 and key.shape[1] == 1:
+        key = key.reshape(-1)
+    if key.dim() != 1:
+        raise ValueError(f"a join key of shape {tuple(key.shape)}: keys are 1-D")
+    key = key.contiguous()
+    out = torch.empty(key.numel(), dtype=torch.int64, device=key.device)
+    with torch.cuda.device(key.device):
+        _lib.check(_lib.load().pf_join_key_bits(_lib.ptr(key), _KEY_TYPE[key.dtype], key.numel(),
+                                                int(bool(as_float)), int(side), _lib.ptr(out),
+                                                _lib.stream_of(key.device)))
     return out
 
 

@@ -88,17 +88,19 @@ def partition_sort(cols: Sequence[torch.Tensor], comp: Sequence[int], key_idx: i
     return outs
 
 
-def join_expand(sorted_ids: torch.Tensor, start: torch.Tensor, length: torch.Tensor,
+def join_expand(sorted_ids: torch.Tensor, start: torch.Tensor, len: torch.Tensor,
                 host_out: torch.Tensor) -> List[torch.Tensor]:
     """[build rows, probe rows]: (sorted_ids[start[i] + j], i) for every
     probe row i and j < len[i], probe-row then run order; host_out[0] the
-    pair count."""
+    pair count.  The third parameter is named ``len`` as in the reference's
+    binding (tqpkernels.cu:29-34), so keyword callers keep working."""
+    runs = len  # the run lengths; the builtin is not used below
     device = sorted_ids.device
-    for t, what in ((sorted_ids, "sorted_ids"), (start, "start"), (length, "len")):
+    for t, what in ((sorted_ids, "sorted_ids"), (start, "start"), (runs, "len")):
         _i64_col(t, what, device)
-    if start.numel() != length.numel():
+    if start.numel() != runs.numel():
         raise RuntimeError("join_expand: start and len differ in length")
-    m = length.numel()
+    m = runs.numel()
     if m == 0:
         host_out[0] = 0
         return [torch.empty(0, dtype=torch.int64, device=device) for _ in range(2)]
@@ -107,7 +109,7 @@ def join_expand(sorted_ids: torch.Tensor, start: torch.Tensor, length: torch.Ten
     ws = _lib.workspace(ws_bytes, device)
     st = _lib.stream_of(device)
     with torch.cuda.device(device):
-        _lib.check(lib.pf_join_expand_count(_lib.ptr(length), m, _hp(host_out), _lib.ptr(ws),
+        _lib.check(lib.pf_join_expand_count(_lib.ptr(runs), m, _hp(host_out), _lib.ptr(ws),
                                             ws_bytes, st))
         n_flat = int(host_out[0])
         cs = torch.empty(n_flat, dtype=torch.int64, device=device)

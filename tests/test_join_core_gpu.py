@@ -19,13 +19,13 @@ I64_MIN = -(1 << 63)
 I64_MAX = (1 << 63) - 1
 
 
-def _cuda(values):
-    return torch.tensor(values, dtype=torch.int64, device="cuda")
+def _dev(values):
+    return torch.as_tensor(np.asarray(values, dtype=np.int64)).cuda()
 
 
-def _rand(n, lo, hi, seed):
-    g = torch.Generator().manual_seed(seed)
-    return torch.randint(lo, hi, (n,), generator=g, dtype=torch.int64).cuda()
+def _ints(n, lo, hi, seed):
+    """n seeded int64 values in [lo, hi) on the GPU."""
+    return _dev(np.random.default_rng(seed).integers(lo, hi, n, dtype=np.int64))
 
 
 def _np(t):
@@ -37,10 +37,10 @@ def _np(t):
 @pytest.mark.parametrize("lengths", [[], [0], [0, 0, 0], [1], [3], [1, 0, 2, 0, 0, 3, 0],
                                      [0, 0, 5], [5, 0, 0], [2, 2, 2, 2]])
 def test_expand_hand_cases(lengths):
-    length = _cuda(lengths)
+    length = _dev(lengths)
     nb = max(8, sum(lengths) + 3)
     sorted_ids = torch.randperm(nb, device="cuda")
-    start = _cuda([(3 * i) % (nb - n) if nb > n else 0 for i, n in enumerate(lengths)])
+    start = _dev([(3 * i) % (nb - n) if nb > n else 0 for i, n in enumerate(lengths)])
     cs, cr = join_core.expand(sorted_ids, start, length)
     rcs, rcr = oracle.expand(_np(sorted_ids), _np(start), _np(length))
     assert np.array_equal(_np(cs), rcs) and np.array_equal(_np(cr), rcr)
@@ -48,9 +48,9 @@ def test_expand_hand_cases(lengths):
 
 @pytest.mark.parametrize("n_probe,n_build", [(60_000, 200_000), (3_000_000, 1_000_000)])
 def test_expand_random_runs(n_probe, n_build):
-    length = _rand(n_probe, 0, 7, 1)
-    length[_rand(n_probe, 0, 10, 2) < 4] = 0
-    start = torch.minimum(_rand(n_probe, 0, n_build, 3), n_build - length)
+    length = _ints(n_probe, 0, 7, 1)
+    length[_ints(n_probe, 0, 10, 2) < 4] = 0
+    start = torch.minimum(_ints(n_probe, 0, n_build, 3), n_build - length)
     sorted_ids = torch.randperm(n_build, device="cuda")
     cs, cr = join_core.expand(sorted_ids, start, length)
     rcs, rcr = oracle.expand(_np(sorted_ids), _np(start), _np(length))
@@ -59,8 +59,8 @@ def test_expand_random_runs(n_probe, n_build):
 
 def test_expand_one_long_run():
     # one probe row with a 100K-pair run between empty ones (skew)
-    length = _cuda([0, 3, 100_000, 0, 1])
-    start = _cuda([0, 5, 0, 0, 7])
+    length = _dev([0, 3, 100_000, 0, 1])
+    start = _dev([0, 5, 0, 0, 7])
     sorted_ids = torch.randperm(100_005, device="cuda")
     cs, cr = join_core.expand(sorted_ids, start, length)
     rcs, rcr = oracle.expand(_np(sorted_ids), _np(start), _np(length))
@@ -75,10 +75,10 @@ def test_expand_one_long_run():
                                              (40_000, 3000)])
 def test_semi_anti_matches_oracle(anti, op, n_build, n_probe):
     seed = n_build * 7 + n_probe
-    bk = _rand(n_build, -50, max(n_build // 3, 1), seed)
-    pk = _rand(n_probe, -60, 2 * max(n_build // 3, 1), seed + 1)
-    bv = _rand(n_build, 0, 4, seed + 2)
-    pv = _rand(n_probe, 0, 4, seed + 3)
+    bk = _ints(n_build, -50, max(n_build // 3, 1), seed)
+    pk = _ints(n_probe, -60, 2 * max(n_build // 3, 1), seed + 1)
+    bv = _ints(n_build, 0, 4, seed + 2)
+    pv = _ints(n_probe, 0, 4, seed + 3)
     summary = (bv, pv, op) if op is not None else None
     got = join_core.semi_anti(bk, pk, anti, summary)
     want = oracle.semi_anti(_np(bk), _np(pk), anti,
@@ -90,26 +90,26 @@ def test_semi_anti_matches_oracle(anti, op, n_build, n_probe):
 
 @pytest.mark.parametrize("anti", [False, True])
 def test_summary_all_true_and_all_false(anti):
-    bk = _cuda([1, 1, 1, 2, 2, 3])
-    pk = _cuda([1, 2, 3, 4, 1])
-    same = _cuda([9] * 6)
-    got = join_core.semi_anti(bk, pk, anti, (same, _cuda([9] * 5), '<>'))
+    bk = _dev([1, 1, 1, 2, 2, 3])
+    pk = _dev([1, 2, 3, 4, 1])
+    same = _dev([9] * 6)
+    got = join_core.semi_anti(bk, pk, anti, (same, _dev([9] * 5), '<>'))
     assert got.tolist() == ([0, 1, 2, 3, 4] if anti else [])
-    got = join_core.semi_anti(bk, pk, anti, (same, _cuda([5] * 5), '<>'))
+    got = join_core.semi_anti(bk, pk, anti, (same, _dev([5] * 5), '<>'))
     assert got.tolist() == ([3] if anti else [0, 1, 2, 4])
 
 
 def test_semi_anti_extreme_keys_and_null_sentinel():
     keys = [I64_MIN + 1, -1, 0, -64, I64_MAX, 42]
-    bk = _cuda(keys[:4])
-    pk = _cuda(keys + [I64_MIN])
+    bk = _dev(keys[:4])
+    pk = _dev(keys + [I64_MIN])
     assert join_core.semi_anti(bk, pk, False).tolist() == [0, 1, 2, 3]
     assert join_core.semi_anti(bk, pk, True).tolist() == [4, 5, 6]
 
 
 def test_semi_anti_refuses_the_empty_slot_key():
-    assert join_core.semi_anti(_cuda([3, I64_MIN]), _cuda([3]), False) is None
-    assert join_core.semi_anti_mask(_cuda([3, I64_MIN]), _cuda([3]), False) is None
+    assert join_core.semi_anti(_dev([3, I64_MIN]), _dev([3]), False) is None
+    assert join_core.semi_anti_mask(_dev([3, I64_MIN]), _dev([3]), False) is None
 
 
 def _one_bucket_keys(n, B=128, bucket=5, seed=0):
@@ -136,8 +136,8 @@ def test_unique_hash_random_tuples(two):
     n_build, n_probe = 30_000, 50_000
     k0 = torch.randperm(n_build * 4, device="cuda")[:n_build]
     build = [k0 // 3, k0 % 3] if two else [k0]
-    picks = _rand(n_probe, 0, n_build, 11)
-    miss = _rand(n_probe, 0, 4, 12) == 0
+    picks = _ints(n_probe, 0, n_build, 11)
+    miss = _ints(n_probe, 0, 4, 12) == 0
     p0 = torch.where(miss, k0[picks] + n_build * 8, k0[picks])
     probe = [p0 // 3, p0 % 3] if two else [p0]
     build_idx, probe_idx = unique_hash_pairs(build, probe)
@@ -163,8 +163,8 @@ def _colliding_keys(cap, n):
 
 def test_single_key_pairs_are_stable_and_collision_safe():
     coll = _colliding_keys(12, 6)
-    build = _cuda(coll)
-    probe = _cuda([coll[5], -999, coll[0], coll[3], coll[0]])
+    build = _dev(coll)
+    probe = _dev([coll[5], -999, coll[0], coll[3], coll[0]])
     build_idx, probe_idx = unique_hash_pairs([build], [probe])
     assert probe_idx.tolist() == [0, 2, 3, 4]
     assert build_idx.tolist() == [5, 0, 3, 0]
@@ -174,8 +174,8 @@ def test_single_key_pairs_are_stable_and_collision_safe():
 
 def test_two_column_tuple_comparison_and_repeated_probe_keys():
     build_idx, probe_idx = unique_hash_pairs(
-        [_cuda([1, 1, 2, 2]), _cuda([7, 8, 7, 8])],
-        [_cuda([2, 1, 2, 3, 1, 1]), _cuda([8, 8, 9, 7, 7, 7])])
+        [_dev([1, 1, 2, 2]), _dev([7, 8, 7, 8])],
+        [_dev([2, 1, 2, 3, 1, 1]), _dev([8, 8, 9, 7, 7, 7])])
     assert probe_idx.tolist() == [0, 1, 4, 5]
     assert build_idx.tolist() == [3, 1, 0, 0]
 
@@ -184,22 +184,22 @@ def test_two_column_tuple_comparison_and_repeated_probe_keys():
     ([], [1, 2], [], []), ([1, 2], [], [], []), ([1, 2], [3, 4], [], []),
     ([1, 2, 3], [3, 2, 1], [2, 1, 0], [0, 1, 2])])
 def test_empty_no_match_and_all_match_edges(b, p, eb, ep):
-    build_idx, probe_idx = unique_hash_pairs([_cuda(b)], [_cuda(p)])
+    build_idx, probe_idx = unique_hash_pairs([_dev(b)], [_dev(p)])
     assert build_idx.tolist() == eb and probe_idx.tolist() == ep
 
 
 def test_capacity_boundaries_match_a_reference_lookup():
     for nb in (7, 8, 9, 15, 16, 17, 31, 32, 33):
         build = torch.arange(nb, dtype=torch.int64, device="cuda") * 17 + 3
-        probe = torch.cat((build.flip(0), _cuda([-1]), build[:3]))
+        probe = torch.cat((build.flip(0), _dev([-1]), build[:3]))
         build_idx, probe_idx = unique_hash_pairs([build], [probe])
         assert build_idx.tolist() == list(range(nb - 1, -1, -1)) + [0, 1, 2]
         assert probe_idx.tolist() == list(range(nb)) + [nb + 1, nb + 2, nb + 3]
 
 
 def test_extreme_keys_and_the_empty_slot_value_in_a_unique_build():
-    build = _cuda([I64_MIN, 5, I64_MAX, -64, 0])
-    probe = _cuda([0, I64_MIN, 7, I64_MAX, I64_MIN, -64])
+    build = _dev([I64_MIN, 5, I64_MAX, -64, 0])
+    probe = _dev([0, I64_MIN, 7, I64_MAX, I64_MIN, -64])
     build_idx, probe_idx = unique_hash_pairs([build], [probe])
     assert probe_idx.tolist() == [0, 1, 3, 4, 5]
     assert build_idx.tolist() == [4, 0, 2, 0, 3]
@@ -207,8 +207,8 @@ def test_extreme_keys_and_the_empty_slot_value_in_a_unique_build():
 
 def test_a_repeated_build_key_takes_the_lowest_row():
     # outside the uniqueness proof: the documented "first build row"
-    build = _cuda([4, 9, 4, 4, 9])
-    probe = _cuda([9, 4, 1])
+    build = _dev([4, 9, 4, 4, 9])
+    probe = _dev([9, 4, 1])
     build_idx, probe_idx = unique_hash_pairs([build], [probe])
     assert probe_idx.tolist() == [0, 1] and build_idx.tolist() == [1, 0]
 

@@ -45,7 +45,7 @@ def test_extension_module_has_the_reference_functions():
     from pageflow_amd import tqpkernels
     want = {"partition_sort": ["cols", "comp", "key_idx", "mask", "n_valid", "goff", "B",
                                "select_first", "host_out"],
-            "join_expand": ["sorted_ids", "start", "length", "host_out"],
+            "join_expand": ["sorted_ids", "start", "len", "host_out"],
             "join_unique_hash": ["build_keys", "probe_keys", "host_out"],
             "join_semi_anti": ["build_key", "probe_key", "build_val", "probe_val", "op", "anti",
                                "host_out"]}

@@ -1,16 +1,24 @@
-"""The partition (pf_partition_sort through
-``pageflow_amd.operators.partition_sort.partition``) against the oracle.
+"""``partition`` (pf_partition_sort through
+``pageflow_amd.operators.partition_sort``) against ``oracle.partition``.
 
-Every output (each column, then the gidx) must hold, in the padded layout,
-the bucket counts and in every bucket the kept rows whose key hashes there,
-in source order, the gidx being goff plus the pre-mask row.  Padding rows are
-unwritten and not compared.  The cases are the reference's
-(tests/operators/test_partition_sort.py of the reference repository): tick
-sizes around the 2048-row tile, every B, six columns of four widths, every
-width, full skew, a global offset near 6e9, masks in every mode, wrong counts
-that must raise and leave the GPU usable, ticks past a thousand tiles, the
-limits."""
+Each case is a tick spec: rows, the column kinds beside the int64 key, the
+key's position, B, the global offset, a mask (kept fraction, which columns
+arrive compacted, whether the caller passes the kept count), a misaligned
+start for the device views and the select-first thresholds.  A case passes
+when every output (the columns, then the gidx) has the oracle's bucket
+counts and, at each bucket's padded start, exactly the oracle's rows;
+padding is not compared.
+
+The case list re-expresses the coverage of the reference's partition tests
+(tests/operators/test_partition_sort.py): tile-edge tick sizes, every B,
+six columns of four widths, each width, full skew, an offset near 6e9, masks
+in every kernel mode, ticks past a thousand tiles, wrong counts that raise
+and leave the GPU usable, and the kernel's limits.
+"""
 from __future__ import annotations
+
+from dataclasses import dataclass, field, replace
+from typing import List, Optional
 
 import numpy as np
 import pytest
@@ -23,271 +31,255 @@ from pageflow_amd.operators.partitions import JOIN_B_MAX
 
 pytestmark = pytest.mark.gpu
 
-
-def _gen(seed):
-    return torch.Generator(device="cpu").manual_seed(seed)
-
-
-def _keys(n, seed, skew=False):
-    if skew:
-        return torch.full((n,), 7, dtype=torch.int64)
-    return torch.randint(-(1 << 40), 1 << 40, (n,), dtype=torch.int64, generator=_gen(seed))
-
-
-def _column(n, kind, seed):
-    g = _gen(seed)
-    if kind == "i64":
-        return torch.randint(-(1 << 62), 1 << 62, (n,), dtype=torch.int64, generator=g)
-    if kind == "i32":
-        return torch.randint(-(1 << 30), 1 << 30, (n,), dtype=torch.int32, generator=g)
-    if kind == "i16":
-        return torch.randint(-(1 << 14), 1 << 14, (n,), dtype=torch.int16, generator=g)
-    if kind == "i8":
-        return torch.randint(-128, 128, (n,), dtype=torch.int8, generator=g)
-    if kind == "s25":
-        return torch.randint(0, 256, (n, 25), dtype=torch.uint8, generator=g)
-    if kind.startswith("u8x"):
-        return torch.randint(0, 256, (n, int(kind[3:])), dtype=torch.uint8, generator=g)
-    if kind == "i64x2":
-        return torch.randint(-(1 << 62), 1 << 62, (n, 2), dtype=torch.int64, generator=g)
-    raise ValueError(kind)
+# kind -> (numpy dtype, elements a row, low, high)
+KINDS = {
+    "i64": (np.int64, 1, -(1 << 62), 1 << 62),
+    "i32": (np.int32, 1, -(1 << 30), 1 << 30),
+    "i16": (np.int16, 1, -(1 << 14), 1 << 14),
+    "i8": (np.int8, 1, -128, 128),
+    "s25": (np.uint8, 25, 0, 256),
+    "u8x2": (np.uint8, 2, 0, 256),
+    "u8x3": (np.uint8, 3, 0, 256),
+    "u8x4": (np.uint8, 4, 0, 256),
+    "u8x8": (np.uint8, 8, 0, 256),
+    "i64x2": (np.int64, 2, -(1 << 62), 1 << 62),
+}
 
 
-def _table(n, kinds, seed, skew=False, key_pos=0):
-    cols = [_column(n, k, seed + 1 + c) for c, k in enumerate(kinds)]
-    cols.insert(key_pos, _keys(n, seed, skew))
-    return cols
+@dataclass
+class Tick:
+    n: int
+    kinds: List[str] = field(default_factory=lambda: ["i64"])
+    B: int = 32
+    key_pos: int = 0
+    goff: int = 0
+    seed: int = 0
+    one_key: bool = False            # every row the same key: one bucket holds all
+    keep: Optional[float] = None     # mask kept fraction; None: no mask
+    compacted: Optional[List[bool]] = None   # per column; None: all compacted
+    give_count: bool = False
+    offset: int = 0                  # device views start this many rows in
+    select_first_below: Optional[float] = None
+    raw_select_first_below: Optional[float] = None
+
+    def columns(self) -> List[np.ndarray]:
+        rng = np.random.default_rng(self.seed)
+        key = (np.full(self.n, 7, np.int64) if self.one_key
+               else rng.integers(-(1 << 40), 1 << 40, self.n, dtype=np.int64))
+        cols = []
+        for kind in self.kinds:
+            dt, width, lo, hi = KINDS[kind]
+            shape = (self.n,) if width == 1 else (self.n, width)
+            cols.append(rng.integers(lo, hi, shape, dtype=dt))
+        cols.insert(self.key_pos, key)
+        return cols
+
+    def mask(self) -> Optional[np.ndarray]:
+        if self.keep is None:
+            return None
+        rng = np.random.default_rng(self.seed + 7919)
+        return rng.random(self.n) < self.keep
 
 
-def _mask(n, keep, seed):
-    if keep <= 0:
-        return torch.zeros(n, dtype=torch.bool)
-    if keep >= 1:
-        return torch.ones(n, dtype=torch.bool)
-    return torch.rand(n, generator=_gen(seed)) < keep
+def _on_device(a: np.ndarray, offset: int) -> torch.Tensor:
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if offset:
+        lead = torch.zeros((offset,) + tuple(t.shape[1:]), dtype=t.dtype)
+        return torch.cat((lead, t)).cuda()[offset:]
+    return t.cuda()
 
 
-def _bucket_rows(pt):
-    t = pt.tensor.cpu()
-    rb = t.element_size() * (t.shape[1] if t.dim() == 2 else 1)
-    unit = oracle.rows_per_unit(rb)
-    rows, at = [], 0
-    for c in pt.partition_row_counts:
-        rows.append(t[at:at + c].numpy())
-        at += -(-c // unit) * unit
-    assert t.shape[0] == at, (t.shape[0], at)
-    return rows
+def _buckets_of(pt) -> List[np.ndarray]:
+    """The rows of each bucket of a padded output, read at its padded start."""
+    host = pt.tensor.cpu().numpy()
+    row_b = host.dtype.itemsize * (host.shape[1] if host.ndim == 2 else 1)
+    starts, total = oracle.padded_layout(pt.partition_row_counts, row_b)
+    assert host.shape[0] == total
+    return [host[s:s + c] for s, c in zip(starts, pt.partition_row_counts)]
 
 
-def _check(cols, key_pos, mask, goff, B, compact=None, count=False, offset=0):
-    """The kernel on ``cols`` (CPU tensors; raw, or compacted by ``mask``
-    where ``compact`` says so, every column when None) against the oracle.
-    ``offset``: the device columns and mask are views starting that many
-    rows into their allocation (off any 16-byte boundary)."""
-    def dev(t):
-        if not offset:
-            return t.cuda()
-        pad = torch.zeros((offset,) + tuple(t.shape[1:]), dtype=t.dtype)
-        return torch.cat((pad, t)).cuda()[offset:]
-    if mask is None:
-        given = list(cols)
-    else:
-        compact = compact if compact is not None else [True] * len(cols)
-        given = [c[mask] if k else c for c, k in zip(cols, compact)]
-    n_valid = int(mask.sum()) if count else None
-    got = partition([dev(g) for g in given], key_pos,
-                    dev(mask) if mask is not None else None, goff, B, n_valid=n_valid)
-    counts, want = oracle.partition([c.numpy() for c in cols], key_pos,
-                                    mask.numpy() if mask is not None else None, goff, B)
-    assert len(got) == len(want)
-    for g, w in zip(got, want):
-        assert g.partition_row_counts == counts
-        for b, (gb, wb) in enumerate(zip(_bucket_rows(g), w)):
-            assert np.array_equal(gb, wb), f"bucket {b} differs"
+def run(tick: Tick, monkeypatch=None) -> None:
+    if monkeypatch is not None:
+        if tick.select_first_below is not None:
+            monkeypatch.setattr(partition_sort, "SELECT_FIRST_BELOW", tick.select_first_below)
+        if tick.raw_select_first_below is not None:
+            monkeypatch.setattr(partition_sort, "SELECT_FIRST_RAW_BELOW",
+                                tick.raw_select_first_below)
+    cols, mask = tick.columns(), tick.mask()
+    given = cols
+    if mask is not None:
+        packed = tick.compacted if tick.compacted is not None else [True] * len(cols)
+        given = [c[mask] if p else c for c, p in zip(cols, packed)]
+    n_valid = int(mask.sum()) if (mask is not None and tick.give_count) else None
+    outs = partition([_on_device(c, tick.offset) for c in given], tick.key_pos,
+                     None if mask is None else _on_device(mask, tick.offset), tick.goff, tick.B,
+                     n_valid=n_valid)
+    counts, want = oracle.partition(cols, tick.key_pos, mask, tick.goff, tick.B)
+    assert len(outs) == len(want) == len(cols) + 1
+    for out, want_buckets in zip(outs, want):
+        assert out.partition_row_counts == counts
+        got_buckets = _buckets_of(out)
+        for b in range(tick.B):
+            assert np.array_equal(got_buckets[b], want_buckets[b]), f"bucket {b}"
 
 
-@pytest.mark.parametrize("n", [0, 1, 25, 2047, 2048, 2049, 300_000])
-def test_unmasked_tick_sizes(n):
-    _check(_table(n, ["i64"], seed=n), 0, None, 0, 32)
+# ----------------------------------------------------------------- cases --
+
+PLAIN = {
+    **{f"rows={n}": Tick(n, seed=n) for n in (0, 1, 25, 2047, 2048, 2049, 300_000)},
+    **{f"B={b}": Tick(100_000, B=b, goff=12345, seed=b)
+       for b in (1, 2, 8, 16, 32, 64, 128, 256, 512)},
+    **{f"six-columns-B{b}": Tick(70_001, ["i64", "i32", "i8", "s25", "i64"], B=b, key_pos=2,
+                                 seed=3) for b in (64, 512)},
+    **{f"width-{k}": Tick(50_000, [k], B=16, seed=11)
+       for k in ("i32", "i16", "i8", "s25", "u8x2", "u8x3", "u8x4", "u8x8", "i64x2")},
+    "one-bucket-holds-all": Tick(40_000, ["i64", "s25"], B=64, one_key=True, seed=5),
+    "goff-near-6e9": Tick(1 << 18, B=64, goff=6_000_000_000 - (1 << 18), seed=6),
+    "past-1000-tiles-B16": Tick(5_000_011, ["i32"], B=16, goff=7, seed=96),
+    "past-1000-tiles-B512": Tick(5_000_011, ["i32"], B=512, goff=7, seed=96),
+}
 
 
-@pytest.mark.parametrize("B", [1, 2, 8, 16, 32, 64, 128, 256, 512])
-def test_every_bucket_count(B):
-    _check(_table(100_000, ["i64"], seed=B), 0, None, 12345, B)
+@pytest.mark.parametrize("name", list(PLAIN))
+def test_unmasked_ticks(name):
+    run(PLAIN[name])
 
 
-@pytest.mark.parametrize("B", [64, 512])
-def test_six_columns_of_four_widths_with_the_key_inside(B):
-    _check(_table(70_001, ["i64", "i32", "i8", "s25", "i64"], seed=3, key_pos=2), 2, None, 0, B)
-
-
-@pytest.mark.parametrize("kind", ["i32", "i16", "i8", "s25", "u8x2", "u8x3", "u8x4", "u8x8",
-                                  "i64x2"])
-def test_each_row_width(kind):
-    _check(_table(50_000, [kind], seed=11), 0, None, 0, 16)
-
-
-def test_one_bucket_holds_every_row():
-    _check(_table(40_000, ["i64", "s25"], seed=5, skew=True), 0, None, 0, 64)
-
-
-def test_global_offset_near_6e9():
-    n = 1 << 18
-    _check(_table(n, ["i64"], seed=6), 0, None, 6_000_000_000 - n, 64)
-
-
-@pytest.mark.parametrize("keep", [0.0, 0.01, 0.3, 0.63, 1.0])
-@pytest.mark.parametrize("select_first_below", [0.0, 2.0])
-def test_compacted_columns_under_a_mask(keep, select_first_below, monkeypatch):
-    monkeypatch.setattr(partition_sort, "SELECT_FIRST_BELOW", select_first_below)
-    n = 200_003
-    cols = _table(n, ["i64", "s25"], seed=int(keep * 1000) + 1)
-    _check(cols, 0, _mask(n, keep, seed=21), 777, 32, count=True)
-
-
-def test_raw_columns_without_the_count_raise():
-    n = 12_000
-    cols = [c.cuda() for c in _table(n, ["i64"], seed=31)]
-    with pytest.raises(ValueError, match="need the mask's kept rows"):
-        partition(cols, 0, _mask(n, 0.4, seed=32).cuda(), 0, 16)
-
-
-@pytest.mark.parametrize("keep", [0.0, 0.095, 0.3, 1.0])
-@pytest.mark.parametrize("raw_below", [0.0, 2.0])
-def test_raw_columns_with_the_mask_count(keep, raw_below, monkeypatch):
-    monkeypatch.setattr(partition_sort, "SELECT_FIRST_RAW_BELOW", raw_below)
-    n = 150_001
-    cols = _table(n, ["i64", "s25", "i16"], seed=int(keep * 1000) + 3)
-    _check(cols, 0, _mask(n, keep, seed=33), 99, 64, compact=[False] * 4, count=True)
-
-
-def test_a_count_the_compacted_rows_contradict_raises():
-    n = 10_000
-    cols = _table(n, ["i64"], seed=34)
-    mask = _mask(n, 0.5, seed=35)
-    packed = [c[mask].cuda() for c in cols]
-    with pytest.raises(RuntimeError, match="compacted columns hold"):
-        partition(packed, 0, mask.cuda(), 0, 32, n_valid=int(mask.sum()) + 1)
-
-
-@pytest.mark.parametrize("raw_below", [0.0, 2.0])
-def test_raw_and_compacted_columns_mixed(raw_below, monkeypatch):
-    monkeypatch.setattr(partition_sort, "SELECT_FIRST_RAW_BELOW", raw_below)
-    n = 90_000
-    cols = _table(n, ["i64", "s25", "i32"], seed=41)
-    mask = _mask(n, 0.4, seed=42)
-    for compact in ([False, True, False, True], [True, False, True, False]):
-        _check(cols, 0, mask, 5, 64, compact=compact, count=True)
-
-
-@pytest.mark.parametrize("offset", [3, 5])
-def test_offset_mask_views(offset, monkeypatch):
-    n = 64_000
-    cols = _table(n, ["i64"], seed=51)
-    mask = _mask(n, 0.2, seed=52)
-    for below in (0.0, 2.0):
-        monkeypatch.setattr(partition_sort, "SELECT_FIRST_BELOW", below)
-        _check(cols, 0, mask, 0, 32, offset=offset)
-    for below in (0.0, 2.0):
-        monkeypatch.setattr(partition_sort, "SELECT_FIRST_RAW_BELOW", below)
-        _check(cols, 0, mask, 0, 32, compact=[False, False], count=True, offset=offset)
-
-
-def test_small_masked_ticks():
+def _masked_cases():
+    cases = {}
+    for keep in (0.0, 0.01, 0.3, 0.63, 1.0):
+        for below in (0.0, 2.0):
+            cases[f"compacted-keep{keep}-below{below}"] = Tick(
+                200_003, ["i64", "s25"], goff=777, seed=int(keep * 1000) + 1, keep=keep,
+                give_count=True, select_first_below=below)
+    for keep in (0.0, 0.095, 0.3, 1.0):
+        for below in (0.0, 2.0):
+            cases[f"raw-keep{keep}-below{below}"] = Tick(
+                150_001, ["i64", "s25", "i16"], B=64, goff=99, seed=int(keep * 1000) + 3,
+                keep=keep, compacted=[False] * 4, give_count=True, raw_select_first_below=below)
+    for i, packed in enumerate(([False, True, False, True], [True, False, True, False])):
+        for below in (0.0, 2.0):
+            cases[f"mixed{i}-below{below}"] = Tick(
+                90_000, ["i64", "s25", "i32"], B=64, goff=5, seed=41, keep=0.4,
+                compacted=packed, give_count=True, raw_select_first_below=below)
+    for off in (3, 5):
+        for below in (0.0, 2.0):
+            base = Tick(64_000, B=32, seed=51, keep=0.2, offset=off)
+            cases[f"offset{off}-compacted-below{below}"] = replace(base,
+                                                                 select_first_below=below)
+            cases[f"offset{off}-raw-below{below}"] = replace(
+                base, compacted=[False, False], give_count=True, raw_select_first_below=below)
     for n in (1, 25, 2049):
-        cols = _table(n, ["i64"], seed=n)
-        _check(cols, 0, _mask(n, 0.5, seed=n), 0, 8)
-        _check(cols, 0, _mask(n, 0.0, seed=n), 0, 8)
+        for keep in (0.5, 0.0):
+            # the count is given: a compacted column that keeps every row is
+            # indistinguishable from a raw one
+            cases[f"small{n}-keep{keep}"] = Tick(n, B=8, seed=n, keep=keep, give_count=True)
+    for b in (16, 512):
+        for mode, below, raw_below, packed in (("compacted", 0.0, 0.0, True),
+                                               ("select-first", 2.0, 0.0, True),
+                                               ("fold", 0.0, 0.0, False),
+                                               ("raw-select-first", 0.0, 2.0, False)):
+            cases[f"past-1000-tiles-{mode}-B{b}"] = Tick(
+                5_000_011, ["i32"], B=b, goff=7, seed=96, keep=0.45, compacted=[packed] * 2,
+                give_count=not packed, select_first_below=below,
+                raw_select_first_below=raw_below)
+    for b in (1, 128, 512):
+        for below in (0.0, 2.0):
+            for packed in (True, False):
+                cases[f"B{b}-below{below}-{'compacted' if packed else 'raw'}"] = Tick(
+                    80_000, ["i64", "s25"], B=b, seed=98, keep=0.37, compacted=[packed] * 3,
+                    give_count=not packed, select_first_below=below,
+                    raw_select_first_below=below)
+    return cases
 
 
-def test_repeated_calls_reuse_the_count_buffer():
-    for i, B in enumerate((64, 8, 512, 1, 128, 256, 16)):
-        _check(_table(30_000 + i, ["i32"], seed=60 + i), 0, None, i, B)
+MASKED = _masked_cases()
 
 
-@pytest.mark.parametrize("case", ["five widths", "seven columns", "int32 key", "B=1024",
-                                  "B=24", "B=0"])
-def test_ticks_outside_the_limits_raise(case):
+@pytest.mark.parametrize("name", list(MASKED))
+def test_masked_ticks(name, monkeypatch):
+    run(MASKED[name], monkeypatch)
+
+
+def test_counts_buffer_is_reused_across_calls_of_any_B():
+    for i, b in enumerate((64, 8, 512, 1, 128, 256, 16)):
+        run(Tick(30_000 + i, ["i32"], B=b, goff=i, seed=60 + i))
+
+
+def test_a_repeated_call_is_identical():
+    cols = [_on_device(c, 0) for c in Tick(100_000, seed=7).columns()]
+    first, second = partition(cols, 0, None, 0, 64), partition(cols, 0, None, 0, 64)
+    for a, b in zip(first, second):
+        assert a.partition_row_counts == b.partition_row_counts
+        assert all(np.array_equal(x, y) for x, y in zip(_buckets_of(a), _buckets_of(b)))
+
+
+# ---------------------------------------------------------------- errors --
+
+def test_raw_columns_need_the_kept_count():
+    tick = Tick(12_000, seed=31, keep=0.4)
+    cols = [_on_device(c, 0) for c in tick.columns()]
+    with pytest.raises(ValueError, match="need the mask's kept rows"):
+        partition(cols, 0, _on_device(tick.mask(), 0), 0, 16)
+
+
+def test_compacted_rows_contradicting_the_count_raise():
+    tick = Tick(10_000, seed=34, keep=0.5)
+    mask = tick.mask()
+    packed = [_on_device(c[mask], 0) for c in tick.columns()]
+    with pytest.raises(RuntimeError, match="compacted columns hold"):
+        partition(packed, 0, _on_device(mask, 0), 0, 32, n_valid=int(mask.sum()) + 1)
+
+
+@pytest.mark.parametrize("raw_below", [0.0, 2.0])
+@pytest.mark.parametrize("error", [-37, 41])
+def test_a_wrong_kept_count_raises_and_the_gpu_stays_usable(raw_below, error, monkeypatch):
+    tick = Tick(100_003, ["i64", "s25"], seed=91, keep=0.1, compacted=[False] * 3,
+                give_count=True, raw_select_first_below=raw_below)
+    monkeypatch.setattr(partition_sort, "SELECT_FIRST_RAW_BELOW", raw_below)
+    mask = tick.mask()
+    with pytest.raises(RuntimeError, match="the mask keeps"):
+        partition([_on_device(c, 0) for c in tick.columns()], 0, _on_device(mask, 0), 0, 32,
+                  n_valid=int(mask.sum()) + error)
+    torch.cuda.synchronize()
+    run(tick, monkeypatch)
+
+
+@pytest.mark.parametrize("below", [0.0, 2.0])
+def test_columns_compacted_by_another_mask_raise(below, monkeypatch):
+    tick = Tick(100_003, ["i64", "s25"], B=16, seed=93, keep=0.3, select_first_below=below)
+    monkeypatch.setattr(partition_sort, "SELECT_FIRST_BELOW", below)
+    kept = tick.mask()
+    packed = [_on_device(c[kept], 0) for c in tick.columns()]
+    for other_keep in (0.2, 0.4):
+        other = np.random.default_rng(95).random(tick.n) < other_keep
+        with pytest.raises(RuntimeError, match="the mask keeps"):
+            partition(list(packed), 0, _on_device(other, 0), 0, 16)
+    torch.cuda.synchronize()
+    run(tick, monkeypatch)
+
+
+def _over_limits(case):
     n = 1000
-    key = _keys(n, 70).cuda()
-    cols, B = {
-        "five widths": ([key] + [_column(n, k, 71).cuda() for k in ("i32", "i16", "i8", "s25")],
-                        32),
+    cols = [_on_device(c, 0) for c in Tick(n, ["i32", "i16", "i8", "s25"], seed=70).columns()]
+    key = cols[0]
+    return {
+        "five widths": (cols, 32),
         "seven columns": ([key] * 7, 32),
         "int32 key": ([key.to(torch.int32)], 32),
         "B=1024": ([key], 1024),
         "B=24": ([key], 24),
         "B=0": ([key], 0),
     }[case]
+
+
+@pytest.mark.parametrize("case", ["five widths", "seven columns", "int32 key", "B=1024",
+                                  "B=24", "B=0"])
+def test_ticks_outside_the_limits_raise(case):
+    cols, b = _over_limits(case)
     with pytest.raises(ValueError, match="outside the kernel's limits"):
-        partition(cols, 0, None, 0, B)
+        partition(cols, 0, None, 0, b)
 
 
-def test_the_bucket_limit_is_the_joins_cap():
+def test_the_bucket_cap_is_the_joins():
     assert partition_sort._B_MAX == JOIN_B_MAX == 512
-
-
-@pytest.mark.parametrize("raw_below", [0.0, 2.0])
-@pytest.mark.parametrize("off", [-37, 41])
-def test_a_count_the_mask_contradicts_raises(raw_below, off, monkeypatch):
-    monkeypatch.setattr(partition_sort, "SELECT_FIRST_RAW_BELOW", raw_below)
-    n = 100_003
-    cols = _table(n, ["i64", "s25"], seed=91)
-    mask = _mask(n, 0.1, seed=92)
-    with pytest.raises(RuntimeError, match="the mask keeps"):
-        partition([c.cuda() for c in cols], 0, mask.cuda(), 0, 32,
-                  n_valid=int(mask.sum()) + off)
-    torch.cuda.synchronize()
-    _check(cols, 0, mask, 0, 32, compact=[False] * 3, count=True)
-
-
-@pytest.mark.parametrize("select_first_below", [0.0, 2.0])
-def test_compacted_columns_of_another_mask_raise(select_first_below, monkeypatch):
-    monkeypatch.setattr(partition_sort, "SELECT_FIRST_BELOW", select_first_below)
-    n = 100_003
-    cols = _table(n, ["i64", "s25"], seed=93)
-    kept = _mask(n, 0.3, seed=94)
-    packed = [c[kept].cuda() for c in cols]
-    for other in (0.2, 0.4):
-        with pytest.raises(RuntimeError, match="the mask keeps"):
-            partition(list(packed), 0, _mask(n, other, seed=95).cuda(), 0, 16)
-    torch.cuda.synchronize()
-    _check(cols, 0, kept, 0, 16)
-
-
-@pytest.mark.parametrize("B", [16, 512])
-@pytest.mark.parametrize("mode", ["plain", "compacted-input", "select-first", "fold",
-                                  "raw-select-first"])
-def test_a_tick_past_a_thousand_tiles(mode, B, monkeypatch):
-    n = 5_000_011
-    cols = _table(n, ["i32"], seed=96)
-    if mode == "plain":
-        _check(cols, 0, None, 7, B)
-        return
-    monkeypatch.setattr(partition_sort, "SELECT_FIRST_BELOW", 2.0 if mode == "select-first" else 0.0)
-    monkeypatch.setattr(partition_sort, "SELECT_FIRST_RAW_BELOW",
-                        2.0 if mode == "raw-select-first" else 0.0)
-    raw = mode in ("fold", "raw-select-first")
-    _check(cols, 0, _mask(n, 0.45, seed=97), 7, B, compact=[not raw] * 2, count=raw)
-
-
-@pytest.mark.parametrize("B", [1, 128, 512])
-@pytest.mark.parametrize("below", [0.0, 2.0])
-def test_masked_ticks_at_one_to_512_buckets(below, B, monkeypatch):
-    monkeypatch.setattr(partition_sort, "SELECT_FIRST_BELOW", below)
-    monkeypatch.setattr(partition_sort, "SELECT_FIRST_RAW_BELOW", below)
-    n = 80_000
-    cols = _table(n, ["i64", "s25"], seed=98)
-    mask = _mask(n, 0.37, seed=99)
-    _check(cols, 0, mask, 0, B)
-    _check(cols, 0, mask, 0, B, compact=[False] * 3, count=True)
-
-
-def test_repeat_is_byte_identical():
-    cols = [c.cuda() for c in _table(100_000, ["i64"], seed=7)]
-    a = partition(cols, 0, None, 0, 64)
-    b = partition(cols, 0, None, 0, 64)
-    for x, y in zip(a, b):
-        assert x.partition_row_counts == y.partition_row_counts
-        for u, v in zip(_bucket_rows(x), _bucket_rows(y)):
-            assert np.array_equal(u, v)

@@ -18,7 +18,7 @@ import oracle
 from pageflow_amd import _lib
 from pageflow_amd.operators import residual
 from pageflow_amd.operators.hashjoin import _filter_pairs, evaluate_join
-from residual_cases import condition, sides
+from residual_cases import condition, sides, torch_mask
 
 pytestmark = pytest.mark.gpu
 
@@ -61,25 +61,39 @@ def test_filter_matches_oracle_on_random_conditions(n):
 
 @pytest.mark.parametrize("seed", range(3))
 def test_evaluate_join_mask_matches_oracle(seed):
+    """A tree the device program compiles must equal the oracle; one it
+    does not (a float against an integer, an integer column against a float
+    literal, ...) is evaluated with torch on the GPU and must equal torch's
+    own evaluation on the CPU, raising exactly where torch raises."""
     rng = np.random.default_rng(seed)
     left, right = sides(rng, 2000, 2000)
     dl, dr = _dev(left), _dev(right)
     n = 70_001
     lm, rm = rng.integers(0, 2000, n), rng.integers(0, 2000, n)
     tl, tr = torch.from_numpy(lm).cuda(), torch.from_numpy(rm).cuda()
-    ran = 0
+    on_program = on_torch = 0
     for _ in range(40):
         tree = condition(rng)
-        want = _oracle_or_none(lambda: oracle.residual_mask(left, right, tree, lm, rm))
         try:
-            got = evaluate_join("cuda", dl, dr, tree, tl, tr)
+            residual.compile_conditions(dl, dr, [tree])
+            compiled = True
         except NotImplementedError:
+            compiled = False
+        if compiled:
+            want = _oracle_or_none(lambda: oracle.residual_mask(left, right, tree, lm, rm))
+            assert want is not None, tree
+            on_program += 1
+        else:
+            want = _oracle_or_none(lambda: torch_mask(left, right, tree, lm, rm))
+            on_torch += 1
+        if want is None:
+            with pytest.raises((TypeError, RuntimeError, ValueError)):
+                evaluate_join("cuda", dl, dr, tree, tl, tr)
             continue
-        assert want is not None, tree
+        got = evaluate_join("cuda", dl, dr, tree, tl, tr)
         assert got.dtype == torch.bool
         assert np.array_equal(got.cpu().numpy(), want), tree
-        ran += 1
-    assert ran >= 15
+    assert on_program >= 15 and on_torch >= 1
 
 
 @pytest.mark.parametrize("op", ['=', '<>', '<', '<=', '>', '>='])
