@@ -36,25 +36,79 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name: str, nb: int, np_: int, pairs: int) -> float:
-    """Algorithmic bytes of one step's launches of a kernel: what the kernel
-    must read once and write once for its function (inputs, its own outputs;
-    no table or re-read traffic).  The partition kernels run twice a step
-    (two 6-bit passes): the first reads the int64 key and writes (hash 8 B,
-    row id 4 B), the second reads and writes (hash, row id)."""
-    table = {
-        # pass 1 reads the key, pass 2 the hash: 8 B a row each
-        "hj_probe_hist": np_ * 16, "hj_build_hist": nb * 16,
-        # pass 1: key 8 B in, 12 B out; pass 2: 12 B in, 12 B out
-        "hj_probe_scatter": np_ * 44, "hj_build_scatter": nb * 44,
-        # partitioned probe hashes read, build hashes read
+def kernel_bytes(name: str, nb: int, np_: int, pairs: int, launches: float) -> float:
+    """Algorithmic bytes of ONE launch of a kernel of the large join.
+
+    A complete radix partition of a side counts once what it must move: its
+    keys and row ids read once and written once, 24 B a row (the standalone
+    partition's figure).  The join partitions each side in two 6-bit passes
+    of a scatter kernel, so each scatter launch carries half of that; the
+    histogram launches move no algorithmic bytes of their own (their reads
+    are partition work already counted).  The count pass reads the
+    partitioned probe and build hashes; the write pass reads the partitioned
+    probe (hash, row id), build (hash, row id) and build payload once and
+    writes the pairs and the gathered payload."""
+    per_step = {
+        "hj_probe_scatter": np_ * 24, "hj_build_scatter": nb * 24,
         "hj_count": np_ * 8 + nb * 8,
-        # probe (hash, row id) read; build (hash, row id) and the build
-        # payload in partition order read once (staged in LDS per bucket);
-        # pairs (2 x 8 B) and the gathered payload (8 B) written
         "hj_write": np_ * 12 + nb * 20 + pairs * 24,
     }
-    return float(table.get(name, 0))
+    return float(per_step.get(name, 0)) / max(launches, 1.0)
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_threads() -> int:
+    """The cores this process may run on (its affinity set)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline_c1():
+    """The reference's join program on the CPU at the plain join's full size
+    (build 1M rows, int32 keys a seeded permutation of 1..1M; probe 6M int32
+    foreign keys, each matching once): the torch restatement of its
+    sort/search core (oracle/torch_cpu.py), which is the route the reference
+    takes for int32 keys, on every core this process may use; median of 3
+    runs after a warm-up."""
+    import torch
+    from oracle import torch_cpu
+
+    nb, np_ = 1_000_000, 6_000_000
+    g = torch.Generator().manual_seed(1)
+    build = (torch.randperm(nb, generator=g) + 1).to(torch.int32)
+    probe = build[torch.randint(0, nb, (np_,), generator=g)]
+    threads = _host_threads()
+    before = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    try:
+        li, ri = torch_cpu.sort_search_pairs(build, probe)
+        assert li.numel() == np_ and bool((build[li] == probe[ri]).all())
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            torch_cpu.sort_search_pairs(build, probe)
+            times.append(time.perf_counter() - t0)
+    finally:
+        torch.set_num_threads(before)
+    t = sorted(times)[1]
+    return {"value": (nb + np_) / t, "unit": "tuples/s", "cores": threads,
+            "os_cpu_count": os.cpu_count(), "cpu_model": _cpu_model(), "kind": "port",
+            "sample": f"C1 full size: the plain inner join of a 1M-row int32 build and a 6M-row "
+                      f"int32 probe (every probe row matching once), the reference's sort/search "
+                      f"join program restated in torch on the CPU (sort, two searchsorted, "
+                      f"expand), {threads} threads, median of 3 ({t * 1e3:.1f} ms)"}
 
 
 def cpu_baseline(build_cpu, probe_cpu, sample_rows: int):
@@ -92,18 +146,15 @@ def cpu_sample(fn, units: float, unit: str, what: str, reps: int = 3):
             "sample": f"{what}; median of {reps} ({t:.2f} s)"}
 
 
-def bench_partition(args, dev) -> None:
-    """The standalone radix partition of the large join's 640M probe keys
-    (int64) with int32 row ids into 2^bits buckets, for each bits given.
-    Algorithmic bytes: keys and row ids read once and written once (24 B a
-    row)."""
+def partition_sweep(args, dev, probe):
+    """The standalone radix partition of the large join's probe keys (int64)
+    with int32 row ids into 2^bits buckets, for each bits of
+    ``args.part_bits``.  Algorithmic bytes: keys and row ids read once and
+    written once (24 B a row)."""
     import torch
     from pageflow_amd import _lib, radix
-    from pageflow_amd.workloads import make_large_join
 
-    nb, np_ = args.build_rows, args.probe_rows
-    log(f"generating {np_} probe keys")
-    _, _, probe, _ = make_large_join(nb, np_, 2, dev)
+    np_ = probe.numel()
     rows = torch.arange(np_, dtype=torch.int32, device=dev)
     kout = torch.empty_like(probe)
     vout = torch.empty_like(rows)
@@ -127,19 +178,32 @@ def bench_partition(args, dev) -> None:
         prof = _lib.profile_read()
         _lib.profile(False)
         ms = (t1 - t0) * 1e3 / args.steps
-        sc = prof["rp_scatter"][0] / prof["rp_scatter"][1]
-        hi = prof["rp_hist"][0] / prof["rp_hist"][1]
-        # stability and counts: offsets end at n, a sample of buckets checked
+        sc = prof["rp_scatter"][0] / args.steps
+        hi = prof["rp_hist"][0] / args.steps
+        # the offsets end at n, and a sample of rows sits in its bucket
         assert int(offs[-1]) == np_
         key = f"{bits}/{shape}"
         sweep[key] = {"bits": bits, "shape": shape, "ms": ms, "rows_per_s": np_ / (ms * 1e-3),
-                       "GBps": np_ * 24 / (ms * 1e-3) / 1e9,
-                       "scatter_ms": sc, "hist_ms": hi,
-                       "scatter_GBps": np_ * 24 / (sc * 1e-3) / 1e9}
-        log(f"bits={bits} shape={shape}: {ms:.3f} ms ({sweep[key]['GBps']:.0f} GB/s), "
-            f"scatter {sc:.3f} ms, hist {hi:.3f} ms")
+                      "GBps": np_ * 24 / (ms * 1e-3) / 1e9,
+                      "frac": np_ * 24 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                      "scatter_ms_per_step": sc, "hist_ms_per_step": hi,
+                      "scatter_launches_per_step": prof["rp_scatter"][1] / args.steps}
+        log(f"partition bits={bits} shape={shape}: {ms:.3f} ms "
+            f"({sweep[key]['GBps']:.0f} GB/s, {sweep[key]['frac']:.3f} of peak), "
+            f"scatter {sc:.3f} ms, hist {hi:.3f} ms a step")
+    del kout, vout, rows
+    return sweep
+
+
+def bench_partition(args, dev) -> None:
+    import torch
+    from pageflow_amd.workloads import make_large_join
+
+    nb, np_ = args.build_rows, args.probe_rows
+    log(f"generating {np_} probe keys")
+    _, _, probe, _ = make_large_join(nb, np_, 2, dev)
+    sweep = partition_sweep(args, dev, probe)
     best = min(sweep, key=lambda b: sweep[b]["ms"])
-    del kout, vout
     agg = bench_agg_pass1(args, dev, probe)
     line = {
         "metric": "radix partition rows/s (640M int64 keys + int32 row ids)",
@@ -149,10 +213,9 @@ def bench_partition(args, dev) -> None:
         "data": "synthetic",
         "config": {"workload": "standalone radix partition of 640M int64 keys with int32 row "
                                "ids", "bits": sweep[best]["bits"], "shape": sweep[best]["shape"]},
-        "roofline": {"bound": "hbm", "kernel": "rp_scatter",
-                     "achieved": sweep[best]["scatter_GBps"], "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": sweep[best]["scatter_GBps"] / HBM_PEAK_GBPS,
-                     "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "radix partition (hist + scatter)",
+                     "achieved": sweep[best]["GBps"], "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": sweep[best]["frac"], "traffic": None},
         "sweep": sweep,
         "agg_pass1": agg,
     }
@@ -827,28 +890,45 @@ def main() -> None:
     kernels = {}
     for name, (tot_ms, launches) in prof.items():
         per_step = launches / args.steps
-        kernels[name] = {"avg_ms": tot_ms / launches, "launches_per_step": per_step}
+        alg = kernel_bytes(name, nb, np_, pairs, per_step)
+        kernels[name] = {"avg_ms": tot_ms / launches, "launches_per_step": per_step,
+                         "algorithmic_bytes_per_launch": alg,
+                         "GBps": alg / (tot_ms / launches * 1e-3) / 1e9}
     dom = max(prof, key=lambda k: prof[k][0])
     dom_ms = prof[dom][0] / prof[dom][1]
-    per_launch = kernel_bytes(dom, nb, np_, pairs) / max(1.0, prof[dom][1] / args.steps)
+    per_launch = kernel_bytes(dom, nb, np_, pairs, prof[dom][1] / args.steps)
     achieved = per_launch / (dom_ms * 1e-3) / 1e9
-    traffic = None
+    # traffic: the rocprofv3 FETCH_SIZE + WRITE_SIZE reading of this kernel
+    # committed under profiles/ (counters need their own profiler run); the
+    # line names the file and the commit it was measured at
+    traffic, traffic_source = None, None
     tpath = os.path.join(ROOT, "profiles", f"{dom}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            rec = json.load(f)
+        traffic = rec.get("bytes_per_launch")
+        traffic_source = (f"profiles/{dom}_traffic.json: rocprofv3 --pmc FETCH_SIZE and "
+                          f"WRITE_SIZE runs, scaled by a device copy's readings; measured at "
+                          f"commit {rec.get('commit', 'unrecorded')} ({rec.get('date', 'undated')})")
     e2e_bytes = nb * 8 + np_ * 8 + pairs * 16 + pairs * 16
     tuples = (nb + np_) / (ms * 1e-3)
 
-    cpu = None
-    if not args.no_cpu:
-        log("timing the CPU baseline")
-        cpu = cpu_baseline(build.cpu(), probe[:args.cpu_sample].cpu(), args.cpu_sample)
-
     bits = plan.bits
     tiers = {}
+    log("standalone partition sweep of the same probe keys")
+    del out_pay, src, dst, op, ob
+    del plan
+    torch.cuda.empty_cache()
+    tiers["partition"] = partition_sweep(args, dev, probe)
+
+    cpu = cpu_c2 = None
+    if not args.no_cpu:
+        log("timing the CPU baselines")
+        cpu = cpu_baseline_c1()
+        cpu_c2 = cpu_baseline(build.cpu(), probe[:args.cpu_sample].cpu(), args.cpu_sample)
+
     if args.tiers == "all":
-        del build, pays, probe, out_p, out_b, out_pay, src, dst, plan, op, ob
+        del build, pays, probe, out_p, out_b
         torch.cuda.empty_cache()
         tiers["skew"] = bench_skew(args, dev, emit=False)
         torch.cuda.empty_cache()
@@ -877,13 +957,15 @@ def main() -> None:
                    "build_rows": nb, "probe_rows": np_, "pairs": pairs, "bits": bits},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": traffic},
+                     "traffic": traffic, "traffic_source": traffic_source,
+                     "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": dom_ms},
         "e2e": {"algorithmic_bytes": e2e_bytes,
                 "GBps": e2e_bytes / (ms * 1e-3) / 1e9,
                 "frac": e2e_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS},
         "kernels": kernels,
         "copy_GBps": copy_gbps,
         "cpu_baseline": cpu,
+        "cpu_baseline_c2_sample": cpu_c2,
         **tiers,
     }
     print(json.dumps(line), flush=True)

@@ -275,6 +275,32 @@ def test_sort_search_pairs_is_the_dense_join():
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("buckets", [None, 4])
+def test_torch_cpu_sort_search_is_the_dense_join(buckets):
+    import torch
+    from oracle import torch_cpu
+    rng = np.random.default_rng(10)
+    lk = rng.integers(0, 40, 3000).astype(np.int32)
+    rk = rng.integers(0, 60, 2000).astype(np.int32)
+    counts = None
+    if buckets:
+        # a tick in bucket order: rows sorted by key mod 4, the runs never
+        # cross a bucket
+        lo, ro = np.argsort(lk % buckets, kind="stable"), np.argsort(rk % buckets, kind="stable")
+        lk, rk = lk[lo], rk[ro]
+        counts = (np.bincount(lk % buckets, minlength=buckets).tolist(),
+                  np.bincount(rk % buckets, minlength=buckets).tolist())
+    li, ri = torch_cpu.sort_search_pairs(torch.from_numpy(lk), torch.from_numpy(rk), counts)
+    assert li.dtype == ri.dtype == torch.int64
+    # right-row order, and each right row's left rows hold its key
+    assert bool((ri[1:] >= ri[:-1]).all())
+    got = sorted(zip(li.tolist(), ri.tolist()))
+    dl, dr = np.nonzero(lk[:, None] == rk[None, :])
+    assert got == sorted(zip(dl.tolist(), dr.tolist()))
+    nli, nri = oracle.sort_search_pairs(lk.astype(np.int64), rk.astype(np.int64), counts)
+    assert got == sorted(zip(nli.tolist(), nri.tolist()))
+
+
 # ------------------------------------------------------- partition/layout --
 
 def test_partition_is_a_stable_bucket_sort():
