@@ -247,6 +247,31 @@ int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int6
                        int64_t* out_build, int64_t* const* pay_out, void* workspace,
                        size_t workspace_bytes, void* stream);
 
+/* --------------------------------------------------- keyed group-by --
+ * Replaces the grouping of a keyed aggregation's pass 1
+ * (operators/aggregate.py:1511-1563: the linear group hash of
+ * _hash_key_columns, :551-564, then torch.unique(return_inverse)) and its
+ * SUM/COUNT (operators/operator_utils.py:342-399).  Rows group by the exact
+ * tuple of nkeys (1 or 2) int64 key columns k0, k1 (k1 ignored for 1 key),
+ * n < 2^30 rows.  Each group's hash is the reference's linear hash
+ * sum_i key_i * coeffs[i] (int64, wrapping); coeffs is a host array of
+ * nkeys values.  Outputs (caller-allocated, n rows each; the first G hold
+ * the groups, in ascending signed hash order): hash_out, key0_out (and
+ * key1_out for 2 keys) the group's key tuple, sum_out/cnt_out the exact
+ * int64 SUM(val) and COUNT(val != -64) when val is not NULL, bucket_out
+ * (optional) the hash's bucket among B (the top log2 B bits of its mix64,
+ * B a power of two <= 4096).  host_out (pinned int64[2 + B]): [0] G, [1]
+ * the groups whose hash equals the previous group's (distinct tuples the
+ * reference's hash would merge; 0 when the hash is injective on the
+ * input), [2 + b] the groups of bucket b.  Waits twice (for G, then for
+ * the counts). */
+size_t pf_group_by_keys_workspace_bytes(int64_t n, int32_t nkeys, int32_t B);
+int pf_group_by_keys(const int64_t* k0, const int64_t* k1, int32_t nkeys, int64_t n,
+                     const int64_t* coeffs, const int64_t* val, int32_t B, int64_t* hash_out,
+                     int64_t* key0_out, int64_t* key1_out, int64_t* sum_out, int64_t* cnt_out,
+                     int32_t* bucket_out, int64_t* host_out, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------ gather --
  * Replaces the payload gather of operators/hashjoin.py:359-377
  * (torch.index_select per output column) and conversion.py:39-50

@@ -35,6 +35,9 @@ Parity pinning (what checks each function besides the formula itself):
 * ``sort_search_pairs``: the dense build x probe comparison
   (``test_join_core.py:263-281``).
 * ``group_sum_count``: closed form (per-group sums by brute force).
+* ``keyed_pass1``: a brute force over Python dicts keyed by the tuple, the
+  hash computed with Python integers reduced mod 2^64, and the bucket by
+  the golden-pinned ``bucket_of``.
 * ``partition_outputs``: each bucket's rows in source order (brute force per
   bucket) at the starts ``padded_layout`` gives, padding unused.
 * ``zero_copy_rearrange`` / ``zero_copy_mask``: the reference test's closed
@@ -383,6 +386,45 @@ def group_sum_count(gid, val, G: int) -> Tuple[np.ndarray, np.ndarray]:
     c = np.empty(G, np.int64)
     _lib().o_group_sum_count(_p(g), _p(v), g.shape[0], G, _p(s), _p(c))
     return s, c
+
+
+def keyed_pass1(keys: Sequence[np.ndarray], vals: Optional[np.ndarray],
+                coeffs: Sequence[int], B: int = 1):
+    """Pass 1 of a keyed aggregation for int64 keys (aggregate.py:1511-1563):
+    the rows grouped by their exact key tuple; per group the linear grouping
+    hash sum_i key_i * coeffs[i] in wrapping int64 (_hash_key_columns,
+    aggregate.py:551-564), the key tuple, SUM(vals) and COUNT(vals != -64)
+    (operator_utils.py:342-399), in ascending hash order as torch.unique
+    orders them; and each group's bucket among B (bucket_of the hash) with
+    the bucket counts.  Returns (hash, [key columns], sums, counts, buckets,
+    bucket counts); sums and counts are None without vals."""
+    ks = [_i64(k) for k in keys]
+    n = ks[0].shape[0]
+    with np.errstate(over="ignore"):
+        h = np.zeros(n, np.int64)
+        for k, c in zip(ks, coeffs):
+            h += k * np.int64(c)
+    if n == 0:
+        e = np.empty(0, np.int64)
+        return e, [e.copy() for _ in ks], (e if vals is not None else None), \
+            (e.copy() if vals is not None else None), np.empty(0, np.int32), [0] * B
+    table = np.stack(ks, axis=1)
+    uniq, first, inv = np.unique(table, axis=0, return_index=True, return_inverse=True)
+    inv = inv.reshape(-1)
+    G = uniq.shape[0]
+    gh = h[first]
+    order = np.argsort(gh, kind="stable")
+    sums = cnts = None
+    if vals is not None:
+        v = _i64(vals)
+        sums = np.zeros(G, np.int64)
+        np.add.at(sums, inv, v)
+        cnts = np.bincount(inv, weights=(v != NULL), minlength=G).astype(np.int64)
+        sums, cnts = sums[order], cnts[order]
+    gh = gh[order]
+    bucket = bucket_of(gh, B) if B > 1 else np.zeros(G, np.int32)
+    return (gh, [uniq[order, i].copy() for i in range(len(ks))], sums, cnts, bucket,
+            np.bincount(bucket, minlength=B).astype(np.int64).tolist())
 
 
 # ------------------------------------------------- residual conditions --

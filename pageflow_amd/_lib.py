@@ -60,6 +60,9 @@ _SIGNATURES = {
                                             P]),
     "pf_pair_filter_write": (ctypes.c_int, [P, P, I64, P, P, P, P]),
     "pf_group_sum_count": (ctypes.c_int, [P, P, I64, I32, P, P, I32, P, P]),
+    "pf_group_by_keys_workspace_bytes": (SZ, [I64, I32, I32]),
+    "pf_group_by_keys": (ctypes.c_int, [P, P, I32, I64, P, P, I32, P, P, P, P, P, P, P, P, SZ,
+                                        P]),
     "pf_dim_table_slots": (I64, [I64]),
     "pf_dim_table_build": (ctypes.c_int, [P, P, I64, I32, P, I64, P, P]),
     "pf_fact_join_groupby": (ctypes.c_int, [P, P, I64, P, I64, P, I32, P, P, P]),

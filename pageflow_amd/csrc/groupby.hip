@@ -35,6 +35,13 @@
 #include "../../include/pageflow_join.h"
 
 namespace pf {
+
+size_t radix_partition_ws(int64_t n, int bits);
+int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
+                    int shift, int mode, void* keys_out, int32_t* vals_out,
+                    int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st,
+                    int who, int shape);
+
 namespace {
 
 constexpr int GB_NT = 512;
@@ -353,6 +360,249 @@ unsigned groupby_grid(int64_t n) {
   return (unsigned)g;
 }
 
+// ------------------------------------------------------ keyed group-by --
+//
+// Rows grouped by an exact tuple of one or two int64 key columns: an
+// open-addressing table of one slot a group (2n slots at most, CAS
+// inserts), every row's slot kept; the occupied slots numbered densely in
+// slot order; SUM/COUNT by those ids; then the groups ordered by the
+// reference's linear grouping hash (sum of key x coefficient, wrapping
+// int64) ascending, by stable LSD radix passes over its bits.
+
+constexpr int GK_NT = 256;
+constexpr int GK_IT = 16;
+constexpr int GK_TILE = GK_NT * GK_IT;
+constexpr int GK_W = GK_NT / kWave;
+static_assert(GK_IT * GK_W == kWave, "one wave scans a tile's (round, wave) counts");
+constexpr uint64_t kSignBit = 1ULL << 63;
+
+// 1 key: the slot holds the key (kEmptyKey: empty); the key int64 min
+// itself is slot `cap`, one past the table.
+__global__ void k_gk_insert1(const int64_t* __restrict__ k0, int64_t n, int64_t* tkey,
+                             uint32_t cap, int32_t* __restrict__ rowslot, int32_t* special) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t k = k0[i];
+  if (k == kEmptyKey) {
+    rowslot[i] = (int32_t)cap;
+    atomicOr(special, 1);
+    return;
+  }
+  uint32_t s = slot_of(mix64((uint64_t)k), cap);
+  for (;;) {
+    int64_t prev = tkey[s];
+    if (prev == kEmptyKey)
+      prev = (int64_t)atomicCAS((unsigned long long*)&tkey[s], (unsigned long long)kEmptyKey,
+                                (unsigned long long)k);
+    if (prev == kEmptyKey || prev == k) {
+      rowslot[i] = (int32_t)s;
+      return;
+    }
+    s = next_slot(s, cap);
+  }
+}
+
+// 2 keys: the slot holds the row that claimed it (-1: empty); tuples are
+// compared through the key columns.
+__global__ void k_gk_insert2(const int64_t* __restrict__ k0, const int64_t* __restrict__ k1,
+                             int64_t n, int32_t* trow, uint32_t cap,
+                             int32_t* __restrict__ rowslot) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t a = k0[i], b = k1[i];
+  uint32_t s = slot_of(tuple_hash(a, b, true), cap);
+  for (;;) {
+    int32_t r = trow[s];
+    if (r == -1) r = atomicCAS(&trow[s], -1, (int32_t)i);
+    if (r == -1 || (k0[r] == a && k1[r] == b)) {
+      rowslot[i] = (int32_t)s;
+      return;
+    }
+    s = next_slot(s, cap);
+  }
+}
+
+template <bool TWO>
+__device__ __forceinline__ bool gk_occupied(int64_t s, uint32_t cap, const int64_t* tkey,
+                                            const int32_t* trow, const int32_t* special) {
+  if (s > (int64_t)cap) return false;
+  if (TWO) return s < (int64_t)cap && trow[s] != -1;
+  return s == (int64_t)cap ? *special != 0 : tkey[s] != kEmptyKey;
+}
+
+// occupied slots a tile of GK_TILE slots (slots 0..cap, the extra one too)
+template <bool TWO>
+__global__ __launch_bounds__(GK_NT) void k_gk_count(const int64_t* __restrict__ tkey,
+                                                    const int32_t* __restrict__ trow,
+                                                    const int32_t* __restrict__ special,
+                                                    uint32_t cap, int32_t* __restrict__ tilecnt) {
+  __shared__ int red[GK_W];
+  const int64_t base = (int64_t)blockIdx.x * GK_TILE;
+  int c = 0;
+#pragma unroll
+  for (int j = 0; j < GK_IT; ++j)
+    c += gk_occupied<TWO>(base + j * GK_NT + threadIdx.x, cap, tkey, trow, special);
+  c = wave_sum(c);
+  if (lane_id() == 0) red[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < GK_W; ++w) t += red[w];
+    tilecnt[blockIdx.x] = t;
+  }
+}
+
+// The occupied slots numbered in slot order: each group's dense id, its key
+// tuple, and its sort key (the linear hash with the sign bit flipped, so
+// that unsigned order is signed order) beside its id.
+template <bool TWO>
+__global__ __launch_bounds__(GK_NT) void k_gk_number(
+    const int64_t* __restrict__ tkey, const int32_t* __restrict__ trow,
+    const int32_t* __restrict__ special, uint32_t cap, const int64_t* __restrict__ k0,
+    const int64_t* __restrict__ k1, int64_t c0, int64_t c1, const int64_t* __restrict__ offs,
+    int32_t* __restrict__ slot_gid, int64_t* __restrict__ rk0, int64_t* __restrict__ rk1,
+    int64_t* __restrict__ sort_key, int32_t* __restrict__ sort_val) {
+  __shared__ int cnt[kWave];
+  const int64_t base = (int64_t)blockIdx.x * GK_TILE;
+  const int lane = lane_id(), w = threadIdx.x / kWave;
+  bool occ[GK_IT];
+  int rank[GK_IT];
+#pragma unroll
+  for (int j = 0; j < GK_IT; ++j) {
+    occ[j] = gk_occupied<TWO>(base + j * GK_NT + threadIdx.x, cap, tkey, trow, special);
+    const uint64_t bal = ballot(occ[j]);
+    rank[j] = mbcnt(bal);
+    if (lane == 0) cnt[j * GK_W + w] = __popcll(bal);
+  }
+  __syncthreads();
+  if (w == 0) {
+    const int v = cnt[lane];
+    cnt[lane] = wave_incl_scan(v) - v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < GK_IT; ++j) {
+    if (!occ[j]) continue;
+    const int64_t s = base + j * GK_NT + threadIdx.x;
+    const int64_t g = offs[blockIdx.x] + cnt[j * GK_W + w] + rank[j];
+    int64_t a, b = 0;
+    if (TWO) {
+      const int32_t r = trow[s];
+      a = k0[r];
+      b = k1[r];
+      rk1[g] = b;
+    } else {
+      a = s == (int64_t)cap ? kEmptyKey : tkey[s];
+    }
+    rk0[g] = a;
+    slot_gid[s] = (int32_t)g;
+    const uint64_t h = (uint64_t)a * (uint64_t)c0 + (TWO ? (uint64_t)b * (uint64_t)c1 : 0ULL);
+    sort_key[g] = (int64_t)(h ^ kSignBit);
+    sort_val[g] = (int32_t)g;
+  }
+}
+
+__global__ void k_gk_row_gid(const int32_t* __restrict__ rowslot, int64_t n,
+                             const int32_t* __restrict__ slot_gid, int32_t* __restrict__ gid) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
+    gid[i] = slot_gid[rowslot[i]];
+}
+
+// The groups in hash order: hash, keys, sums, counts and buckets at their
+// sorted positions; bucket counts and repeated hashes counted.
+__global__ void k_gk_emit(const int64_t* __restrict__ sort_key,
+                          const int32_t* __restrict__ sort_val, int64_t G, bool two,
+                          const int64_t* __restrict__ rk0, const int64_t* __restrict__ rk1,
+                          const int64_t* __restrict__ sum_s, const int64_t* __restrict__ cnt_s,
+                          int logB, int64_t* __restrict__ hash_out, int64_t* __restrict__ key0_out,
+                          int64_t* __restrict__ key1_out, int64_t* __restrict__ sum_out,
+                          int64_t* __restrict__ cnt_out, int32_t* __restrict__ bucket_out,
+                          int64_t* bcount, int64_t* repeats) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < G; j += stride) {
+    const int32_t g = sort_val[j];
+    const uint64_t h = (uint64_t)sort_key[j] ^ kSignBit;
+    hash_out[j] = (int64_t)h;
+    key0_out[j] = rk0[g];
+    if (two) key1_out[j] = rk1[g];
+    if (sum_out) sum_out[j] = sum_s[g];
+    if (cnt_out) cnt_out[j] = cnt_s[g];
+    const int b = logB ? (int)(mix64(h) >> (64 - logB)) : 0;
+    if (bucket_out) bucket_out[j] = b;
+    atomicAdd((unsigned long long*)&bcount[b], 1ULL);
+    if (j > 0 && sort_key[j - 1] == sort_key[j]) atomicAdd((unsigned long long*)repeats, 1ULL);
+  }
+}
+
+__global__ void k_gk_readback(const int64_t* __restrict__ repeats,
+                              const int64_t* __restrict__ bcount, int B, int64_t* host) {
+  for (int b = threadIdx.x; b < B + 1; b += blockDim.x)
+    store_host(&host[1 + b], b == 0 ? *repeats : bcount[b - 1]);
+}
+
+__global__ void k_gk_read_groups(const int64_t* __restrict__ count, int64_t* host) {
+  if (threadIdx.x == 0) store_host(&host[0], *count);
+}
+
+struct GkWs {
+  int64_t* tkey;
+  int32_t* trow;
+  int32_t* special;
+  int32_t* rowslot;
+  int32_t* tilecnt;
+  int64_t* offs;
+  int64_t* count;
+  void* scan;
+  int32_t* slot_gid;
+  int32_t* gid;
+  int64_t *rk0, *rk1, *sum_s, *cnt_s;
+  int64_t *sk[2];
+  int32_t *sv[2];
+  int64_t* sort_off;
+  void* rp_ws;
+  size_t rp_bytes;
+  int64_t *bcount, *repeats;
+  uint32_t cap;
+  int64_t tiles;
+  size_t bytes;
+};
+
+GkWs carve_gk(void* ws, int64_t n, bool two, int B) {
+  Carve cv(ws);
+  GkWs w;
+  uint32_t cap = 16;
+  while ((int64_t)cap < 2 * n) cap <<= 1;
+  w.cap = cap;
+  const int64_t m = n > 0 ? n : 1;
+  w.tiles = ((int64_t)cap + 1 + GK_TILE - 1) / GK_TILE;
+  w.tkey = cv.take<int64_t>(two ? 1 : cap);
+  w.trow = cv.take<int32_t>(two ? cap : 1);
+  w.special = cv.take<int32_t>(1);
+  w.rowslot = cv.take<int32_t>(m);
+  w.tilecnt = cv.take<int32_t>(w.tiles);
+  w.offs = cv.take<int64_t>(w.tiles);
+  w.count = cv.take<int64_t>(2);
+  w.scan = cv.take<char>(scan_i32_scratch(w.tiles));
+  w.slot_gid = cv.take<int32_t>((int64_t)cap + 1);
+  w.gid = cv.take<int32_t>(m);
+  w.rk0 = cv.take<int64_t>(m);
+  w.rk1 = cv.take<int64_t>(two ? m : 1);
+  w.sum_s = cv.take<int64_t>(m);
+  w.cnt_s = cv.take<int64_t>(m);
+  for (int c = 0; c < 2; ++c) {
+    w.sk[c] = cv.take<int64_t>(m);
+    w.sv[c] = cv.take<int32_t>(m);
+  }
+  w.sort_off = cv.take<int64_t>(257);
+  w.rp_bytes = radix_partition_ws(m, 8);
+  w.rp_ws = cv.take<char>(w.rp_bytes);
+  w.bcount = cv.take<int64_t>(B > 0 ? B : 1);
+  w.repeats = cv.take<int64_t>(1);
+  w.bytes = cv.bytes();
+  return w;
+}
+
 }  // namespace
 }  // namespace pf
 
@@ -380,6 +630,100 @@ int pf_group_sum_count(const int32_t* gid, const int64_t* val, int64_t n, int32_
                                                                   bad);
   }
   if (launch_check("group_sum_count")) return 1;
+  return ok();
+}
+
+size_t pf_group_by_keys_workspace_bytes(int64_t n, int32_t nkeys, int32_t B) {
+  return pf::carve_gk(nullptr, n, nkeys == 2, B).bytes;
+}
+
+int pf_group_by_keys(const int64_t* k0, const int64_t* k1, int32_t nkeys, int64_t n,
+                     const int64_t* coeffs, const int64_t* val, int32_t B, int64_t* hash_out,
+                     int64_t* key0_out, int64_t* key1_out, int64_t* sum_out, int64_t* cnt_out,
+                     int32_t* bucket_out, int64_t* host_out, void* workspace,
+                     size_t workspace_bytes, void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (nkeys != 1 && nkeys != 2) return fail("group_by_keys: one or two key columns");
+  if (n < 0 || n >= (1LL << 30)) return fail("group_by_keys: n must be in [0, 2^30)");
+  if (B < 1 || B > 4096 || (B & (B - 1)))
+    return fail("group_by_keys: B must be a power of two in [1, 4096]");
+  if (!coeffs) return fail("group_by_keys: the hash coefficients are required");
+  if (val && (!sum_out || !cnt_out)) return fail("group_by_keys: a value column needs sum and count outputs");
+  const bool two = nkeys == 2;
+  GkWs w = carve_gk(workspace, n, two, B);
+  if (workspace_bytes < w.bytes) return fail("group_by_keys: workspace too small");
+  void* hdev = nullptr;
+  if (host_alias(host_out, &hdev)) return 1;
+  int logB = 0;
+  while ((1 << logB) < B) ++logB;
+  const uint32_t cap = w.cap;
+  if (n == 0) {
+    for (int b = 0; b < 2 + B; ++b) host_out[b] = 0;
+    return ok();
+  }
+  // 1. every row's slot
+  if (two) {
+    if (fill_i32(w.trow, cap, -1, st)) return 1;
+    k_gk_insert2<<<grid_for(n, 256), 256, 0, st>>>(k0, k1, n, w.trow, cap, w.rowslot);
+  } else {
+    if (fill_i64(w.tkey, cap, kEmptyKey, st) || fill_i32(w.special, 1, 0, st)) return 1;
+    k_gk_insert1<<<grid_for(n, 256), 256, 0, st>>>(k0, n, w.tkey, cap, w.rowslot, w.special);
+  }
+  // 2. the groups numbered in slot order, with their keys and sort keys
+  if (two)
+    k_gk_count<true><<<(unsigned)w.tiles, GK_NT, 0, st>>>(w.tkey, w.trow, w.special, cap,
+                                                          w.tilecnt);
+  else
+    k_gk_count<false><<<(unsigned)w.tiles, GK_NT, 0, st>>>(w.tkey, w.trow, w.special, cap,
+                                                           w.tilecnt);
+  if (launch_check("group_by_keys")) return 1;
+  if (scan_i32_to_i64_excl(w.tilecnt, w.offs, w.tiles, w.count, w.scan, st)) return 1;
+  const int64_t c1 = two ? coeffs[1] : 0;
+  if (two)
+    k_gk_number<true><<<(unsigned)w.tiles, GK_NT, 0, st>>>(
+        w.tkey, w.trow, w.special, cap, k0, k1, coeffs[0], c1, w.offs, w.slot_gid, w.rk0,
+        w.rk1, w.sk[0], w.sv[0]);
+  else
+    k_gk_number<false><<<(unsigned)w.tiles, GK_NT, 0, st>>>(
+        w.tkey, w.trow, w.special, cap, k0, k0, coeffs[0], 0, w.offs, w.slot_gid, w.rk0,
+        w.rk1, w.sk[0], w.sv[0]);
+  k_gk_read_groups<<<1, 64, 0, st>>>(w.count, (int64_t*)hdev);
+  if (launch_check("group_by_keys")) return 1;
+  if (wait_stream(st, "group_by_keys")) return 1;
+  const int64_t G = host_out[0];
+  if (G < 1 || G > n) return fail("group_by_keys: counted " + std::to_string(G) + " groups");
+  // 3. SUM / COUNT by dense id
+  if (val) {
+    k_gk_row_gid<<<groupby_grid(n), GB_NT, 0, st>>>(w.rowslot, n, w.slot_gid, w.gid);
+    if (fill_i64(w.sum_s, G, 0, st) || fill_i64(w.cnt_s, G, 0, st)) return 1;
+    const bool lds = G <= GB_LDS_GROUPS;
+    const int R = lds ? replicas_for((int)G) : 1;
+    KTimer _kt("group_by_keys_sum", st);
+    if (lds)
+      k_group_sum_count<true><<<groupby_grid(n), GB_NT, 0, st>>>(w.gid, val, n, (int)G, R,
+                                                                 w.sum_s, w.cnt_s, nullptr);
+    else
+      k_group_sum_count<false><<<groupby_grid(n), GB_NT, 0, st>>>(w.gid, val, n, (int)G, R,
+                                                                  w.sum_s, w.cnt_s, nullptr);
+  }
+  // 4. the groups in hash order: 8 stable passes of 8 bits, ending in sk[0]
+  if (G > 1) {
+    for (int pass = 0; pass < 8; ++pass) {
+      const int c = pass & 1;
+      if (radix_partition(w.sk[c], 8, w.sv[c], G, 8, pass * 8, 2, w.sk[c ^ 1], w.sv[c ^ 1],
+                          w.sort_off, w.rp_ws, w.rp_bytes, st, 5, -1))
+        return 1;
+    }
+  }
+  if (fill_i64(w.bcount, B, 0, st) || fill_i64(w.repeats, 1, 0, st)) return 1;
+  k_gk_emit<<<grid_for(G, 256) < 4096 ? grid_for(G, 256) : 4096, 256, 0, st>>>(
+      w.sk[0], w.sv[0], G, two, w.rk0, w.rk1, w.sum_s, w.cnt_s, logB, hash_out, key0_out,
+      key1_out, val ? sum_out : nullptr, val ? cnt_out : nullptr, bucket_out, w.bcount,
+      w.repeats);
+  k_gk_readback<<<1, 256, 0, st>>>(w.repeats, w.bcount, B, (int64_t*)hdev);
+  if (launch_check("group_by_keys")) return 1;
+  if (wait_stream(st, "group_by_keys")) return 1;
   return ok();
 }
 

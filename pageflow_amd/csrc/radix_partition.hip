@@ -458,10 +458,10 @@ RpWs carve_rp(void* ws, int64_t n, int bits) {
 
 // kernel names for the profiler, by caller: the C entry point, the join's
 // build side, the join's probe side, the zero-copy gather's index sort
-const char* const kHistName[5] = {"rp_hist", "hj_build_hist", "hj_probe_hist", "zc_sort_hist",
-                                  "hj_bigsort_hist"};
-const char* const kScatterName[5] = {"rp_scatter", "hj_build_scatter", "hj_probe_scatter",
-                                     "zc_sort_scatter", "hj_bigsort_scatter"};
+const char* const kHistName[6] = {"rp_hist", "hj_build_hist", "hj_probe_hist", "zc_sort_hist",
+                                  "hj_bigsort_hist", "gk_sort_hist"};
+const char* const kScatterName[6] = {"rp_scatter", "hj_build_scatter", "hj_probe_scatter",
+                                     "zc_sort_scatter", "hj_bigsort_scatter", "gk_sort_scatter"};
 
 template <typename K, typename KO, int MODE, int NT, int IT>
 void launch8(const K* keys, const int32_t* vals, int64_t n, int64_t crows, int bits, int shift,

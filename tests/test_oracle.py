@@ -301,6 +301,40 @@ def test_torch_cpu_sort_search_is_the_dense_join(buckets):
     assert got == sorted(zip(nli.tolist(), nri.tolist()))
 
 
+def _brute_keyed(keys, vals, coeffs):
+    """Python dicts: tuple -> (hash, sum, count), the hash from Python ints
+    reduced mod 2^64 and read as signed."""
+    groups = {}
+    for r in range(len(keys[0])):
+        t = tuple(int(k[r]) for k in keys)
+        h = sum(a * c for a, c in zip(t, coeffs)) % (1 << 64)
+        h = h - (1 << 64) if h >= 1 << 63 else h
+        g = groups.setdefault(t, [h, 0, 0])
+        v = int(vals[r])
+        g[1] += v
+        g[2] += v != -64
+    return sorted(((g[0], t, g[1], g[2]) for t, g in groups.items()))
+
+
+@pytest.mark.parametrize("nkeys,n,span", [(1, 0, 5), (1, 1, 5), (1, 3000, 40), (1, 5000, 10**6),
+                                          (2, 4000, 7), (2, 5000, 300)])
+def test_keyed_pass1_is_the_brute_force(nkeys, n, span):
+    rng = np.random.default_rng(n + nkeys)
+    keys = [rng.integers(-span, span, n) for _ in range(nkeys)]
+    if n > 10:
+        keys[0][:4] = [I64.min, I64.max, -1, 0]
+    vals = rng.integers(-10**12, 10**12, n)
+    vals[::7] = -64
+    coeffs = [int(c) for c in rng.integers(2, 10_000_000, nkeys)]
+    h, ks, sums, cnts, bucket, bcounts = oracle.keyed_pass1(keys, vals, coeffs, 16)
+    want = _brute_keyed(keys, vals, coeffs)
+    assert h.tolist() == [w[0] for w in want]
+    assert [tuple(int(k[i]) for k in ks) for i in range(len(h))] == [w[1] for w in want]
+    assert sums.tolist() == [w[2] for w in want] and cnts.tolist() == [w[3] for w in want]
+    assert np.array_equal(bucket, oracle.bucket_of(h, 16)) if len(h) else True
+    assert sum(bcounts) == len(h)
+
+
 # ------------------------------------------------------- partition/layout --
 
 def test_partition_is_a_stable_bucket_sort():
