@@ -44,12 +44,15 @@ def kernel_bytes(name: str, nb: int, np_: int, pairs: int, launches: float) -> f
     partition's figure).  The join partitions each side in two 6-bit passes
     of a scatter kernel, so each scatter launch carries half of that; the
     histogram launches move no algorithmic bytes of their own (their reads
-    are partition work already counted).  The count pass reads the
+    are partition work already counted); the two passes are timed
+    apart (hj_probe_scatter, hj_probe_scatter_2).  The count pass (only in
+    the ordered mode) reads the
     partitioned probe and build hashes; the write pass reads the partitioned
     probe (hash, row id), build (hash, row id) and build payload once and
     writes the pairs and the gathered payload."""
     per_step = {
-        "hj_probe_scatter": np_ * 24, "hj_build_scatter": nb * 24,
+        "hj_probe_scatter": np_ * 12, "hj_probe_scatter_2": np_ * 12,
+        "hj_build_scatter": nb * 12, "hj_build_scatter_2": nb * 12,
         "hj_count": np_ * 8 + nb * 8,
         "hj_write": np_ * 12 + nb * 20 + pairs * 24,
     }
@@ -787,6 +790,9 @@ def main() -> None:
     ap.add_argument("--build-rows", type=int, default=16_000_000)
     ap.add_argument("--probe-rows", type=int, default=640_000_000)
     ap.add_argument("--bits", type=int, default=None)
+    ap.add_argument("--ordered", action="store_true",
+                    help="the large join's pairs in the deterministic partition order "
+                         "(adds the count pass)")
     ap.add_argument("--cpu-sample", type=int, default=16_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["join", "partition", "batch", "serve", "skew", "uva"],
@@ -838,16 +844,18 @@ def main() -> None:
     log(f"generating {nb} build / {np_} probe rows")
     build, pays, probe, expect = make_large_join(nb, np_, 2, dev)
     torch.cuda.synchronize()
-    plan = HashJoin(nb, np_, unique=True, bits=args.bits, device=dev)
+    # pairs in any order: no count pass (the pair order is not part of the
+    # join's result, which is compared as a multiset)
+    plan = HashJoin(nb, np_, unique=True, bits=args.bits, device=dev,
+                    ordered=args.ordered)
     out_p = torch.empty(np_, dtype=torch.int64, device=dev)
     out_b = torch.empty(np_, dtype=torch.int64, device=dev)
     out_pay = torch.empty(np_, dtype=torch.int64, device=dev)
     log(f"plan: bits={plan.bits} workspace={plan.ws_bytes / 2**30:.2f} GiB")
 
     def step():
-        n = plan.prepare(build, probe)
-        plan.write(out_p, out_b, [pays[0]], [out_pay])
-        return n
+        plan.prepare(build, probe)
+        return plan.write(out_p, out_b, [pays[0]], [out_pay])
 
     for i in range(args.warmup):
         n = step()
@@ -912,6 +920,14 @@ def main() -> None:
                           f"commit {rec.get('commit', 'unrecorded')} ({rec.get('date', 'undated')})")
     e2e_bytes = nb * 8 + np_ * 8 + pairs * 16 + pairs * 16
     tuples = (nb + np_) / (ms * 1e-3)
+    # the probe side's complete partition (histogram and both scatter
+    # passes): 24 B a row (keys and row ids read once and written once)
+    part_ms = sum(prof[k][0] for k in ("hj_probe_hist", "hj_probe_scatter",
+                                       "hj_probe_scatter_2") if k in prof) / args.steps
+    partition = {"kernels": ["hj_probe_hist", "hj_probe_scatter", "hj_probe_scatter_2"],
+                 "ms_per_step": part_ms, "algorithmic_bytes": np_ * 24,
+                 "GBps": np_ * 24 / (part_ms * 1e-3) / 1e9,
+                 "frac": np_ * 24 / (part_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS}
 
     bits = plan.bits
     tiers = {}
@@ -954,11 +970,13 @@ def main() -> None:
         "config": {"workload": "large plain join: build 16M unique int64 keys + 2 int64 "
                                "payloads, probe 640M int64 keys (10% absent); pairs + 1 "
                                "gathered payload",
-                   "build_rows": nb, "probe_rows": np_, "pairs": pairs, "bits": bits},
+                   "build_rows": nb, "probe_rows": np_, "pairs": pairs, "bits": bits,
+                   "pair_order": "partition order" if args.ordered else "any (no count pass)"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic, "traffic_source": traffic_source,
-                     "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": dom_ms},
+                     "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": dom_ms,
+                     "probe_partition": partition},
         "e2e": {"algorithmic_bytes": e2e_bytes,
                 "GBps": e2e_bytes / (ms * 1e-3) / 1e9,
                 "frac": e2e_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS},

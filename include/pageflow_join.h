@@ -230,22 +230,27 @@ int pf_radix_partition_shape(const void* keys, int32_t key_bytes, const int32_t*
  * with its key); multi = 1: every matching build row, each build bucket
  * sorted by (hash, row) so a key's rows are one run (in LDS up to 8192
  * rows; a larger bucket by stable 8-bit LSD passes of the radix partition
- * over its hashes in HBM, after one more wait).  Two calls on one
- * workspace: prepare partitions, counts and waits, leaving in host_out
- * (pinned int64[3]) [0] the pair count, [1] the largest build bucket, [2]
- * the work items; write fills
- * out_probe/out_build (int64, at least the pair count rows) and gathers
- * npay (<= 2) int64 build payload columns by build row into pay_out.  Pairs
- * come in a deterministic order (bucket, then probe row in partition
- * order, then build row ascending).  Keys are int64; np, nb < 2^31. */
+ * over its hashes in HBM, after one more wait); multi = 2: a unique build
+ * whose pairs may come in any order (no count pass, see below).  Two calls
+ * on one workspace: prepare partitions, counts and waits, leaving in
+ * host_out (pinned int64[3]) [0] the pair count (-1 for multi = 2), [1] the
+ * largest build bucket, [2] the work items; write fills out_probe/out_build
+ * (int64, at least the pair count rows; np rows for multi = 2) and gathers
+ * npay (<= 2) int64 build payload columns by build row into pay_out.  With
+ * multi 0 or 1 pairs come in a deterministic order (bucket, then probe row
+ * in partition order, then build row ascending) and write does not wait
+ * (host_out may be NULL); with multi = 2 each round of a work item takes its
+ * output rows with an atomic add, so pairs come grouped by round in no
+ * fixed order, and write waits once and leaves the pair count in
+ * host_out[0].  Keys are int64; np, nb < 2^31. */
 size_t pf_hash_join_workspace_bytes(int64_t nb, int64_t np, int32_t bits, int32_t multi);
 int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys, int64_t np,
                          int32_t bits, int32_t multi, int64_t* host_out, void* workspace,
                          size_t workspace_bytes, void* stream);
 int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int64_t n_items,
                        const int64_t* const* bpay, int32_t npay, int64_t* out_probe,
-                       int64_t* out_build, int64_t* const* pay_out, void* workspace,
-                       size_t workspace_bytes, void* stream);
+                       int64_t* out_build, int64_t* const* pay_out, int64_t* host_out,
+                       void* workspace, size_t workspace_bytes, void* stream);
 
 /* --------------------------------------------------- keyed group-by --
  * Replaces the grouping of a keyed aggregation's pass 1

@@ -33,6 +33,10 @@
 namespace pf {
 
 size_t radix_partition_ws(int64_t n, int bits);
+size_t radix_level_ws(int64_t n, int bits);
+int radix_partition_level(const int64_t* keys, int64_t n, int bits, uint64_t* tk, int32_t* tv,
+                          uint64_t* h_out, int32_t* row_out, int64_t* boff, void* ws,
+                          size_t ws_bytes, hipStream_t st, int who);
 int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
                     int shift, int mode, void* keys_out, int32_t* vals_out,
                     int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st,
@@ -79,6 +83,7 @@ struct HjParams {
   int64_t n_items;
   int64_t* item_cnt;      // COUNT: pairs per item
   const int64_t* item_pos;  // WRITE: each item's first output row
+  int64_t* next_out;        // WRITE in any order: the next free output row
   int64_t* out_probe;
   int64_t* out_build;
   const int64_t* pay_src[MAX_PAY];
@@ -266,7 +271,10 @@ __device__ __forceinline__ void lds_insert(const uint64_t* keys, uint32_t* word,
   }
 }
 
-// MODE 0: count pairs per item; MODE 1: write them.  Probe rows are taken
+// MODE 0: count pairs per item; MODE 1: write them at the scanned item
+// positions; MODE 2 (unique builds): write them without a count pass, each
+// round of a workgroup taking its output rows with one atomic add on a
+// counter, so rounds land in the order they finish.  Probe rows are taken
 // in rounds of HJ_ROUND, item row i0 + k * HJ_NT + tid in slot k of a thread, so a
 // wave's lanes touch consecutive rows: loads coalesce and, in the write
 // pass, consecutive matches land on consecutive output rows.  A match's
@@ -276,7 +284,9 @@ static_assert(HJ_IT * HJ_W % kWave == 0, "one wave scans a round's (slot, wave) 
 template <bool MULTI, int MODE>
 __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
   constexpr int CAP = Lds<MULTI>::CAP, TS = Lds<MULTI>::TS;
-  constexpr bool W1 = MODE == 1;
+  constexpr bool W1 = MODE >= 1;
+  constexpr bool ANY = MODE == 2;
+  static_assert(!(ANY && MULTI), "the any-order write is for unique builds");
   // rows a thread per round (the count pass stays within 64 VGPRs: two
   // 1024-thread workgroups a CU)
   constexpr int IT = W1 ? HJ_IT : 8;
@@ -293,7 +303,8 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
   if (item >= p.n_items) return;
   const Item it = item_of(p, item);
   int64_t pos = 0;
-  if (W1) pos = p.item_pos[item];
+  if (W1 && !ANY) pos = p.item_pos[item];
+  __shared__ int64_t round_base;
   int64_t count = 0;
   const bool has_pay = W1 && p.npay > 0;
   // a bucket larger than an LDS chunk is joined one build chunk at a time
@@ -389,8 +400,14 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
           cnt[lane * PER + q] = ex;
           ex += v[q];
         }
+        // the last lane holds the round's total: take that many output rows
+        if (ANY && lane == kWave - 1)
+          round_base = ex ? (int64_t)atomicAdd((unsigned long long*)p.next_out,
+                                               (unsigned long long)ex)
+                          : 0;
       }
       __syncthreads();
+      if (ANY) pos = round_base;
       if (MULTI) {
         // A wave writes its (round slot)'s pairs as one contiguous run: lane
         // j of each store takes output j of the run, found by a binary
@@ -519,6 +536,14 @@ struct HjWs {
   size_t bytes;
 };
 
+// The partition workspace of both sides.
+size_t part_ws_bytes(int64_t nb, int64_t np, int bits) {
+  const int64_t n = nb > np ? nb : np;
+  if (bits <= kOnePassBits) return radix_partition_ws(n, bits);
+  const size_t a = radix_level_ws(nb, bits), b = radix_level_ws(np, bits);
+  return a > b ? a : b;
+}
+
 // Upper bound of work items: every bucket's probe chunks.
 int64_t max_items(int64_t np, int bits, bool multi) {
   return np / pch_of(multi) + (1LL << bits) + 1;
@@ -547,7 +572,7 @@ HjWs carve_hj(void* ws, int64_t nb, int64_t np, int bits, bool multi) {
   w.item_cnt = cv.take<int64_t>(mi);
   w.sort_off = cv.take<int64_t>(multi ? 257 : 1);
   for (int c = 0; c < MAX_PAY; ++c) w.pay_part[c] = cv.take<int64_t>(nb > 0 ? nb : 1);
-  size_t pws = radix_partition_ws(nb > np ? nb : np, pass_bits(bits));
+  size_t pws = part_ws_bytes(nb, np, bits);
   if (multi && radix_partition_ws(nb, kSortDigit) > pws) pws = radix_partition_ws(nb, kSortDigit);
   w.part_ws = cv.take<char>(pws);
   w.scan = cv.take<char>(scan_i64_scratch(mi));
@@ -556,25 +581,18 @@ HjWs carve_hj(void* ws, int64_t nb, int64_t np, int bits, bool multi) {
 }
 
 // One side as (mix64(key), row id) ordered by the top `bits` bits of the
-// hash, stable.  Up to kOnePassBits in one pass; above, two LSD passes of
+// hash, stable.  Up to kOnePassBits in one pass; above, the two-level
+// partition: one histogram of the whole digit, then two LSD passes of
 // about half the bits each (the lower digit first, then the upper one over
-// the hashes), so that every pass writes runs long enough to coalesce, then
-// the bucket bounds by binary search over the sorted hashes.
+// the hashes), so that every pass writes runs long enough to coalesce; the
+// bucket bounds come from its offsets.
 int partition_side(const int64_t* keys, int64_t n, int bits, uint64_t* h, int32_t* row,
                    int64_t* boff, uint64_t* th, int32_t* trow, void* pws, size_t pws_bytes,
                    hipStream_t st, int who) {
   if (bits <= kOnePassBits)
     return radix_partition(keys, 8, nullptr, n, bits, 0, 1, h, row, boff, pws, pws_bytes, st,
                            who, -1);
-  const int hi = bits / 2, lo = bits - hi;
-  if (radix_partition(keys, 8, nullptr, n, lo, hi, 1, th, trow, boff, pws, pws_bytes, st, who,
-                      -1))
-    return 1;
-  if (radix_partition(th, 8, trow, n, hi, 64 - hi, 2, h, row, boff, pws, pws_bytes, st, who,
-                      -1))
-    return 1;
-  hj_bucket_bounds<<<grid_for((1 << bits) + 1, 256), 256, 0, st>>>(h, n, bits, boff);
-  return launch_check("hash_join bounds");
+  return radix_partition_level(keys, n, bits, th, trow, h, row, boff, pws, pws_bytes, st, who);
 }
 
 // Multi-match build buckets of more than SORT_CAP rows, sorted by (hash,
@@ -657,7 +675,7 @@ extern "C" {
 size_t pf_hash_join_workspace_bytes(int64_t nb, int64_t np, int32_t bits, int32_t multi) {
   if (bits < 0) bits = 0;
   if (bits > 12) bits = 12;
-  return pf::carve_hj(nullptr, nb, np, bits, multi != 0).bytes;
+  return pf::carve_hj(nullptr, nb, np, bits, multi == 1).bytes;
 }
 
 int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys, int64_t np,
@@ -668,13 +686,16 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
   if (bits < 0 || bits > 12) return fail("hash_join: bits must be in [0, 12]");
   if (nb < 0 || np < 0 || nb >= (1LL << 31) || np >= (1LL << 31))
     return fail("hash_join: a side takes fewer than 2^31 rows");
+  if (multi < 0 || multi > 2) return fail("hash_join: multi is 0, 1 or 2");
+  const bool any_order = multi == 2;
+  multi = multi == 1;
   HjWs w = carve_hj(workspace, nb, np, bits, multi != 0);
   if (workspace_bytes < w.bytes) return fail("hash_join: workspace too small");
   void* hdev = nullptr;
   if (host_alias(host_out, &hdev)) return 1;
   const int P = 1 << bits;
   const int cap = multi ? CAP_M : CAP_U;
-  const size_t pws = radix_partition_ws(nb > np ? nb : np, pass_bits(bits));
+  const size_t pws = part_ws_bytes(nb, np, bits);
   // both sides as (mix64(key), row) by the top `bits` bits of the hash
   if (partition_side(bkeys, nb, bits, w.bh, w.brow, w.boff, w.th, w.trow, w.part_ws, pws, st, 1))
     return 1;
@@ -706,6 +727,13 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
   }
     if (launch_check("hash_join sort")) return 1;
   }
+  if (any_order) {
+    // no count pass: the write pass finds the pairs (at most one a probe row)
+    host_out[0] = -1;
+    host_out[1] = largest;
+    host_out[2] = n_items;
+    return ok();
+  }
   HjParams p = params_of(w, bits, multi != 0, n_items);
   if (n_items > 0) {
     {
@@ -731,15 +759,29 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
 
 int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int64_t n_items,
                        const int64_t* const* bpay, int32_t npay, int64_t* out_probe,
-                       int64_t* out_build, int64_t* const* pay_out, void* workspace,
-                       size_t workspace_bytes, void* stream) {
+                       int64_t* out_build, int64_t* const* pay_out, int64_t* host_out,
+                       void* workspace, size_t workspace_bytes, void* stream) {
   using namespace pf;
   hipStream_t st = (hipStream_t)stream;
   if (npay < 0 || npay > MAX_PAY) return fail("hash_join: at most 2 payload columns");
+  if (multi < 0 || multi > 2) return fail("hash_join: multi is 0, 1 or 2");
+  const bool any_order = multi == 2;
+  multi = multi == 1;
   HjWs w = carve_hj(workspace, nb, np, bits, multi != 0);
   if (workspace_bytes < w.bytes) return fail("hash_join: workspace too small");
+  void* hdev = nullptr;
+  if (any_order) {
+    if (!host_out) return fail("hash_join: the any-order write reads its pair count back");
+    if (host_alias(host_out, &hdev)) return 1;
+    if (n_items <= 0) {
+      host_out[0] = 0;
+      return ok();
+    }
+    if (fill_i64(w.total, 1, 0, st)) return 1;
+  }
   if (n_items <= 0) return ok();
   HjParams p = params_of(w, bits, multi != 0, n_items);
+  p.next_out = w.total;
   p.out_probe = out_probe;
   p.out_build = out_build;
   p.npay = npay;
@@ -756,11 +798,18 @@ int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int6
   {
     KTimer _kt("hj_write", st);
     if (multi)
-    hj_probe<true, 1><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
-  else
-    hj_probe<false, 1><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
+      hj_probe<true, 1><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
+    else if (any_order)
+      hj_probe<false, 2><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
+    else
+      hj_probe<false, 1><<<(unsigned)n_items, HJ_NT, 0, st>>>(p);
   }
   if (launch_check("hash_join write")) return 1;
+  if (any_order) {
+    hj_readback<<<1, 64, 0, st>>>(w.total, nullptr, (int64_t*)hdev);
+    if (launch_check("hash_join write")) return 1;
+    if (wait_stream(st, "hash_join write")) return 1;
+  }
   return ok();
 }
 

@@ -77,7 +77,10 @@ struct RpShape<12> {
   static constexpr int NT = 512, IT = 8;
 };
 
-template <typename K, typename KO, int MODE, int LOGB>
+// A chunk's histogram of the digit into hist: bucket-major
+// (hist[b * nchunks + chunk], the layout the offsets scan reads) or, with
+// CHUNK_MAJOR, the chunk's counts contiguous (hist[chunk << bits | b]).
+template <typename K, typename KO, int MODE, int LOGB, bool CHUNK_MAJOR = false>
 __device__ __forceinline__ void rp_hist_body(const K* __restrict__ keys, int64_t n,
                                              int64_t crows, int bits, int shift, int64_t nchunks,
                                              int32_t* __restrict__ hist, int64_t chunk) {
@@ -88,20 +91,33 @@ __device__ __forceinline__ void rp_hist_body(const K* __restrict__ keys, int64_t
   __syncthreads();
   const int64_t c0 = chunk * crows;
   const int64_t c1 = c0 + crows < n ? c0 + crows : n;
-  for (int64_t i = c0 + threadIdx.x; i < c1; i += NT) {
-    KO o;
-    atomicAdd(&h[rp_digit<K, KO, MODE>(keys[i], bits, shift, o)], 1);
+  // U keys a thread in flight before their atomics
+  constexpr int U = 8;
+  for (int64_t b0 = c0 + threadIdx.x; b0 < c1; b0 += (int64_t)NT * U) {
+    K k[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = b0 + (int64_t)u * NT;
+      k[u] = i < c1 ? keys[i] : (K)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      KO o;
+      if (b0 + (int64_t)u * NT < c1) atomicAdd(&h[rp_digit<K, KO, MODE>(k[u], bits, shift, o)], 1);
+    }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < B; b += NT) hist[(int64_t)b * nchunks + chunk] = h[b];
+  for (int b = threadIdx.x; b < B; b += NT)
+    hist[CHUNK_MAJOR ? (chunk << bits) + b : (int64_t)b * nchunks + chunk] = h[b];
 }
 
-template <typename K, typename KO, int MODE, int LOGB>
+template <typename K, typename KO, int MODE, int LOGB, bool CHUNK_MAJOR = false>
 __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_hist(const K* __restrict__ keys,
                                                             int64_t n, int64_t crows, int bits,
                                                             int shift, int64_t nchunks,
                                                             int32_t* __restrict__ hist) {
-  rp_hist_body<K, KO, MODE, LOGB>(keys, n, crows, bits, shift, nchunks, hist, blockIdx.x);
+  rp_hist_body<K, KO, MODE, LOGB, CHUNK_MAJOR>(keys, n, crows, bits, shift, nchunks, hist,
+                                               blockIdx.x);
 }
 
 // The segment a block works for: the last whose first block is <= it.
@@ -271,11 +287,13 @@ __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_scatter(
 // loaded while the current one is ranked (registers double-buffered), each
 // bucket owned by one thread across the per-tile bookkeeping (wave prefix,
 // tile scan, global base, cursor), and the tile's bucket ids staged as bytes.
+// Rows [c0, c1) of `keys`; bucket b's first output row is
+// offs_col[b * ostride] + delta.
 template <typename K, typename KO, int MODE, int NT, int IT>
 __device__ __forceinline__ void rp_scatter8_body(
-    const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t n, int64_t crows,
-    int bits, int shift, int64_t nchunks, const int64_t* __restrict__ offs,
-    KO* __restrict__ kout, int32_t* __restrict__ vout, int64_t chunk, int64_t delta) {
+    const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t c0, int64_t c1,
+    int bits, int shift, const int64_t* __restrict__ offs_col, int64_t ostride,
+    KO* __restrict__ kout, int32_t* __restrict__ vout, int64_t delta) {
   constexpr int W = NT / kWave, TILE = NT * IT, BMAX = 256, WROWS = kWave * IT;
   static_assert(BMAX <= NT, "a thread owns each bucket");
   __shared__ uint16_t wh[W * BMAX];
@@ -288,27 +306,28 @@ __device__ __forceinline__ void rp_scatter8_body(
 
   const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   const int B = 1 << bits;
-  const int64_t c0 = chunk * crows;
-  const int64_t c1 = c0 + crows < n ? c0 + crows : n;
-  uint32_t cursor = tid < B ? (uint32_t)(offs[(int64_t)tid * nchunks + chunk] + delta) : 0u;
+  uint32_t cursor = tid < B ? (uint32_t)(offs_col[(int64_t)tid * ostride] + delta) : 0u;
   for (int x = tid; x < W * B; x += NT) wh[x] = 0;
 
   K kn[IT];
   int32_t vn[IT];
+  // tiles start on a 64-row boundary (a wave's loads cover whole lines);
+  // rows before c0 are loaded by no lane and ranked as no row
+  const int64_t a0 = c0 & ~(int64_t)(kWave - 1);
   auto load_tile = [&](int64_t t0) {
     const int64_t seg = t0 + (int64_t)w * WROWS;
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
       const int64_t i = seg + k * kWave + lane;
-      if (i < c1) {
+      if (i >= c0 && i < c1) {
         kn[k] = keys[i];
         vn[k] = vals ? vals[i] : (int32_t)i;
       }
     }
   };
-  load_tile(c0);
+  load_tile(a0);
   __syncthreads();
-  for (int64_t t0 = c0; t0 < c1; t0 += TILE) {
+  for (int64_t t0 = a0; t0 < c1; t0 += TILE) {
     K kc[IT];
     int32_t vc[IT];
 #pragma unroll
@@ -323,7 +342,7 @@ __device__ __forceinline__ void rp_scatter8_body(
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
       const int64_t i = seg + k * kWave + lane;
-      d[k] = i < c1 ? rp_digit<K, KO, MODE>(kc[k], bits, shift, ov[k]) : ~0u;
+      d[k] = (i >= c0 && i < c1) ? rp_digit<K, KO, MODE>(kc[k], bits, shift, ov[k]) : ~0u;
     }
     int rk[IT];
 #pragma unroll
@@ -373,8 +392,10 @@ __device__ __forceinline__ void rp_scatter8_body(
         sb[pos] = (uint8_t)d[k];
       }
     }
-    const int64_t left = c1 - t0;
-    const int tvalid = left < TILE ? (int)left : TILE;
+    // the tile's rows: those of [t0, t0 + TILE) inside [c0, c1)
+    const int64_t lo_row = t0 > c0 ? t0 : c0;
+    const int64_t hi_row = t0 + TILE < c1 ? t0 + TILE : c1;
+    const int tvalid = (int)(hi_row - lo_row);
     __syncthreads();
     for (int s = tid; s < tvalid; s += NT) {
       const uint32_t o = gb[sb[s]] + (uint32_t)s;
@@ -391,8 +412,9 @@ __global__ __launch_bounds__(NT) void rp_scatter8(
     const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t n, int64_t crows,
     int bits, int shift, int64_t nchunks, const int64_t* __restrict__ offs,
     KO* __restrict__ kout, int32_t* __restrict__ vout) {
-  rp_scatter8_body<K, KO, MODE, NT, IT>(keys, vals, n, crows, bits, shift, nchunks, offs, kout,
-                                        vout, blockIdx.x, 0);
+  const int64_t c0 = (int64_t)blockIdx.x * crows;
+  rp_scatter8_body<K, KO, MODE, NT, IT>(keys, vals, c0, c0 + crows < n ? c0 + crows : n, bits,
+                                        shift, offs + blockIdx.x, nchunks, kout, vout, 0);
 }
 
 template <int NT, int IT>
@@ -402,9 +424,11 @@ __global__ __launch_bounds__(NT) void rp_scatter8_seg(const int32_t* __restrict_
                                                       int32_t* __restrict__ kout,
                                                       int32_t* __restrict__ vout) {
   const RpSeg& g = segs[seg_of_block(segs, nseg)];
+  const int64_t chunk = (int64_t)blockIdx.x - g.blk_start;
+  const int64_t c0 = chunk * kSegChunkRows;
   rp_scatter8_body<int32_t, int32_t, RP_HASH_KEY, NT, IT>(
-      keys + g.in_off, nullptr, g.n, kSegChunkRows, g.bits, g.shift, g.nchunks,
-      offs + g.hist_off, kout, vout, (int64_t)blockIdx.x - g.blk_start, g.delta);
+      keys + g.in_off, nullptr, c0, c0 + kSegChunkRows < g.n ? c0 + kSegChunkRows : g.n, g.bits,
+      g.shift, offs + g.hist_off + chunk, g.nchunks, kout, vout, g.delta);
 }
 
 __global__ void rp_offsets(const int64_t* __restrict__ offs, int64_t nchunks, int B, int64_t n,
@@ -462,6 +486,10 @@ const char* const kHistName[6] = {"rp_hist", "hj_build_hist", "hj_probe_hist", "
                                   "hj_bigsort_hist", "gk_sort_hist"};
 const char* const kScatterName[6] = {"rp_scatter", "hj_build_scatter", "hj_probe_scatter",
                                      "zc_sort_scatter", "hj_bigsort_scatter", "gk_sort_scatter"};
+// the second pass of a two-level partition
+const char* const kScatter2Name[6] = {"rp_scatter_2", "hj_build_scatter_2", "hj_probe_scatter_2",
+                                      "zc_sort_scatter_2", "hj_bigsort_scatter_2",
+                                      "gk_sort_scatter_2"};
 
 template <typename K, typename KO, int MODE, int NT, int IT>
 void launch8(const K* keys, const int32_t* vals, int64_t n, int64_t crows, int bits, int shift,
@@ -624,6 +652,215 @@ int two_pass(const void* keys, int key_bytes, const int32_t* vals, int64_t n, in
   return launch_bounds_of<int64_t, RP_HASH_OUT>(keys_out, n, bits, shift, offs, st);
 }
 
+// ------------------------------------------------- two-level partition --
+//
+// A digit of 9-12 bits in two stable passes (the low part L of the digit,
+// then the high part H), both fed by ONE histogram read of the keys: each
+// chunk's histogram of the whole digit.  The first pass's offsets are its
+// sums over H; the second pass works on units of first-pass output (the
+// rows of one L from a group of consecutive chunks), so a unit's count of
+// each H, and with them its output offsets, are sums of the same
+// histogram.  The second pass reads no histogram of its own, and the
+// bucket bounds come from the offsets (no search).
+
+// Chunk rows of the histogram and the first pass: about 2048 chunks,
+// a whole number of 8192-row tiles, 8192 to 131072 rows.
+#ifndef PF_LEVEL_CHUNK_MAX
+#define PF_LEVEL_CHUNK_MAX 131072
+#endif
+int64_t level_chunk_rows(int64_t n) {
+  int64_t c = (n / 2048 + 8191) / 8192 * 8192;
+  return c < 8192 ? 8192 : (c > PF_LEVEL_CHUNK_MAX ? PF_LEVEL_CHUNK_MAX : c);
+}
+
+struct LvPlan {
+  int lo, hi;          // digit bits of the two passes
+  int64_t crows, nc;   // chunk rows, chunks
+  int group;           // chunks a second-pass unit spans
+  int64_t ng;          // chunk groups
+};
+
+LvPlan level_plan(int64_t n, int bits) {
+  LvPlan p;
+  p.hi = bits / 2;
+  p.lo = bits - p.hi;
+  p.crows = level_chunk_rows(n);
+  p.nc = n > 0 ? (n + p.crows - 1) / p.crows : 1;
+  // about 32768 rows a unit: as long a run a bucket as a one-pass chunk has
+  int64_t grp = (32768LL << p.lo) / p.crows;
+  p.group = (int)(grp < 1 ? 1 : grp);
+  p.ng = (p.nc + p.group - 1) / p.group;
+  return p;
+}
+
+struct LvWs {
+  int32_t* J;      // [chunk][digit]
+  int32_t* cnt1;   // [L][chunk]
+  int64_t* offs1;
+  int32_t* cnt2;   // [digit][group]
+  int64_t* offs2;
+  void* scan;
+  size_t bytes;
+};
+
+LvWs carve_lv(void* ws, int64_t n, int bits) {
+  const LvPlan p = level_plan(n, bits);
+  Carve cv(ws);
+  LvWs w;
+  const int64_t e1 = (1LL << p.lo) * p.nc, e2 = (1LL << bits) * p.ng;
+  w.J = cv.take<int32_t>((1LL << bits) * p.nc);
+  w.cnt1 = cv.take<int32_t>(e1);
+  w.offs1 = cv.take<int64_t>(e1);
+  w.cnt2 = cv.take<int32_t>(e2);
+  w.offs2 = cv.take<int64_t>(e2);
+  w.scan = cv.take<char>(scan_i32_scratch(e1 > e2 ? e1 : e2));
+  w.bytes = cv.bytes();
+  return w;
+}
+
+// J[c << bits | d]: chunk c's rows of digit d (d = H << lo | L).
+// cnt1[L * nc + c]: chunk c's rows of low part L; cnt2[d * ng + g]: group
+// g's rows of digit d.
+__global__ void rp_level_counts(const int32_t* __restrict__ J, int64_t nc, int lo, int hi,
+                                int64_t ng, int group, int32_t* __restrict__ cnt1,
+                                int32_t* __restrict__ cnt2) {
+  const int64_t n1 = (1LL << lo) * nc, n2 = (1LL << (lo + hi)) * ng;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n1 + n2; x += stride) {
+    // consecutive threads read consecutive counters of J
+    int32_t s = 0;
+    if (x < n1) {
+      const int64_t c = x >> lo, L = x & ((1LL << lo) - 1);
+      const int32_t* row = J + (c << (lo + hi));
+      for (int64_t H = 0; H < (1LL << hi); ++H) s += row[(H << lo) | L];
+      cnt1[L * nc + c] = s;
+    } else {
+      const int64_t y = x - n1, g = y >> (lo + hi), d = y & ((1LL << (lo + hi)) - 1);
+      const int64_t a = g * group, b = a + group < nc ? a + group : nc;
+      for (int64_t c = a; c < b; ++c) s += J[(c << (lo + hi)) + d];
+      cnt2[d * ng + g] = s;
+    }
+  }
+}
+
+// Second pass: unit u = (L, g) is the first-pass output rows of low part
+// L from chunks [g * group, (g + 1) * group); its bucket H starts at
+// offs2[(H << lo | L) * ng + g].
+template <typename K, typename KO, int MODE, int NT, int IT>
+__global__ __launch_bounds__(NT) void rp_scatter8_units(
+    const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t n, int lo, int64_t nc,
+    int64_t ng, int group, const int64_t* __restrict__ offs1, int bits, int shift,
+    const int64_t* __restrict__ offs2, KO* __restrict__ kout, int32_t* __restrict__ vout) {
+  const int64_t u = blockIdx.x;
+  const int64_t L = u / ng, g = u - L * ng;
+  const int64_t c0 = offs1[L * nc + g * group];
+  const int64_t c1 = (g + 1) * group < nc ? offs1[L * nc + (g + 1) * group]
+                                          : (L + 1 < (1LL << lo) ? offs1[(L + 1) * nc] : n);
+  rp_scatter8_body<K, KO, MODE, NT, IT>(keys, vals, c0, c1, bits, shift, offs2 + u,
+                                        (1LL << lo) * ng, kout, vout, 0);
+}
+
+__global__ void rp_level_offsets(const int64_t* __restrict__ offs2, int64_t ng, int P,
+                                 int64_t n, int64_t* __restrict__ boff) {
+  for (int d = blockIdx.x * blockDim.x + threadIdx.x; d <= P; d += gridDim.x * blockDim.x)
+    boff[d] = d < P ? offs2[(int64_t)d * ng] : n;
+}
+
+template <typename K, typename KO, int MODE, int NT, int IT>
+void launch_units(const K* keys, const int32_t* vals, int64_t n, const LvPlan& p,
+                  const int64_t* offs1, int bits, int shift, const int64_t* offs2, KO* kout,
+                  int32_t* vout, hipStream_t st) {
+  rp_scatter8_units<K, KO, MODE, NT, IT><<<(unsigned)((1LL << p.lo) * p.ng), NT, 0, st>>>(
+      keys, vals, n, p.lo, p.nc, p.ng, p.group, offs1, bits, shift, offs2, kout, vout);
+}
+
+template <typename K, typename KO, int MODE>
+void units_by_shape(int shape, const K* keys, const int32_t* vals, int64_t n, const LvPlan& p,
+                    const int64_t* offs1, int bits, int shift, const int64_t* offs2, KO* kout,
+                    int32_t* vout, hipStream_t st) {
+  switch (shape) {
+    case 0: launch_units<K, KO, MODE, 256, 8>(keys, vals, n, p, offs1, bits, shift, offs2, kout, vout, st); break;
+    case 1: launch_units<K, KO, MODE, 256, 16>(keys, vals, n, p, offs1, bits, shift, offs2, kout, vout, st); break;
+    case 2: launch_units<K, KO, MODE, 512, 8>(keys, vals, n, p, offs1, bits, shift, offs2, kout, vout, st); break;
+    case 3: launch_units<K, KO, MODE, 512, 16>(keys, vals, n, p, offs1, bits, shift, offs2, kout, vout, st); break;
+    default: launch_units<K, KO, MODE, 1024, 8>(keys, vals, n, p, offs1, bits, shift, offs2, kout, vout, st); break;
+  }
+}
+
+template <typename K, typename KO, int MODE>
+void pass1_by_shape(int shape, const K* keys, const int32_t* vals, int64_t n, const LvPlan& p,
+                    int shift, const int64_t* offs1, KO* kout, int32_t* vout, hipStream_t st) {
+  switch (shape) {
+    case 0: launch8<K, KO, MODE, 256, 8>(keys, vals, n, p.crows, p.lo, shift, p.nc, offs1, kout, vout, st); break;
+    case 1: launch8<K, KO, MODE, 256, 16>(keys, vals, n, p.crows, p.lo, shift, p.nc, offs1, kout, vout, st); break;
+    case 2: launch8<K, KO, MODE, 512, 8>(keys, vals, n, p.crows, p.lo, shift, p.nc, offs1, kout, vout, st); break;
+    case 3: launch8<K, KO, MODE, 512, 16>(keys, vals, n, p.crows, p.lo, shift, p.nc, offs1, kout, vout, st); break;
+    default: launch8<K, KO, MODE, 1024, 8>(keys, vals, n, p.crows, p.lo, shift, p.nc, offs1, kout, vout, st); break;
+  }
+}
+
+// The whole two-level partition of n keys by `bits` (9-12) bits of the
+// digit `mode` selects below `shift` (the hashed modes) or from `shift`
+// up (raw); pass 1 writes (tk, tv), pass 2 (keys_out, vals_out).
+template <typename K>
+int level_run(const K* keys, const int32_t* vals, int64_t n, int bits, int shift, int mode,
+              void* tk, int32_t* tv, void* keys_out, int32_t* vals_out, int64_t* boff,
+              const LvWs& w, hipStream_t st, int who, int shape) {
+  const LvPlan p = level_plan(n, bits);
+  const int P = 1 << bits;
+  {
+    KTimer _kt(kHistName[who], st);
+    if (mode == RP_RAW)
+      rp_hist<K, K, RP_RAW, 12, true><<<(unsigned)p.nc, RpShape<12>::NT, 0, st>>>(
+          keys, n, p.crows, bits, shift, p.nc, w.J);
+    else
+      rp_hist<K, K, RP_HASH_KEY, 12, true><<<(unsigned)p.nc, RpShape<12>::NT, 0, st>>>(
+          keys, n, p.crows, bits, shift, p.nc, w.J);
+  }
+  if (launch_check("radix_partition level hist")) return 1;
+  const int64_t e1 = (1LL << p.lo) * p.nc, e2 = (int64_t)P * p.ng;
+  rp_level_counts<<<grid_for(e1 + e2, 256) < 8192 ? grid_for(e1 + e2, 256) : 8192, 256, 0, st>>>(
+      w.J, p.nc, p.lo, p.hi, p.ng, p.group, w.cnt1, w.cnt2);
+  if (launch_check("radix_partition level counts")) return 1;
+  if (scan_i32_to_i64_excl(w.cnt1, w.offs1, e1, nullptr, w.scan, st) ||
+      scan_i32_to_i64_excl(w.cnt2, w.offs2, e2, nullptr, w.scan, st))
+    return 1;
+  if (n > 0) {
+    const int sh1 = shape8_valid(shape) ? shape : default_shape8(n, p.lo);
+    const int sh2 = shape8_valid(shape) ? shape : default_shape8(n, p.hi);
+    {
+      KTimer _kt(kScatterName[who], st);
+      if (mode == RP_RAW)
+        pass1_by_shape<K, K, RP_RAW>(sh1, keys, vals, n, p, shift, w.offs1, (K*)tk, tv, st);
+      else if (mode == RP_HASH_KEY)
+        pass1_by_shape<K, K, RP_HASH_KEY>(sh1, keys, vals, n, p, shift + p.hi, w.offs1, (K*)tk,
+                                          tv, st);
+      else  // HASH_OUT: the first pass writes the hashes, the second reads their bits
+        pass1_by_shape<K, int64_t, RP_HASH_OUT>(sh1, keys, vals, n, p, shift + p.hi, w.offs1,
+                                                (int64_t*)tk, tv, st);
+    }
+    KTimer _kt(kScatter2Name[who], st);
+    if (mode == RP_RAW)
+      units_by_shape<K, K, RP_RAW>(sh2, (const K*)tk, tv, n, p, w.offs1, p.hi, shift + p.lo,
+                                   w.offs2, (K*)keys_out, vals_out, st);
+    else if (mode == RP_HASH_KEY)
+      units_by_shape<K, K, RP_HASH_KEY>(sh2, (const K*)tk, tv, n, p, w.offs1, p.hi, shift,
+                                        w.offs2, (K*)keys_out, vals_out, st);
+    else
+      units_by_shape<int64_t, int64_t, RP_RAW>(sh2, (const int64_t*)tk, tv, n, p, w.offs1, p.hi,
+                                               64 - shift - p.hi, w.offs2,
+                                               (int64_t*)keys_out, vals_out, st);
+  }
+  if (launch_check("radix_partition level scatter")) return 1;
+  rp_level_offsets<<<grid_for(P + 1, 256), 256, 0, st>>>(w.offs2, p.ng, P, n, boff);
+  return launch_check("radix_partition level offsets");
+}
+
+size_t level_ws(int64_t n, int bits) {
+  return align256(carve_lv(nullptr, n, bits).bytes) + align256((size_t)(n > 0 ? n : 1) * 8) +
+         align256((size_t)(n > 0 ? n : 1) * 4);
+}
+
 }  // namespace
 
 // Host-side use by the other tiers of the library.
@@ -651,8 +888,21 @@ int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
 size_t radix_partition_ws(int64_t n, int bits) {
   const size_t one = carve_rp(nullptr, n, bits).bytes;
   if (bits <= kOnePassMax) return one;
-  const size_t two = two_pass_ws(n, bits);
+  const size_t two = level_ws(n, bits);
   return one > two ? one : two;
+}
+
+size_t radix_level_ws(int64_t n, int bits) { return carve_lv(nullptr, n, bits).bytes; }
+
+int radix_partition_level(const int64_t* keys, int64_t n, int bits, uint64_t* tk, int32_t* tv,
+                          uint64_t* h_out, int32_t* row_out, int64_t* boff, void* ws,
+                          size_t ws_bytes, hipStream_t st, int who) {
+  if (bits <= kOnePassMax || bits > 12) return fail("radix_partition_level: 9 to 12 bits");
+  if (n < 0 || n >= (1LL << 31)) return fail("radix_partition_level: n must be below 2^31");
+  LvWs w = carve_lv(ws, n, bits);
+  if (ws_bytes < w.bytes) return fail("radix_partition_level: workspace too small");
+  return level_run<int64_t>(keys, nullptr, n, bits, 0, RP_HASH_OUT, tk, tv, h_out, row_out, boff,
+                            w, st, who, -1);
 }
 
 int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_t n, int bits,
@@ -671,9 +921,17 @@ int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_
   if (mode == RP_HASH_OUT && key_bytes != 8 && keys_out == keys)
     return fail("radix_partition: a hashed output cannot overwrite int32 keys");
   if (bits > kOnePassMax && shape != kShapeOnePass) {
-    if (ws_bytes < two_pass_ws(n, bits)) return fail("radix_partition: workspace too small");
-    return two_pass(keys, key_bytes, vals, n, bits, shift, mode, keys_out, vals_out,
-                    bucket_offsets, ws, st, who, shape);
+    if (ws_bytes < level_ws(n, bits)) return fail("radix_partition: workspace too small");
+    const size_t lb = align256(carve_lv(nullptr, n, bits).bytes);
+    char* base = (char*)ws;
+    void* tk = base + lb;
+    int32_t* tv = (int32_t*)(base + lb + align256((size_t)(n > 0 ? n : 1) * 8));
+    LvWs w = carve_lv(ws, n, bits);
+    if (key_bytes == 8)
+      return level_run<int64_t>((const int64_t*)keys, vals, n, bits, shift, mode, tk, tv,
+                                keys_out, vals_out, bucket_offsets, w, st, who, shape);
+    return level_run<int32_t>((const int32_t*)keys, vals, n, bits, shift, mode, tk, tv,
+                              keys_out, vals_out, bucket_offsets, w, st, who, shape);
   }
   RpWs r = carve_rp(ws, n, bits);
   if (ws_bytes < r.bytes) return fail("radix_partition: workspace too small");

@@ -39,28 +39,39 @@ def _check_keys(t: torch.Tensor, what: str) -> None:
 
 
 class HashJoin:
-    """One join's plan: sizes, bucket bits and the workspace."""
+    """One join's plan: sizes, bucket bits and the workspace.
+
+    ``ordered=False`` (unique builds only) skips the count pass: ``prepare``
+    returns None, the outputs of ``write`` must hold every probe row (a
+    unique build gives a probe row at most one pair), pairs come grouped by
+    the probe kernel's rounds in no fixed order, and ``write`` returns the
+    pair count after one wait."""
 
     def __init__(self, n_build: int, n_probe: int, unique: bool = True,
-                 bits: Optional[int] = None, device=None):
+                 bits: Optional[int] = None, device=None, ordered: bool = True):
         self.nb, self.np = int(n_build), int(n_probe)
         self.unique = bool(unique)
+        self.ordered = bool(ordered) or not self.unique
+        # pf_hash_join_*'s multi: 0 unique, 1 multi-match, 2 unique in any order
+        self.mode = 1 if not self.unique else (0 if self.ordered else 2)
         self.bits = default_bits(self.nb, self.unique) if bits is None else int(bits)
         if not 0 <= self.bits <= MAX_BITS:
             raise ValueError(f"bits must be in [0, {MAX_BITS}]")
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         lib = _lib.load()
         self.ws_bytes = int(lib.pf_hash_join_workspace_bytes(self.nb, self.np, self.bits,
-                                                             int(not self.unique)))
+                                                             self.mode))
         self.ws = _lib.workspace(self.ws_bytes, self.device)
         self.host = torch.empty(3, dtype=torch.int64, pin_memory=True)
         self.n_items = 0
         self.pairs = 0
+        self.prepared = False
 
-    def prepare(self, build_keys: torch.Tensor, probe_keys: torch.Tensor) -> int:
-        """Partitions both sides and counts the pairs (two host waits; three
-        when a multi-match build bucket is larger than the LDS sort takes
-        and is sorted in HBM).  Returns the pair count."""
+    def prepare(self, build_keys: torch.Tensor, probe_keys: torch.Tensor) -> Optional[int]:
+        """Partitions both sides and, when ordered, counts the pairs (two
+        host waits; three when a multi-match build bucket is larger than the
+        LDS sort takes and is sorted in HBM; one when not ordered).  Returns
+        the pair count, or None when not ordered."""
         _check_keys(build_keys, "build keys")
         _check_keys(probe_keys, "probe keys")
         if build_keys.numel() != self.nb or probe_keys.numel() != self.np:
@@ -69,48 +80,63 @@ class HashJoin:
         with torch.cuda.device(self.device):
             _lib.check(lib.pf_hash_join_prepare(
                 _lib.ptr(build_keys), self.nb, _lib.ptr(probe_keys), self.np, self.bits,
-                int(not self.unique), ctypes.c_void_p(self.host.data_ptr()), _lib.ptr(self.ws),
+                self.mode, ctypes.c_void_p(self.host.data_ptr()), _lib.ptr(self.ws),
                 self.ws_bytes, _lib.stream_of(self.device)))
-        self.pairs, _, self.n_items = (int(v) for v in self.host.tolist())
+        pairs, _, self.n_items = (int(v) for v in self.host.tolist())
+        self.prepared = True
+        if not self.ordered:
+            return None
+        self.pairs = pairs
         return self.pairs
 
     def write(self, out_probe: torch.Tensor, out_build: torch.Tensor,
               payloads: Sequence[torch.Tensor] = (),
-              pay_out: Sequence[torch.Tensor] = ()) -> None:
-        """Writes the prepared pairs (and gathered build payloads); no wait."""
-        if self.pairs < 0:
+              pay_out: Sequence[torch.Tensor] = ()) -> int:
+        """Writes the prepared pairs (and gathered build payloads) and
+        returns their count; no wait when ordered, one otherwise."""
+        if not self.prepared:
+            raise RuntimeError("HashJoin: write before prepare")
+        if self.ordered and self.pairs < 0:
             raise RuntimeError("HashJoin: the prepared join took no LDS path")
-        if out_probe.numel() < self.pairs or out_build.numel() < self.pairs:
-            raise ValueError("HashJoin: outputs smaller than the pair count")
+        need = self.pairs if self.ordered else self.np
+        if out_probe.numel() < need or out_build.numel() < need:
+            raise ValueError(f"HashJoin: outputs of {need} rows are needed")
         if len(payloads) != len(pay_out) or len(payloads) > 2:
             raise ValueError("HashJoin: at most two payload columns, one output each")
         for t in payloads:
             if t.dtype != torch.int64 or t.numel() != self.nb or not t.is_contiguous():
                 raise ValueError("HashJoin: payloads are contiguous int64 build columns")
         for t in pay_out:
-            if t.dtype != torch.int64 or t.numel() < self.pairs:
-                raise ValueError("HashJoin: payload outputs are int64 of the pair count")
+            if t.dtype != torch.int64 or t.numel() < need:
+                raise ValueError(f"HashJoin: payload outputs are int64 of {need} rows")
         lib = _lib.load()
         with torch.cuda.device(self.device):
             _lib.check(lib.pf_hash_join_write(
-                self.nb, self.np, self.bits, int(not self.unique), self.n_items,
+                self.nb, self.np, self.bits, self.mode, self.n_items,
                 _lib.void_array([t.data_ptr() for t in payloads]), len(payloads),
                 _lib.ptr(out_probe), _lib.ptr(out_build),
-                _lib.void_array([t.data_ptr() for t in pay_out]), _lib.ptr(self.ws),
-                self.ws_bytes, _lib.stream_of(self.device)))
+                _lib.void_array([t.data_ptr() for t in pay_out]),
+                ctypes.c_void_p(self.host.data_ptr()), _lib.ptr(self.ws), self.ws_bytes,
+                _lib.stream_of(self.device)))
+        if not self.ordered:
+            self.pairs = int(self.host[0])
+        return self.pairs
 
 
 def hash_join(build_keys: torch.Tensor, probe_keys: torch.Tensor,
               payloads: Sequence[torch.Tensor] = (), unique: bool = True,
-              bits: Optional[int] = None
+              bits: Optional[int] = None, ordered: bool = True
               ) -> Tuple[torch.Tensor, torch.Tensor, List[torch.Tensor]]:
     """(probe rows, build rows, gathered build payloads) of an inner
-    equi-join of int64 keys, in the deterministic partition order."""
-    plan = HashJoin(build_keys.numel(), probe_keys.numel(), unique, bits, build_keys.device)
+    equi-join of int64 keys, in the deterministic partition order (or, for
+    a unique build with ``ordered=False``, in no fixed order)."""
+    plan = HashJoin(build_keys.numel(), probe_keys.numel(), unique, bits, build_keys.device,
+                    ordered=ordered)
     n = plan.prepare(build_keys, probe_keys)
     dev = build_keys.device
-    out_p = torch.empty(n, dtype=torch.int64, device=dev)
-    out_b = torch.empty(n, dtype=torch.int64, device=dev)
-    outs = [torch.empty(n, dtype=torch.int64, device=dev) for _ in payloads]
-    plan.write(out_p, out_b, list(payloads), outs)
-    return out_p, out_b, outs
+    rows = n if n is not None else probe_keys.numel()
+    out_p = torch.empty(rows, dtype=torch.int64, device=dev)
+    out_b = torch.empty(rows, dtype=torch.int64, device=dev)
+    outs = [torch.empty(rows, dtype=torch.int64, device=dev) for _ in payloads]
+    n = plan.write(out_p, out_b, list(payloads), outs)
+    return out_p[:n], out_b[:n], [t[:n] for t in outs]

@@ -50,17 +50,21 @@ def c2():
     torch.cuda.empty_cache()
 
 
-def test_c2_large_join_at_full_size(c2):
+@pytest.mark.parametrize("ordered", [True, False])
+def test_c2_large_join_at_full_size(c2, ordered):
     build, pays, probe, expect = c2
     nb, np_ = build.numel(), probe.numel()
-    say(f"C2 join {nb} x {np_}")
-    plan = HashJoin(nb, np_, unique=True, device=build.device)
+    say(f"C2 join {nb} x {np_} ordered={ordered}")
+    plan = HashJoin(nb, np_, unique=True, device=build.device, ordered=ordered)
     n = plan.prepare(build, probe)
-    assert n == expect
-    op = torch.empty(n, dtype=torch.int64, device=build.device)
+    assert n == (expect if ordered else None)
+    rows = n if ordered else np_
+    op = torch.empty(rows, dtype=torch.int64, device=build.device)
     ob = torch.empty_like(op)
     pay = torch.empty_like(op)
-    plan.write(op, ob, [pays[0]], [pay])
+    n = plan.write(op, ob, [pays[0]], [pay])
+    assert n == expect
+    op, ob, pay = op[:n], ob[:n], pay[:n]
     torch.cuda.synchronize()
     del plan
     say("C2 properties")

@@ -25,11 +25,11 @@ def _sorted_pairs(probe_idx, build_idx):
     return pairs
 
 
-def _check(bk, pk, unique, bits=None, pay=True):
+def _check(bk, pk, unique, bits=None, pay=True, ordered=True):
     g = torch.Generator().manual_seed(len(bk))
     payload = torch.randint(-(1 << 60), 1 << 60, (len(bk),), generator=g, dtype=torch.int64)
     res = hash_join(torch.as_tensor(bk).cuda(), torch.as_tensor(pk).cuda(),
-                    [payload.cuda()] if pay else [], unique=unique, bits=bits)
+                    [payload.cuda()] if pay else [], unique=unique, bits=bits, ordered=ordered)
     assert res is not None
     pi, bi, outs = res
     got = _sorted_pairs(pi, bi)
@@ -46,13 +46,24 @@ def _check(bk, pk, unique, bits=None, pay=True):
     return pi, bi
 
 
+@pytest.mark.parametrize("ordered", [True, False])
 @pytest.mark.parametrize("bits", [0, 4, 8, 12])
-def test_unique_random(bits):
+def test_unique_random(bits, ordered):
     rng = np.random.default_rng(bits)
     bk = rng.permutation(200_000).astype(np.int64) * 7 - 5
     pk = bk[rng.integers(0, bk.size, 1_000_000)]
     pk[rng.random(pk.size) < 0.1] = 1 << 50
-    _check(bk, pk, True, bits)
+    _check(bk, pk, True, bits, ordered=ordered)
+
+
+def test_unique_any_order_edges_and_oversized_buckets():
+    for bk, pk in (([], [1, 2]), ([1, 2], []), ([1, 2], [3, 4]), ([5], [5, 5, 5])):
+        _check(np.array(bk, np.int64), np.array(pk, np.int64), True, pay=False, ordered=False)
+    # a build bucket past one LDS table (bits 0: one bucket of 20K rows)
+    rng = np.random.default_rng(77)
+    bk = rng.permutation(20_000).astype(np.int64)
+    pk = rng.integers(-100, 20_100, 300_000)
+    _check(bk, pk, True, bits=0, ordered=False)
 
 
 @pytest.mark.parametrize("bits,nb", [(0, 8000), (6, 150_000), (12, 150_000)])
