@@ -637,6 +637,7 @@ def bench_serve(args, dev, emit: bool = True):
     for r, res in zip(wl.requests, opened):
         assert int(res.cnt.sum()) == r.fk.numel(), r.rid
     lat = serving.latency_summary(opened)
+    pressure = serve_under_pressure(args, dev, wl, closed, eng)
     ss_ms, ss_n = prof.get("serve_step", (0.0, 0))
     step_bytes = 12.0 * fact_rows / max(steps, 1)
     achieved = step_bytes / (ss_ms / max(ss_n, 1) * 1e-3) / 1e9 if ss_n else None
@@ -649,7 +650,8 @@ def bench_serve(args, dev, emit: bool = True):
            "hbm_budget_MiB": args.serve_budget, "peak_engine_MiB": eng.peak_used / 2**20,
            "serve_step_avg_ms": ss_ms / max(ss_n, 1),
            "serve_step_GBps": achieved,
-           "serve_step_frac": achieved / HBM_PEAK_GBPS if achieved else None}
+           "serve_step_frac": achieved / HBM_PEAK_GBPS if achieved else None,
+           "pressure": pressure}
     log(f"serving: p50 {lat['p50_ms']:.2f} ms, p99 {lat['p99_ms']:.2f} ms, "
         f"{fact_rows / t_open / 1e9:.2f} G fact rows/s over {len(reqs)} requests")
     if emit:
@@ -667,6 +669,53 @@ def bench_serve(args, dev, emit: bool = True):
                              "traffic": None},
                 "serving": res}
         print(json.dumps(line), flush=True)
+    return res
+
+
+def serve_under_pressure(args, dev, wl, reference, base_engine):
+    """The same request stream through an engine whose HBM budget holds
+    only args.serve_pressure of the dimension tables' total bytes (never
+    less than the largest table with every slot's accumulators): requests
+    whose table is not resident wait for room, tables are evicted and built
+    again.  Closed-loop capacity, then an open-loop run offering 0.7 of it,
+    with its p50/p99; every result equals the unpressured engine's."""
+    import numpy as np
+    from pageflow_amd import serving
+
+    dims = base_engine.dims
+    tables = sum(d.table_bytes for d in dims.values())
+    acc = 32 * 16 * wl.G
+    largest = max(d.table_bytes for d in dims.values())
+    budget = max(int(args.serve_pressure * tables), largest + acc)
+    eng = serving.ServingEngine(dev, hbm_budget=budget, max_active=32,
+                                step_rows=args.serve_step_rows, rows_per_wg=1 << 16)
+    for d, (k, g) in wl.dims.items():
+        eng.register_dim(d, k, g, wl.G)
+    eng.run(wl.requests[:16], realtime=False)
+    t0 = time.perf_counter()
+    closed = eng.run(wl.requests, realtime=False)
+    t_closed = time.perf_counter() - t0
+    capacity = len(wl.requests) / t_closed
+    reqs = serving.with_arrivals(wl.requests, args.serve_load * capacity, seed=5)
+    ev0, b0 = eng.stats["evictions"], eng.stats["builds"]
+    t0 = time.perf_counter()
+    opened = eng.run(reqs)
+    t_open = time.perf_counter() - t0
+    for a, b, c in zip(reference, closed, opened):
+        assert np.array_equal(a.sum, b.sum) and np.array_equal(a.cnt, b.cnt), a.rid
+        assert np.array_equal(a.sum, c.sum) and np.array_equal(a.cnt, c.cnt), a.rid
+    lat = serving.latency_summary(opened)
+    fact_rows = sum(r.fk.numel() for r in wl.requests)
+    res = {"hbm_budget_MiB": budget / 2**20, "tables_total_MiB": tables / 2**20,
+           "budget_frac_of_tables": budget / tables, "capacity_rps": capacity,
+           "offered_rate_rps": args.serve_load * capacity, **lat,
+           "tuples_per_s": fact_rows / t_open,
+           "evictions": eng.stats["evictions"] - ev0, "table_builds": eng.stats["builds"] - b0,
+           "peak_engine_MiB": eng.peak_used / 2**20}
+    log(f"serving under a {budget / 2**20:.1f} MiB budget ({budget / tables:.2f} of the "
+        f"tables): capacity {capacity:.0f} requests/s, p50 {lat['p50_ms']:.2f} ms, p99 "
+        f"{lat['p99_ms']:.2f} ms, {res['evictions']} evictions, {res['table_builds']} builds")
+    assert res["evictions"] > 0, "the budget did not bind"
     return res
 
 
@@ -809,6 +858,8 @@ def main() -> None:
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--serve-budget", type=int, default=8192, help="MiB")
     ap.add_argument("--serve-step-rows", type=int, default=1 << 25)
+    ap.add_argument("--serve-pressure", type=float, default=0.5,
+                    help="the pressured serving run's budget, a fraction of the tables' bytes")
     ap.add_argument("--part-bits", default="8,9,10,11,12")
     ap.add_argument("--part-shapes", default="-1")
     args = ap.parse_args()

@@ -90,6 +90,7 @@ class _Active:
     pos: int = 0
     nbytes: int = 0
     sent: bool = False       # every row is in a launched step
+    item: Optional[ServeItem] = None  # the request's step item, all but its row range
 
     @property
     def remaining(self) -> int:
@@ -130,6 +131,11 @@ class ServingEngine:
         self.stream = torch.cuda.Stream(self.device)
         self.build_stream = torch.cuda.Stream(self.device)
         self.stats = {"steps": 0, "builds": 0, "evictions": 0, "max_items": 0}
+        # one item array for every step: pf_serve_step copies the items into
+        # its launch's arguments, so the array is free again when it returns
+        self._items = (ServeItem * MAX_ITEMS)()
+        self._items_arg = ctypes.cast(self._items, ctypes.c_void_p)
+        self._stream_arg = ctypes.c_void_p(self.stream.cuda_stream)
 
     # ------------------------------------------------------------ dims --
     def register_dim(self, dim_id: int, keys: torch.Tensor, groups: torch.Tensor,
@@ -189,7 +195,10 @@ class ServingEngine:
         with torch.cuda.stream(self.stream):
             acc.zero_()
         host = torch.empty((2, d.G), dtype=torch.int64, pin_memory=True)
-        return _Active(req, table, acc, host, now, nbytes=acc_bytes)
+        item = ServeItem(req.fk.data_ptr(), req.amt.data_ptr(), 0, table.table.data_ptr(),
+                         table.slots, table.info.data_ptr(), acc[0].data_ptr(),
+                         acc[1].data_ptr(), d.G, 0)
+        return _Active(req, table, acc, host, now, nbytes=acc_bytes, item=item)
 
     def _release(self, st: _Active) -> None:
         self.used -= st.nbytes
@@ -205,24 +214,19 @@ class ServingEngine:
         n = len(live)
         order = [live[(start + i) % n] for i in range(min(n, MAX_ITEMS))]
         quantum = max(self.step_rows // len(order), self.rows_per_wg)
-        items = (ServeItem * len(order))()
+        items = self._items
         waits = set()
         finished = []
         for i, a in enumerate(order):
             take = min(a.remaining, quantum)
-            d = self.dims[a.req.dim]
-            if d.ready is not None:
+            if self.dims[a.req.dim].ready is not None:
                 waits.add(a.req.dim)
+            tpl = a.item
+            items[i] = tpl
             it = items[i]
-            it.fk = a.req.fk.data_ptr() + 4 * a.pos
-            it.amt = a.req.amt.data_ptr() + 8 * a.pos
+            it.fk = (tpl.fk or 0) + 4 * a.pos      # (an empty tensor's pointer is null)
+            it.amt = (tpl.amt or 0) + 8 * a.pos
             it.n = take
-            it.table = a.table.table.data_ptr()
-            it.slots = a.table.slots
-            it.info = a.table.info.data_ptr()
-            it.sum = a.acc[0].data_ptr()
-            it.cnt = a.acc[1].data_ptr()
-            it.G = d.G
             a.pos += take
             if a.remaining == 0:  # (an empty request rides along as a 0-row item)
                 a.sent = True
@@ -232,9 +236,8 @@ class ServingEngine:
             self.stream.wait_event(d.ready)
             d.ready = None  # later steps are ordered after this one
         with torch.cuda.device(self.device):
-            _lib.check(_lib.load().pf_serve_step(ctypes.cast(items, ctypes.c_void_p), len(order),
-                                                 self.rows_per_wg,
-                                                 ctypes.c_void_p(self.stream.cuda_stream)))
+            _lib.check(_lib.load().pf_serve_step(self._items_arg, len(order), self.rows_per_wg,
+                                                 self._stream_arg))
             with torch.cuda.stream(self.stream):
                 for a in finished:
                     a.host.copy_(a.acc, non_blocking=True)
