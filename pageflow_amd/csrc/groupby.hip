@@ -10,8 +10,9 @@
 // values that are not the null sentinel (-64).
 //
 // gfx950 design: each workgroup accumulates into LDS copies of the group
-// table (R replicas, lane % R picks one, so lanes of a wave that share a
-// group hit different banks and addresses), reduces the replicas and adds
+// table (R replicas, lane % R picks one, laid out replica-minor so that a
+// wave's adds spread over the banks whatever their groups), reduces the
+// replicas and adds
 // its partials to the output with one global int64 atomic per group.
 //
 // Serving: a dimension table (int32 key -> int32 group in [0, G)) is built
@@ -55,6 +56,17 @@ __host__ __device__ inline int replicas_for(int G) {
   return r;
 }
 
+// LDS group accumulators: R replicas of (SUM, COUNT) for each of G groups,
+// laid out [group][sum | count][replica], lane l adding into replica l % R.
+// With R a power of two (32 for G <= 64) the adds of one wave instruction
+// fall on R consecutive 8-byte words of a row of the table whatever the
+// lanes' groups, so two lanes share an LDS bank only when they share a
+// replica (a [replica][group] layout puts lanes of different groups on the
+// same banks).
+__device__ __forceinline__ int64_t* acc_at(int64_t* acc, int R, int rep, int g, int c) {
+  return acc + ((int64_t)(2 * g + c) * R + rep);
+}
+
 __device__ __forceinline__ void lds_add(int64_t* s, int64_t v) {
   atomicAdd((unsigned long long*)s, (unsigned long long)v);
 }
@@ -71,7 +83,7 @@ __global__ __launch_bounds__(GB_NT) void k_group_sum_count(const int32_t* __rest
                                                            int64_t n, int G, int R,
                                                            int64_t* sum, int64_t* cnt,
                                                            int64_t* bad) {
-  __shared__ int64_t acc[LDS ? 32 * 1024 / 8 : 1];  // [R][G][2]
+  __shared__ int64_t acc[LDS ? 32 * 1024 / 8 : 1];  // [G][2][R]
   const int tid = threadIdx.x;
   if (LDS) {
     for (int x = tid; x < R * G * 2; x += GB_NT) acc[x] = 0;
@@ -88,9 +100,8 @@ __global__ __launch_bounds__(GB_NT) void k_group_sum_count(const int32_t* __rest
       continue;
     }
     if (LDS) {
-      int64_t* a = acc + ((int64_t)rep * G + g) * 2;
-      lds_add(a, v);
-      if (v != kNull) lds_add(a + 1, 1);
+      lds_add(acc_at(acc, R, rep, g, 0), v);
+      if (v != kNull) lds_add(acc_at(acc, R, rep, g, 1), 1);
     } else {
       global_add(&sum[g], v);
       if (v != kNull) global_add(&cnt[g], 1);
@@ -105,8 +116,8 @@ __global__ __launch_bounds__(GB_NT) void k_group_sum_count(const int32_t* __rest
     for (int g = tid; g < G; g += GB_NT) {
       int64_t s = 0, c = 0;
       for (int r = 0; r < R; ++r) {
-        s += acc[((int64_t)r * G + g) * 2];
-        c += acc[((int64_t)r * G + g) * 2 + 1];
+        s += *acc_at(acc, R, r, g, 0);
+        c += *acc_at(acc, R, r, g, 1);
       }
       if (s) global_add(&sum[g], s);
       if (c) global_add(&cnt[g], c);
@@ -200,7 +211,8 @@ __device__ __forceinline__ void fact_rows_into(const int32_t* __restrict__ fk,
                                                const int64_t* __restrict__ amt, int64_t i0,
                                                int64_t i1, int64_t step,
                                                const uint64_t* __restrict__ table, uint32_t mask,
-                                               int64_t kmin, int64_t range, int64_t* acc_rep) {
+                                               int64_t kmin, int64_t range, int64_t* acc,
+                                               int R, int rep) {
   constexpr int U = 8;
   if (range > 0) {  // dense: one byte a key value
     const uint8_t* __restrict__ arr = (const uint8_t*)table;
@@ -218,9 +230,8 @@ __device__ __forceinline__ void fact_rows_into(const int32_t* __restrict__ fk,
         const uint64_t off = (uint64_t)((int64_t)k[u] - kmin);
         const uint32_t g = b + u * step < i1 && off < (uint64_t)range ? arr[off] : 0xFFu;
         if (g == 0xFFu) continue;
-        int64_t* a = acc_rep + (int64_t)g * 2;
-        lds_add(a, v[u]);
-        if (v[u] != kNull) lds_add(a + 1, 1);
+        lds_add(acc_at(acc, R, rep, (int)g, 0), v[u]);
+        if (v[u] != kNull) lds_add(acc_at(acc, R, rep, (int)g, 1), 1);
       }
     }
     return;
@@ -246,20 +257,20 @@ __device__ __forceinline__ void fact_rows_into(const int32_t* __restrict__ fk,
         t[u] = table[s[u]];
       }
       if (t[u] == kEmptySlot) continue;  // no dimension row (or past the end)
-      int64_t* a = acc_rep + (int64_t)(uint32_t)t[u] * 2;
-      lds_add(a, v[u]);
-      if (v[u] != kNull) lds_add(a + 1, 1);
+      const int g = (int)(uint32_t)t[u];
+      lds_add(acc_at(acc, R, rep, g, 0), v[u]);
+      if (v[u] != kNull) lds_add(acc_at(acc, R, rep, g, 1), 1);
     }
   }
 }
 
-__device__ __forceinline__ void flush_groups(const int64_t* acc, int G, int R, int64_t* sum,
+__device__ __forceinline__ void flush_groups(int64_t* acc, int G, int R, int64_t* sum,
                                              int64_t* cnt, int nt) {
   for (int g = threadIdx.x; g < G; g += nt) {
     int64_t s = 0, c = 0;
     for (int r = 0; r < R; ++r) {
-      s += acc[((int64_t)r * G + g) * 2];
-      c += acc[((int64_t)r * G + g) * 2 + 1];
+      s += *acc_at(acc, R, r, g, 0);
+      c += *acc_at(acc, R, r, g, 1);
     }
     if (s) global_add(&sum[g], s);
     if (c) global_add(&cnt[g], c);
@@ -278,7 +289,7 @@ __global__ __launch_bounds__(GB_NT) void k_fact_join_groupby(
   const int rep = (int)(lane_id() % R);
   const int64_t stride = (int64_t)gridDim.x * GB_NT;
   fact_rows_into(fk, amt, blockIdx.x * (int64_t)GB_NT + threadIdx.x, n, stride, table, mask,
-                 info[2], info[3], acc + (int64_t)rep * G * 2);
+                 info[2], info[3], acc, R, rep);
   __syncthreads();
   flush_groups(acc, G, R, sum, cnt, GB_NT);
 }
@@ -322,7 +333,6 @@ __global__ __launch_bounds__(SV_NT) void k_serve_step(const ServeArgs a) {
   const int64_t r0 = (int64_t)(wg - a.wg_begin[i]) * a.rows_per_wg;
   const int64_t r1 = r0 + a.rows_per_wg < it.n ? r0 + a.rows_per_wg : it.n;
   const int rep = (int)(lane_id() % R);
-  int64_t* accr = acc + (int64_t)rep * G * 2;
   if (stage) {
     const uint8_t* arr = (const uint8_t*)dimw;
     constexpr int U = 8;
@@ -340,14 +350,13 @@ __global__ __launch_bounds__(SV_NT) void k_serve_step(const ServeArgs a) {
         const uint64_t off = (uint64_t)((int64_t)k[u] - kmin);
         const uint32_t g = b + (int64_t)u * SV_NT < r1 && off < (uint64_t)range ? arr[off] : 0xFFu;
         if (g == 0xFFu) continue;
-        int64_t* ap = accr + (int64_t)g * 2;
-        lds_add(ap, v[u]);
-        if (v[u] != kNull) lds_add(ap + 1, 1);
+        lds_add(acc_at(acc, R, rep, (int)g, 0), v[u]);
+        if (v[u] != kNull) lds_add(acc_at(acc, R, rep, (int)g, 1), 1);
       }
     }
   } else {
     fact_rows_into(it.fk, it.amt, r0 + threadIdx.x, r1, SV_NT, it.table,
-                   (uint32_t)(it.slots - 1), kmin, range, accr);
+                   (uint32_t)(it.slots - 1), kmin, range, acc, R, rep);
   }
   __syncthreads();
   flush_groups(acc, G, R, it.sum, it.cnt, SV_NT);
