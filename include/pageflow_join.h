@@ -252,6 +252,45 @@ int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int6
                        int64_t* out_build, int64_t* const* pay_out, int64_t* host_out,
                        void* workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------- residual inside the join --
+ * Replaces the sort/search core's residual step (operators/hashjoin.py:
+ * 421-439: the pair count, expand, the per-condition evaluate_join masks,
+ * nonzero and the two index selects) for the partitioned join above: the
+ * residual program (the PF_RES_* program of the pair filter below, column
+ * side 0 = build, 1 = probe) is evaluated on every candidate pair inside the
+ * count and write passes, so only kept pairs are counted and written.
+ * The first two probe columns the program names are carried through the
+ * probe side's partition (their values ride with the hashes), the first two
+ * build columns are gathered into partition order (the first staged in
+ * LDS); any other column is read at the pair's row.  Every column must hold
+ * at least its side's rows.  multi: 0 unique build (a probe row's lowest
+ * build row with its hash), 1 multi-match; pairs come in the deterministic
+ * order of pf_hash_join_write.
+ * workspace: pf_hash_join_residual_workspace_bytes with the program's
+ * count of distinct probe-side and build-side columns.
+ * prepare: partitions, counts kept pairs (two waits); host_out (pinned
+ * int64[4]) [0] kept pairs, [1] the largest build bucket, [2] work items,
+ * [3] candidate pairs before the residual (the pair count the reference's
+ * INT_ROWS_MAX check reads).  write: the kept pairs (and npay gathered
+ * int64 build payloads) into arrays of host_out[0] rows; no wait.  The
+ * program passed to write must be the one passed to prepare. */
+size_t pf_hash_join_residual_workspace_bytes(int64_t nb, int64_t np, int32_t bits, int32_t multi,
+                                             int32_t probe_cols, int32_t build_cols);
+int pf_hash_join_residual_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
+                                  int64_t np, int32_t bits, int32_t multi,
+                                  const void* const* cols, const int32_t* col_types,
+                                  const int32_t* col_sides, const int64_t* col_rows,
+                                  int32_t n_cols, const int64_t* ins, int32_t n_ins,
+                                  int64_t* host_out, void* workspace, size_t workspace_bytes,
+                                  void* stream);
+int pf_hash_join_residual_write(int64_t nb, int64_t np, int32_t bits, int32_t multi,
+                                int64_t n_items, const int64_t* const* bpay, int32_t npay,
+                                int64_t* out_probe, int64_t* out_build, int64_t* const* pay_out,
+                                const void* const* cols, const int32_t* col_types,
+                                const int32_t* col_sides, const int64_t* col_rows,
+                                int32_t n_cols, const int64_t* ins, int32_t n_ins,
+                                void* workspace, size_t workspace_bytes, void* stream);
+
 /* --------------------------------------------------- keyed group-by --
  * Replaces the grouping of a keyed aggregation's pass 1
  * (operators/aggregate.py:1511-1563: the linear group hash of

@@ -84,6 +84,11 @@ _SIGNATURES = {
     "pf_hash_join_prepare": (ctypes.c_int, [P, I64, P, I64, I32, I32, P, P, SZ, P]),
     "pf_hash_join_write": (ctypes.c_int, [I64, I64, I32, I32, I64, PP, I32, P, P, PP, P, P,
                                           SZ, P]),
+    "pf_hash_join_residual_workspace_bytes": (SZ, [I64, I64, I32, I32, I32, I32]),
+    "pf_hash_join_residual_prepare": (ctypes.c_int, [P, I64, P, I64, I32, I32, P, P, P, P, I32,
+                                                     P, I32, P, P, SZ, P]),
+    "pf_hash_join_residual_write": (ctypes.c_int, [I64, I64, I32, I32, I64, PP, I32, P, P, PP,
+                                                   P, P, P, P, I32, P, I32, P, SZ, P]),
 }
 
 _lib: Optional[ctypes.CDLL] = None

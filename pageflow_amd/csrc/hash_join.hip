@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "pf_common.hpp"
+#include "pf_residual.hpp"
 #include "../../include/pageflow_join.h"
 
 namespace pf {
@@ -41,6 +42,11 @@ int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_
                     int shift, int mode, void* keys_out, int32_t* vals_out,
                     int64_t* bucket_offsets, void* ws, size_t ws_bytes, hipStream_t st,
                     int who, int shape);
+int radix_partition_carry(const int64_t* keys, int64_t n, int bits, int nv,
+                          const void* const* src, const int32_t* type, int64_t* const* dst,
+                          uint64_t* tk, int32_t* tv, int64_t* const* tx, uint64_t* h_out,
+                          int32_t* row_out, int64_t* boff, void* ws, size_t ws_bytes,
+                          hipStream_t st, int who);
 
 namespace {
 
@@ -479,6 +485,291 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
   }
 }
 
+// ------------------------------------------------- residual in the join --
+//
+// The join with its residual program evaluated on every candidate pair in
+// the count and write passes, so that only kept pairs are counted and
+// written (the reference filters the materialised pairs:
+// operators/hashjoin.py:428-439).  Operand sources (pf_residual.hpp): the
+// first build column staged in LDS with the chunk (gathered into partition
+// order once), a second one read in partition order from HBM, up to two
+// probe columns carried through the probe side's partition, any other
+// column read at the pair's row.
+//   unique  a probe row's one candidate is evaluated in its own lane;
+//   multi   a wave's candidates are evaluated 64 at a time: lane j of a
+//           window takes the wave's candidate j (its source lane found by a
+//           binary search over the lanes' candidate offsets staged in LDS),
+//           so skewed runs keep every lane busy.  The count pass and the
+//           write pass's first sweep count the kept candidates of each
+//           (round slot, wave); the second sweep evaluates them again and
+//           writes the kept ones compacted by ballot, at positions that
+//           depend on nothing but the inputs.
+// probe rows a thread a round: multi-match passes keep each slot's lookup
+// across two sweeps, so they take fewer rows a round
+constexpr int hr_it(bool multi) { return multi ? 4 : 8; }
+static_assert(4 * HJ_W % kWave == 0, "one wave scans a round's (slot, wave) totals");
+
+struct HjRes {
+  const int64_t* pv[2];  // carried probe values in partition order (nullptr: none)
+  const int64_t* bv0;    // the LDS-staged build column in partition order (nullptr: none)
+  int64_t* cand;         // count pass: candidate pairs before the residual (one counter)
+};
+
+// One (round slot)'s candidates of a wave, a window of 64 at a time; WRITE
+// stores the kept ones at base + the kept candidates before them.  Returns
+// the wave's kept count (uniform).  The s_* arrays hold the wave's lanes:
+// candidate offset, first chunk row of the run, probe row, carried values.
+template <bool WRITE>
+__device__ __forceinline__ int res_sweep(const HjParams& p, const JoinResidual& R, PairCtx c,
+                                         int T, int wb, const int32_t* s_ex, const int32_t* s_hi,
+                                         const int32_t* s_pr, const int64_t* s_p0,
+                                         const int64_t* s_p1, const int32_t* rows,
+                                         int64_t base) {
+  const int lane = lane_id();
+  int kept = 0, lo0 = 0;
+  for (int j0 = 0; j0 < T; j0 += kWave) {
+    const int j = j0 + lane;
+    bool keep = false;
+    int lo = lo0;  // the last lane whose run starts at or before j
+    if (j < T) {
+      int hi = kWave;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_ex[wb + mid] <= j)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      c.li = s_hi[wb + lo] + (j - s_ex[wb + lo]);
+      c.prow = s_pr[wb + lo];
+      c.pv0 = s_p0[wb + lo];
+      c.pv1 = s_p1[wb + lo];
+      keep = jr_eval(R, c);
+    }
+    const uint64_t bal = ballot(keep);
+    if (WRITE && keep) {
+      const int64_t o = base + kept + mbcnt(bal);
+      p.out_probe[o] = c.prow;
+      p.out_build[o] = (int64_t)rows[c.li];
+      for (int q = 0; q < p.npay; ++q) p.pay_out[q][o] = p.pay_src[q][c.cb + c.li];
+    }
+    kept += __popcll(bal);
+    lo0 = __shfl(lo, kWave - 1, kWave);
+  }
+  return kept;
+}
+
+// W1 = false: count the kept pairs of each item (and every candidate, into
+// x.cand); true: write them at the scanned item positions.
+template <bool MULTI, bool W1>
+__global__ __launch_bounds__(HJ_NT) void hj_probe_res(HjParams p, JoinResidual R, HjRes x) {
+  constexpr int CAP = Lds<MULTI>::CAP, TS = Lds<MULTI>::TS;
+  constexpr int IT = hr_it(MULTI), ROUND = HJ_NT * IT;
+  constexpr int MT = MULTI ? HJ_NT : 1;
+  __shared__ uint64_t keys[CAP];
+  __shared__ uint32_t slotw[TS / 2];
+  const uint16_t* slot = reinterpret_cast<const uint16_t*>(slotw);
+  __shared__ int32_t rows[CAP];
+  __shared__ int64_t bv[CAP];
+  __shared__ uint16_t glen[MULTI ? CAP : 1];
+  __shared__ int64_t cnt[IT * HJ_W];
+  __shared__ int32_t s_ex[MT], s_hi[MT], s_pr[MT];
+  __shared__ int64_t s_p0[MT], s_p1[MT];
+  __shared__ int64_t last_tot;
+  const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave, wb = w * kWave;
+  const int64_t item = blockIdx.x;
+  if (item >= p.n_items) return;
+  const Item it = item_of(p, item);
+  int64_t pos = W1 ? p.item_pos[item] : 0;
+  int64_t kept_n = 0, cand_n = 0;
+  const bool has_p0 = x.pv[0] != nullptr, has_p1 = x.pv[1] != nullptr;
+  for (int64_t cb = it.bs; cb < it.be; cb += CAP) {
+    const int nloc = (int)(it.be - cb < CAP ? it.be - cb : CAP);
+    __syncthreads();  // the previous chunk's lookups are done
+    for (int s = tid; s < TS / 2; s += HJ_NT) slotw[s] = 0;
+    for (int i = tid; i < nloc; i += HJ_NT) {
+      keys[i] = p.bh[cb + i];
+      rows[i] = p.brow[cb + i];
+      if (x.bv0) bv[i] = x.bv0[cb + i];
+    }
+    __syncthreads();
+    for (int i = tid; i < nloc; i += HJ_NT) {
+      const uint64_t h = keys[i];
+      if (MULTI && i > 0 && keys[i - 1] == h) continue;  // not a run head
+      if (MULTI) {
+        int len = 1;
+        while (i + len < nloc && keys[i + len] == h) ++len;
+        glen[i] = (uint16_t)(len > 65535 ? 65535 : len);
+      }
+      lds_insert<TS>(keys, slotw, h, i);
+    }
+    __syncthreads();
+
+    const uint64_t* __restrict__ ph = p.ph + it.ps;
+    const int32_t* __restrict__ prw = p.prow + it.ps;
+    const int64_t* __restrict__ pv0 = has_p0 ? x.pv[0] + it.ps : nullptr;
+    const int64_t* __restrict__ pv1 = has_p1 ? x.pv[1] + it.ps : nullptr;
+    const int np_item = (int)(it.pe - it.ps);
+    PairCtx c;
+    c.bv = bv;
+    c.rows = rows;
+    c.cb = cb;
+    for (int i0 = 0; i0 < np_item; i0 += ROUND) {
+      // unique: a lane's rows and values are loaded with the hashes; multi:
+      // they are loaded as each slot is staged (fewer live registers)
+      constexpr int QT = MULTI ? 1 : IT;
+      uint64_t hv[IT];
+      int32_t pr[QT];
+      int64_t q0[QT], q1[QT];
+      int hidx[IT], m[IT];
+#pragma unroll
+      for (int k = 0; k < IT; ++k) {
+        const int i = i0 + k * HJ_NT + tid;
+        const int ic = i < np_item ? i : np_item - 1;
+        hv[k] = ph[ic];
+        if constexpr (!MULTI) {
+          pr[k] = prw[ic];
+          q0[k] = has_p0 ? pv0[ic] : 0;
+          q1[k] = has_p1 ? pv1[ic] : 0;
+        }
+      }
+      // stages slot k's lanes for the window sweeps (multi)
+      auto stage = [&](int k, int inw_k) {
+        const int i = i0 + k * HJ_NT + tid;
+        const int ic = i < np_item ? i : np_item - 1;
+        s_ex[tid] = inw_k;
+        s_hi[tid] = hidx[k];
+        s_pr[tid] = prw[ic];
+        s_p0[tid] = has_p0 ? pv0[ic] : 0;
+        s_p1[tid] = has_p1 ? pv1[ic] : 0;
+        wave_lds_sync();  // a wave reads only its own lanes' entries
+      };
+#pragma unroll
+      for (int k = 0; k < IT; ++k) {
+        const int i = i0 + k * HJ_NT + tid;
+        const int f = lds_find<TS>(keys, slot, hv[k]);
+        hidx[k] = i < np_item ? f : -1;
+        int len = hidx[k] >= 0 ? 1 : 0;
+        if (MULTI && hidx[k] >= 0) {
+          len = glen[hidx[k]];
+          if (len == 65535)  // a run longer than glen's range, counted to its end
+            while (hidx[k] + len < nloc && keys[hidx[k] + len] == hv[k]) ++len;
+        }
+        m[k] = len;
+        cand_n += len;
+      }
+      int kc[IT];  // kept: unique, this lane's (0/1); multi, the wave's in the slot
+      int inw[IT];
+      if constexpr (!MULTI) {
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+          bool keep = false;
+          if (hidx[k] >= 0) {
+            c.li = hidx[k];
+            c.prow = pr[k];
+            c.pv0 = q0[k];
+            c.pv1 = q1[k];
+            keep = jr_eval(R, c);
+          }
+          kc[k] = keep ? 1 : 0;
+          if (!W1) kept_n += kc[k];
+        }
+        if (!W1) continue;
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+          const uint64_t bal = ballot(kc[k] != 0);
+          inw[k] = mbcnt(bal);
+          if (lane == 0) cnt[k * HJ_W + w] = __popcll(bal);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+          const int incl = wave_incl_scan(m[k]);
+          inw[k] = incl - m[k];
+          m[k] = __shfl(incl, kWave - 1, kWave);  // from here: the wave's candidates
+        }
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+          stage(k, inw[k]);
+          kc[k] = res_sweep<false>(p, R, c, m[k], wb, s_ex, s_hi, s_pr, s_p0, s_p1, rows, 0);
+          wave_lds_sync();  // s_* are rewritten for the next slot
+          if (!W1 && lane == 0) kept_n += kc[k];
+        }
+        if (!W1) continue;
+        if (lane == 0) {
+#pragma unroll
+          for (int k = 0; k < IT; ++k) cnt[k * HJ_W + w] = kc[k];
+        }
+      }
+      __syncthreads();
+      if (w == 0) {  // PER consecutive (slot, wave) totals a lane
+        constexpr int PER = IT * HJ_W / kWave;
+        int64_t v[PER], sum = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          v[q] = cnt[lane * PER + q];
+          sum += v[q];
+        }
+        int64_t ex = wave_incl_scan(sum) - sum;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          cnt[lane * PER + q] = ex;
+          ex += v[q];
+        }
+        if (lane == kWave - 1) last_tot = ex;
+      }
+      __syncthreads();
+      if constexpr (!MULTI) {
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+          if (!kc[k]) continue;
+          const int64_t o = pos + cnt[k * HJ_W + w] + inw[k];
+          p.out_probe[o] = (int64_t)pr[k];
+          p.out_build[o] = (int64_t)rows[hidx[k]];
+          for (int q = 0; q < p.npay; ++q) p.pay_out[q][o] = p.pay_src[q][cb + hidx[k]];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+          stage(k, inw[k]);
+          res_sweep<true>(p, R, c, m[k], wb, s_ex, s_hi, s_pr, s_p0, s_p1, rows,
+                          pos + cnt[k * HJ_W + w]);
+          wave_lds_sync();
+        }
+      }
+      pos += last_tot;
+      __syncthreads();  // last_tot and cnt are rewritten by the next round
+    }
+  }  // build chunks
+  if (!W1) {
+    const int64_t t = wave_sum(kept_n), a = wave_sum(cand_n);
+    if (lane == 0) {
+      cnt[w] = t;
+      cnt[HJ_W + w] = a;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int64_t s = 0, sa = 0;
+      for (int q = 0; q < HJ_W; ++q) {
+        s += cnt[q];
+        sa += cnt[HJ_W + q];
+      }
+      p.item_cnt[item] = s;
+      if (sa) atomicAdd((unsigned long long*)x.cand, (unsigned long long)sa);
+    }
+  }
+}
+
+// dst[i] = the widened value of a build column at idx[i]: a residual
+// operand in partition order.
+__global__ void hj_gather_res(const void* __restrict__ src, int type,
+                              const int32_t* __restrict__ idx, int64_t n,
+                              int64_t* __restrict__ dst) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = rf_widen(src, type, idx[i]);
+}
+
 // dst[i] = src[idx[i]]: a build payload column in partition order.
 __global__ void hj_gather_pay(const int64_t* __restrict__ src, const int32_t* __restrict__ idx,
                               int64_t n, int64_t* __restrict__ dst) {
@@ -649,6 +940,126 @@ int sort_big_buckets(const HjWs& w, int64_t nb, int bits, hipStream_t st, bool* 
   return 0;
 }
 
+// Value columns the probe side's partition carries (the residual's probe
+// operands).
+struct ProbeCarry {
+  int nv = 0;
+  const void* src[2] = {nullptr, nullptr};
+  int32_t type[2] = {0, 0};
+  int64_t* dst[2] = {nullptr, nullptr};
+  int64_t* tx[2] = {nullptr, nullptr};
+};
+
+// Both sides partitioned as (mix64(key), row) by the top `bits` bits of the
+// hash (the probe side with its carried columns when pc has any), the work
+// items built and read back (one wait), and for multi-match builds every
+// bucket sorted by (hash, row).  *n_items and *largest from the read-back.
+int prepare_sides(const int64_t* bkeys, int64_t nb, const int64_t* pkeys, int64_t np, int bits,
+                  bool multi, const HjWs& w, const ProbeCarry& pc, int64_t* host_out,
+                  void* hdev, hipStream_t st, int64_t* n_items, int64_t* largest) {
+  const int P = 1 << bits;
+  const size_t pws = part_ws_bytes(nb, np, bits);
+  if (partition_side(bkeys, nb, bits, w.bh, w.brow, w.boff, w.th, w.trow, w.part_ws, pws, st, 1))
+    return 1;
+  if (pc.nv > 0) {
+    if (radix_partition_carry(pkeys, np, bits, pc.nv, pc.src, pc.type, pc.dst, w.th, w.trow,
+                              pc.tx, w.ph, w.prow, w.poff, w.part_ws, pws, st, 2))
+      return 1;
+  } else if (partition_side(pkeys, np, bits, w.ph, w.prow, w.poff, w.th, w.trow, w.part_ws, pws,
+                            st, 2)) {
+    return 1;
+  }
+  hj_items<<<1, 1024, 0, st>>>(w.boff, w.poff, P, pch_of(multi), max_items(np, bits, multi),
+                               w.item_off, w.item_bucket, w.info);
+  hj_readback<<<1, 64, 0, st>>>(w.info, nullptr, (int64_t*)hdev);
+  if (launch_check("hash_join items")) return 1;
+  if (wait_stream(st, "hash_join")) return 1;
+  *n_items = host_out[0];
+  *largest = host_out[1];
+  if (*n_items > max_items(np, bits, multi)) return fail("hash_join: work items overflow");
+  bool all_sorted = false;  // every bucket sorted in HBM already
+  if (multi && *largest > SORT_CAP && sort_big_buckets(w, nb, bits, st, &all_sorted)) return 1;
+  if (multi && nb > 0 && !all_sorted) {
+    {
+      KTimer _kt("hj_sort_buckets", st);
+      int cap = 2;  // the largest bucket sorted in LDS, to a power of two
+      while (cap < *largest && cap < SORT_CAP) cap <<= 1;
+      const size_t lds = (size_t)cap * (sizeof(uint64_t) + sizeof(int32_t));
+      if (hip_check(hipFuncSetAttribute((const void*)hj_sort_buckets,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)(SORT_CAP * (sizeof(uint64_t) + sizeof(int32_t)))),
+                    "hash_join sort LDS"))
+        return 1;
+      hj_sort_buckets<<<(unsigned)P, SORT_NT, lds, st>>>(w.bh, w.brow, w.boff, cap);
+    }
+    if (launch_check("hash_join sort")) return 1;
+  }
+  return 0;
+}
+
+// ------------------------------------------ the residual join's host side --
+
+// The residual's extra workspace after the join's own (HjWs): the carried
+// probe columns (and the two-level partition's first-pass copies), the
+// build columns in partition order, and the candidate counter.
+struct HrWs {
+  int64_t* px[2] = {nullptr, nullptr};
+  int64_t* tx[2] = {nullptr, nullptr};
+  int64_t* bx[2] = {nullptr, nullptr};
+  int64_t* dev = nullptr;
+  size_t bytes = 0;
+};
+
+HrWs carve_hr(void* ws, size_t base, int64_t nb, int64_t np, int bits, int nv, int nbv) {
+  Carve cv(ws ? (char*)ws + base : nullptr);
+  HrWs r;
+  for (int c = 0; c < nv; ++c) r.px[c] = cv.take<int64_t>(np > 0 ? np : 1);
+  if (bits > kOnePassBits)
+    for (int c = 0; c < nv; ++c) r.tx[c] = cv.take<int64_t>(np > 0 ? np : 1);
+  for (int c = 0; c < nbv; ++c) r.bx[c] = cv.take<int64_t>(nb > 0 ? nb : 1);
+  r.dev = cv.take<int64_t>(2);
+  r.bytes = base + cv.bytes();
+  return r;
+}
+
+// Where each column slot's values come from: the first two probe columns
+// the program names are carried through the partition, the first build
+// column is staged in LDS and the second read in partition order, and any
+// other is read at the pair's row.  Every column must cover its side's
+// rows.
+int residual_plan(const RfProgram& P, int n_cols, int64_t nb, int64_t np, JoinResidual* R,
+                  int* nv, int* nbv, int pslot[2], int bslot[2]) {
+  R->P = P;
+  R->bpart1 = nullptr;
+  for (int s = 0; s < PF_RES_MAX_COLS; ++s) R->src[s] = RS_ROW;
+  *nv = *nbv = 0;
+  bool seen[PF_RES_MAX_COLS] = {};
+  for (int k = 0; k < P.n_ins; ++k) {
+    const int64_t w0 = P.ins[k][0];
+    if ((w0 & 0xff) != PF_RES_CMP) continue;
+    for (int o = 0; o < 2; ++o) {
+      if (!(w0 & (o ? PF_RES_B_COL : PF_RES_A_COL))) continue;
+      const int sl = (int)P.ins[k][1 + o];
+      if (seen[sl]) continue;
+      seen[sl] = true;
+      const bool probe = P.side[sl] != 0;
+      if (P.rows[sl] < (probe ? np : nb))
+        return fail("hash_join residual: column slot " + std::to_string(sl) + " holds " +
+                    std::to_string(P.rows[sl]) + " rows, fewer than its side's " +
+                    std::to_string(probe ? np : nb));
+      if (probe && *nv < 2) {
+        R->src[sl] = *nv == 0 ? RS_P0 : RS_P1;
+        pslot[(*nv)++] = sl;
+      } else if (!probe && *nbv < 2) {
+        R->src[sl] = *nbv == 0 ? RS_BLDS : RS_BPART;
+        bslot[(*nbv)++] = sl;
+      }
+    }
+  }
+  (void)n_cols;
+  return 0;
+}
+
 HjParams params_of(const HjWs& w, int bits, bool multi, int64_t n_items) {
   HjParams p{};
   p.pch = pch_of(multi);
@@ -693,40 +1104,10 @@ int pf_hash_join_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
   if (workspace_bytes < w.bytes) return fail("hash_join: workspace too small");
   void* hdev = nullptr;
   if (host_alias(host_out, &hdev)) return 1;
-  const int P = 1 << bits;
-  const int cap = multi ? CAP_M : CAP_U;
-  const size_t pws = part_ws_bytes(nb, np, bits);
-  // both sides as (mix64(key), row) by the top `bits` bits of the hash
-  if (partition_side(bkeys, nb, bits, w.bh, w.brow, w.boff, w.th, w.trow, w.part_ws, pws, st, 1))
+  int64_t n_items = 0, largest = 0;
+  if (prepare_sides(bkeys, nb, pkeys, np, bits, multi != 0, w, ProbeCarry{}, host_out, hdev, st,
+                    &n_items, &largest))
     return 1;
-  if (partition_side(pkeys, np, bits, w.ph, w.prow, w.poff, w.th, w.trow, w.part_ws, pws, st, 2))
-    return 1;
-  hj_items<<<1, 1024, 0, st>>>(w.boff, w.poff, P, pch_of(multi != 0),
-                               max_items(np, bits, multi != 0), w.item_off, w.item_bucket,
-                               w.info);
-  hj_readback<<<1, 64, 0, st>>>(w.info, nullptr, (int64_t*)hdev);
-  if (launch_check("hash_join items")) return 1;
-  if (wait_stream(st, "hash_join")) return 1;
-  const int64_t n_items = host_out[0];
-  const int64_t largest = host_out[1];
-  if (n_items > max_items(np, bits, multi != 0)) return fail("hash_join: work items overflow");
-  bool all_sorted = false;  // every bucket sorted in HBM already
-  if (multi && largest > SORT_CAP && sort_big_buckets(w, nb, bits, st, &all_sorted)) return 1;
-  if (multi && nb > 0 && !all_sorted) {
-    {
-    KTimer _kt("hj_sort_buckets", st);
-    int cap = 2;  // the largest bucket sorted in LDS, to a power of two
-    while (cap < largest && cap < SORT_CAP) cap <<= 1;
-    const size_t lds = (size_t)cap * (sizeof(uint64_t) + sizeof(int32_t));
-    if (hip_check(hipFuncSetAttribute((const void*)hj_sort_buckets,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)(SORT_CAP * (sizeof(uint64_t) + sizeof(int32_t)))),
-                  "hash_join sort LDS"))
-      return 1;
-    hj_sort_buckets<<<(unsigned)P, SORT_NT, lds, st>>>(w.bh, w.brow, w.boff, cap);
-  }
-    if (launch_check("hash_join sort")) return 1;
-  }
   if (any_order) {
     // no count pass: the write pass finds the pairs (at most one a probe row)
     host_out[0] = -1;
@@ -810,6 +1191,134 @@ int pf_hash_join_write(int64_t nb, int64_t np, int32_t bits, int32_t multi, int6
     if (launch_check("hash_join write")) return 1;
     if (wait_stream(st, "hash_join write")) return 1;
   }
+  return ok();
+}
+
+size_t pf_hash_join_residual_workspace_bytes(int64_t nb, int64_t np, int32_t bits, int32_t multi,
+                                             int32_t probe_cols, int32_t build_cols) {
+  if (bits < 0) bits = 0;
+  if (bits > 12) bits = 12;
+  const int nv = probe_cols < 0 ? 0 : probe_cols > 2 ? 2 : probe_cols;
+  const int nbv = build_cols < 0 ? 0 : build_cols > 2 ? 2 : build_cols;
+  const size_t base = pf::carve_hj(nullptr, nb, np, bits, multi == 1).bytes;
+  return pf::carve_hr(nullptr, base, nb, np, bits, nv, nbv).bytes;
+}
+
+int pf_hash_join_residual_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
+                                  int64_t np, int32_t bits, int32_t multi,
+                                  const void* const* cols, const int32_t* col_types,
+                                  const int32_t* col_sides, const int64_t* col_rows,
+                                  int32_t n_cols, const int64_t* ins, int32_t n_ins,
+                                  int64_t* host_out, void* workspace, size_t workspace_bytes,
+                                  void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (bits < 0 || bits > 12) return fail("hash_join: bits must be in [0, 12]");
+  if (nb < 0 || np < 0 || nb >= (1LL << 31) || np >= (1LL << 31))
+    return fail("hash_join: a side takes fewer than 2^31 rows");
+  if (multi != 0 && multi != 1) return fail("hash_join residual: multi is 0 or 1");
+  RfProgram P;
+  if (rf_program_of(cols, col_types, col_sides, col_rows, n_cols, ins, n_ins, &P)) return 1;
+  JoinResidual R;
+  int nv, nbv, pslot[2], bslot[2];
+  if (residual_plan(P, n_cols, nb, np, &R, &nv, &nbv, pslot, bslot)) return 1;
+  HjWs w = carve_hj(workspace, nb, np, bits, multi != 0);
+  HrWs r = carve_hr(workspace, w.bytes, nb, np, bits, nv, nbv);
+  if (workspace_bytes < r.bytes) return fail("hash_join: workspace too small");
+  void* hdev = nullptr;
+  if (host_alias(host_out, &hdev)) return 1;
+  ProbeCarry pc;
+  pc.nv = nv;
+  for (int c = 0; c < nv; ++c) {
+    pc.src[c] = P.col[pslot[c]];
+    pc.type[c] = P.type[pslot[c]];
+    pc.dst[c] = r.px[c];
+    pc.tx[c] = r.tx[c];
+  }
+  int64_t n_items = 0, largest = 0;
+  if (prepare_sides(bkeys, nb, pkeys, np, bits, multi != 0, w, pc, host_out, hdev, st, &n_items,
+                    &largest))
+    return 1;
+  // the build operands in the (sorted) partition order
+  for (int c = 0; c < nbv; ++c) {
+    hj_gather_res<<<grid_for(nb, 256) < 16384 ? grid_for(nb, 256) : 16384, 256, 0, st>>>(
+        P.col[bslot[c]], P.type[bslot[c]], w.brow, nb, r.bx[c]);
+    if (launch_check("hash_join residual operands")) return 1;
+  }
+  if (fill_i64(r.dev, 2, 0, st)) return 1;
+  HjParams p = params_of(w, bits, multi != 0, n_items);
+  R.bpart1 = nbv > 1 ? r.bx[1] : nullptr;
+  HjRes x{{nv > 0 ? r.px[0] : nullptr, nv > 1 ? r.px[1] : nullptr},
+          nbv > 0 ? r.bx[0] : nullptr, r.dev};
+  if (n_items > 0) {
+    {
+      KTimer _kt("hj_count_res", st);
+      if (multi)
+        hj_probe_res<true, false><<<(unsigned)n_items, HJ_NT, 0, st>>>(p, R, x);
+      else
+        hj_probe_res<false, false><<<(unsigned)n_items, HJ_NT, 0, st>>>(p, R, x);
+    }
+    if (launch_check("hash_join residual count")) return 1;
+  }
+  if (scan_i64_excl(w.item_cnt, n_items, w.total, w.scan, st)) return 1;
+  hj_readback<<<1, 64, 0, st>>>(w.total, r.dev, (int64_t*)hdev);
+  if (launch_check("hash_join residual count")) return 1;
+  if (wait_stream(st, "hash_join")) return 1;
+  // host_out: [0] kept pairs, [1] the largest build bucket, [2] work items,
+  // [3] candidate pairs before the residual
+  host_out[3] = host_out[2];
+  host_out[1] = largest;
+  host_out[2] = n_items;
+  return ok();
+}
+
+int pf_hash_join_residual_write(int64_t nb, int64_t np, int32_t bits, int32_t multi,
+                                int64_t n_items, const int64_t* const* bpay, int32_t npay,
+                                int64_t* out_probe, int64_t* out_build, int64_t* const* pay_out,
+                                const void* const* cols, const int32_t* col_types,
+                                const int32_t* col_sides, const int64_t* col_rows,
+                                int32_t n_cols, const int64_t* ins, int32_t n_ins,
+                                void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace pf;
+  hipStream_t st = (hipStream_t)stream;
+  if (bits < 0 || bits > 12) return fail("hash_join: bits must be in [0, 12]");
+  if (npay < 0 || npay > MAX_PAY) return fail("hash_join: at most 2 payload columns");
+  if (multi != 0 && multi != 1) return fail("hash_join residual: multi is 0 or 1");
+  RfProgram P;
+  if (rf_program_of(cols, col_types, col_sides, col_rows, n_cols, ins, n_ins, &P)) return 1;
+  JoinResidual R;
+  int nv, nbv, pslot[2], bslot[2];
+  if (residual_plan(P, n_cols, nb, np, &R, &nv, &nbv, pslot, bslot)) return 1;
+  HjWs w = carve_hj(workspace, nb, np, bits, multi != 0);
+  HrWs r = carve_hr(workspace, w.bytes, nb, np, bits, nv, nbv);
+  if (workspace_bytes < r.bytes) return fail("hash_join: workspace too small");
+  if (n_items <= 0) return ok();
+  if (n_items > max_items(np, bits, multi != 0)) return fail("hash_join: work items overflow");
+  HjParams p = params_of(w, bits, multi != 0, n_items);
+  p.out_probe = out_probe;
+  p.out_build = out_build;
+  p.npay = npay;
+  for (int c = 0; c < npay; ++c) {
+    {
+      KTimer _kt("hj_gather_pay", st);
+      hj_gather_pay<<<grid_for(nb, 256) < 16384 ? grid_for(nb, 256) : 16384, 256, 0, st>>>(
+          bpay[c], w.brow, nb, w.pay_part[c]);
+    }
+    if (launch_check("hash_join payload")) return 1;
+    p.pay_src[c] = w.pay_part[c];
+    p.pay_out[c] = pay_out[c];
+  }
+  R.bpart1 = nbv > 1 ? r.bx[1] : nullptr;
+  HjRes x{{nv > 0 ? r.px[0] : nullptr, nv > 1 ? r.px[1] : nullptr},
+          nbv > 0 ? r.bx[0] : nullptr, r.dev};
+  {
+    KTimer _kt("hj_write_res", st);
+    if (multi)
+      hj_probe_res<true, true><<<(unsigned)n_items, HJ_NT, 0, st>>>(p, R, x);
+    else
+      hj_probe_res<false, true><<<(unsigned)n_items, HJ_NT, 0, st>>>(p, R, x);
+  }
+  if (launch_check("hash_join residual write")) return 1;
   return ok();
 }
 

@@ -25,6 +25,7 @@
 #include <type_traits>
 
 #include "pf_common.hpp"
+#include "pf_residual.hpp"
 #include "../../include/pageflow_join.h"
 
 namespace pf {
@@ -474,47 +475,6 @@ int build_set(const SetWs& s, const int64_t* bk, const int64_t* bv, int64_t nb, 
 constexpr int RF_G = 4;  // pairs a thread evaluates together
 static_assert(CT_IT % RF_G == 0, "rounds split into groups");
 
-struct RfProgram {
-  const void* col[PF_RES_MAX_COLS];
-  int64_t rows[PF_RES_MAX_COLS];
-  int32_t type[PF_RES_MAX_COLS];
-  int32_t side[PF_RES_MAX_COLS];
-  int64_t ins[PF_RES_MAX_INS][3];
-  int32_t n_ins;
-};
-
-// torch.isclose(a, b, rtol=1e-5, atol=1e-5) in float64: a == b, or a
-// finite |a - b| within atol + |rtol * b|, each step rounded once.
-__device__ __forceinline__ bool rf_close(double a, double b) {
-  if (a == b) return true;
-  const double err = fabs(__dsub_rn(a, b));
-  const double tol = __dadd_rn(1e-5, fabs(__dmul_rn(1e-5, b)));
-  return isfinite(err) && err <= tol;
-}
-
-__device__ __forceinline__ bool rf_rel_i(int rel, int64_t a, int64_t b) {
-  switch (rel) {
-    case PF_REL_EQ: return a == b;
-    case PF_REL_NE: return a != b;
-    case PF_REL_LT: return a < b;
-    case PF_REL_LE: return a <= b;
-    case PF_REL_GT: return a > b;
-    default: return a >= b;
-  }
-}
-
-__device__ __forceinline__ bool rf_rel_f(int rel, double a, double b) {
-  switch (rel) {
-    case PF_REL_EQ: return a == b;
-    case PF_REL_NE: return a != b;
-    case PF_REL_LT: return a < b;
-    case PF_REL_LE: return a <= b;
-    case PF_REL_GT: return a > b;
-    case PF_REL_GE: return a >= b;
-    default: return rf_close(a, b);
-  }
-}
-
 // Column values at idx: integers sign- (uint8 zero-) extended, float64 as
 // its bits; F64 converts integers to float64 bits.  An index outside the
 // column reads 0 and counts in bad.
@@ -692,46 +652,6 @@ RfWs carve_rf(void* ws, int64_t n) {
   s.scan = cv.take<char>(scan_i32_scratch(tiles));
   s.bytes = cv.bytes();
   return s;
-}
-
-// Checks the program: opcodes, relations, slots, operand domains and the
-// stack depth (at most 32, exactly one result).
-int rf_validate(const RfProgram& P, int n_cols) {
-  int depth = 0;
-  for (int k = 0; k < P.n_ins; ++k) {
-    const int64_t w0 = P.ins[k][0];
-    const int opc = (int)(w0 & 0xff);
-    const bool f64 = (w0 & PF_RES_F64) != 0;
-    if (opc == PF_RES_CMP) {
-      const int rel = (int)((w0 >> 8) & 0xff);
-      if (rel < PF_REL_EQ || rel > PF_REL_CLOSE || (rel == PF_REL_CLOSE && !f64))
-        return fail("pair_filter: instruction " + std::to_string(k) + " has an unknown relation");
-      for (int o = 0; o < 2; ++o) {
-        if (!(w0 & (o ? PF_RES_B_COL : PF_RES_A_COL))) continue;
-        const int64_t s = P.ins[k][1 + o];
-        if (s < 0 || s >= n_cols)
-          return fail("pair_filter: instruction " + std::to_string(k) + " names column slot " +
-                      std::to_string(s) + " of " + std::to_string(n_cols));
-        if (P.type[s] == PF_COL_F64 && !f64)
-          return fail("pair_filter: a float64 column compares in float64 only");
-      }
-      ++depth;
-    } else if (opc == PF_RES_TRUE) {
-      ++depth;
-    } else if (opc == PF_RES_AND || opc == PF_RES_OR) {
-      if (depth < 2) return fail("pair_filter: AND/OR on fewer than two results");
-      --depth;
-    } else if (opc == PF_RES_NOT) {
-      if (depth < 1) return fail("pair_filter: NOT on no result");
-    } else {
-      return fail("pair_filter: instruction " + std::to_string(k) + " has opcode " +
-                  std::to_string(opc));
-    }
-    if (depth > 32) return fail("pair_filter: the program is deeper than 32 results");
-  }
-  if (depth != 1) return fail("pair_filter: the program leaves " + std::to_string(depth) +
-                              " results, not one");
-  return 0;
 }
 
 int unique_hash_run(const int64_t* b0, const int64_t* b1, int64_t nb, const int64_t* p0,
@@ -1001,30 +921,11 @@ int pf_pair_filter_count(const int64_t* left_idx, const int64_t* right_idx, int6
   using namespace pf;
   hipStream_t st = (hipStream_t)stream;
   if (n < 0) return fail("pair_filter: negative pairs");
-  if (n_cols < 0 || n_cols > PF_RES_MAX_COLS)
-    return fail("pair_filter: at most " + std::to_string(PF_RES_MAX_COLS) + " columns");
-  if (n_ins < 1 || n_ins > PF_RES_MAX_INS)
-    return fail("pair_filter: 1 to " + std::to_string(PF_RES_MAX_INS) + " instructions");
-  if (!ins || (n_cols > 0 && (!cols || !col_types || !col_sides || !col_rows)))
-    return fail("pair_filter: missing program or column arrays");
+  RfProgram P;
+  if (rf_program_of(cols, col_types, col_sides, col_rows, n_cols, ins, n_ins, &P)) return 1;
   if (n > 0 && (!left_idx || !right_idx)) return fail("pair_filter: missing pair indexes");
   if (workspace_bytes < pf_pair_filter_workspace_bytes(n))
     return fail("pair_filter: workspace too small");
-  RfProgram P{};
-  for (int s = 0; s < n_cols; ++s) {
-    if (col_types[s] < PF_COL_I64 || col_types[s] > PF_COL_F64)
-      return fail("pair_filter: column slot " + std::to_string(s) + " has an unknown type");
-    if (col_rows[s] < 0 || (col_rows[s] > 0 && !cols[s]))
-      return fail("pair_filter: column slot " + std::to_string(s) + " has no rows pointer");
-    P.col[s] = cols[s];
-    P.rows[s] = col_rows[s];
-    P.type[s] = col_types[s];
-    P.side[s] = col_sides[s] != 0;
-  }
-  P.n_ins = n_ins;
-  for (int k = 0; k < n_ins; ++k)
-    for (int q = 0; q < 3; ++q) P.ins[k][q] = ins[3 * k + q];
-  if (rf_validate(P, n_cols)) return 1;
   void* hdev = nullptr;
   if (host_alias(host_out, &hdev)) return 1;
   RfWs s = carve_rf(workspace, n);

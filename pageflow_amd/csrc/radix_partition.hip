@@ -26,6 +26,7 @@
 #include <string>
 
 #include "pf_common.hpp"
+#include "pf_residual.hpp"
 #include "../../include/pageflow_join.h"
 
 namespace pf {
@@ -38,6 +39,16 @@ int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_
 namespace {
 
 enum RpMode : int { RP_HASH_KEY = 0, RP_HASH_OUT = 1, RP_RAW = 2 };
+
+// Up to two value columns that travel with the rows of a partition (the
+// join's probe-side residual operands): src[c] is read at the row's input
+// index as a PF_COL_* column and widened to 64 bits (rf_widen), dst[c]
+// gets it at the row's output position.
+struct Carry {
+  const void* src[2];
+  int32_t type[2];
+  int64_t* dst[2];
+};
 
 template <typename K>
 struct Unsigned;
@@ -288,13 +299,14 @@ __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_scatter(
 // bucket owned by one thread across the per-tile bookkeeping (wave prefix,
 // tile scan, global base, cursor), and the tile's bucket ids staged as bytes.
 // Rows [c0, c1) of `keys`; bucket b's first output row is
-// offs_col[b * ostride] + delta.
-template <typename K, typename KO, int MODE, int NT, int IT>
+// offs_col[b * ostride] + delta.  NV value columns of `cy` ride along.
+template <typename K, typename KO, int MODE, int NT, int IT, int NV = 0>
 __device__ __forceinline__ void rp_scatter8_body(
     const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t c0, int64_t c1,
     int bits, int shift, const int64_t* __restrict__ offs_col, int64_t ostride,
-    KO* __restrict__ kout, int32_t* __restrict__ vout, int64_t delta) {
+    KO* __restrict__ kout, int32_t* __restrict__ vout, int64_t delta, const Carry& cy) {
   constexpr int W = NT / kWave, TILE = NT * IT, BMAX = 256, WROWS = kWave * IT;
+  constexpr int XV = NV > 0 ? NV : 1, XT = NV > 0 ? TILE : 1;
   static_assert(BMAX <= NT, "a thread owns each bucket");
   __shared__ uint16_t wh[W * BMAX];
   __shared__ uint16_t tstart[BMAX];
@@ -302,6 +314,7 @@ __device__ __forceinline__ void rp_scatter8_body(
   __shared__ KO stage_k[TILE];
   __shared__ int32_t stage_v[TILE];
   __shared__ uint8_t sb[TILE];
+  __shared__ int64_t stage_x[XV][XT];
   __shared__ int32_t tmp[W + 1];
 
   const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
@@ -311,6 +324,7 @@ __device__ __forceinline__ void rp_scatter8_body(
 
   K kn[IT];
   int32_t vn[IT];
+  int64_t xn[XV][IT];
   // tiles start on a 64-row boundary (a wave's loads cover whole lines);
   // rows before c0 are loaded by no lane and ranked as no row
   const int64_t a0 = c0 & ~(int64_t)(kWave - 1);
@@ -322,6 +336,8 @@ __device__ __forceinline__ void rp_scatter8_body(
       if (i >= c0 && i < c1) {
         kn[k] = keys[i];
         vn[k] = vals ? vals[i] : (int32_t)i;
+#pragma unroll
+        for (int c = 0; c < NV; ++c) xn[c][k] = rf_widen(cy.src[c], cy.type[c], i);
       }
     }
   };
@@ -330,10 +346,13 @@ __device__ __forceinline__ void rp_scatter8_body(
   for (int64_t t0 = a0; t0 < c1; t0 += TILE) {
     K kc[IT];
     int32_t vc[IT];
+    int64_t xc[XV][IT];
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
       kc[k] = kn[k];
       vc[k] = vn[k];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) xc[c][k] = xn[c][k];
     }
     if (t0 + TILE < c1) load_tile(t0 + TILE);
     const int64_t seg = t0 + (int64_t)w * WROWS;
@@ -390,6 +409,8 @@ __device__ __forceinline__ void rp_scatter8_body(
         stage_k[pos] = ov[k];
         stage_v[pos] = vc[k];
         sb[pos] = (uint8_t)d[k];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) stage_x[c][pos] = xc[c][k];
       }
     }
     // the tile's rows: those of [t0, t0 + TILE) inside [c0, c1)
@@ -401,20 +422,23 @@ __device__ __forceinline__ void rp_scatter8_body(
       const uint32_t o = gb[sb[s]] + (uint32_t)s;
       kout[o] = stage_k[s];
       vout[o] = stage_v[s];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) cy.dst[c][o] = stage_x[c][s];
     }
     for (int x = tid; x < W * B; x += NT) wh[x] = 0;
     __syncthreads();
   }
 }
 
-template <typename K, typename KO, int MODE, int NT, int IT>
+template <typename K, typename KO, int MODE, int NT, int IT, int NV = 0>
 __global__ __launch_bounds__(NT) void rp_scatter8(
     const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t n, int64_t crows,
     int bits, int shift, int64_t nchunks, const int64_t* __restrict__ offs,
-    KO* __restrict__ kout, int32_t* __restrict__ vout) {
+    KO* __restrict__ kout, int32_t* __restrict__ vout, Carry cy) {
   const int64_t c0 = (int64_t)blockIdx.x * crows;
-  rp_scatter8_body<K, KO, MODE, NT, IT>(keys, vals, c0, c0 + crows < n ? c0 + crows : n, bits,
-                                        shift, offs + blockIdx.x, nchunks, kout, vout, 0);
+  rp_scatter8_body<K, KO, MODE, NT, IT, NV>(keys, vals, c0, c0 + crows < n ? c0 + crows : n,
+                                            bits, shift, offs + blockIdx.x, nchunks, kout, vout,
+                                            0, cy);
 }
 
 template <int NT, int IT>
@@ -428,7 +452,7 @@ __global__ __launch_bounds__(NT) void rp_scatter8_seg(const int32_t* __restrict_
   const int64_t c0 = chunk * kSegChunkRows;
   rp_scatter8_body<int32_t, int32_t, RP_HASH_KEY, NT, IT>(
       keys + g.in_off, nullptr, c0, c0 + kSegChunkRows < g.n ? c0 + kSegChunkRows : g.n, g.bits,
-      g.shift, offs + g.hist_off + chunk, g.nchunks, kout, vout, g.delta);
+      g.shift, offs + g.hist_off + chunk, g.nchunks, kout, vout, g.delta, Carry{});
 }
 
 __global__ void rp_offsets(const int64_t* __restrict__ offs, int64_t nchunks, int B, int64_t n,
@@ -495,7 +519,7 @@ template <typename K, typename KO, int MODE, int NT, int IT>
 void launch8(const K* keys, const int32_t* vals, int64_t n, int64_t crows, int bits, int shift,
              int64_t nchunks, const int64_t* offs, KO* kout, int32_t* vout, hipStream_t st) {
   rp_scatter8<K, KO, MODE, NT, IT><<<(unsigned)nchunks, NT, 0, st>>>(
-      keys, vals, n, crows, bits, shift, nchunks, offs, kout, vout);
+      keys, vals, n, crows, bits, shift, nchunks, offs, kout, vout, Carry{});
 }
 
 template <typename K, typename KO, int MODE, int LOGB>
@@ -746,18 +770,19 @@ __global__ void rp_level_counts(const int32_t* __restrict__ J, int64_t nc, int l
 // Second pass: unit u = (L, g) is the first-pass output rows of low part
 // L from chunks [g * group, (g + 1) * group); its bucket H starts at
 // offs2[(H << lo | L) * ng + g].
-template <typename K, typename KO, int MODE, int NT, int IT>
+template <typename K, typename KO, int MODE, int NT, int IT, int NV = 0>
 __global__ __launch_bounds__(NT) void rp_scatter8_units(
     const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t n, int lo, int64_t nc,
     int64_t ng, int group, const int64_t* __restrict__ offs1, int bits, int shift,
-    const int64_t* __restrict__ offs2, KO* __restrict__ kout, int32_t* __restrict__ vout) {
+    const int64_t* __restrict__ offs2, KO* __restrict__ kout, int32_t* __restrict__ vout,
+    Carry cy) {
   const int64_t u = blockIdx.x;
   const int64_t L = u / ng, g = u - L * ng;
   const int64_t c0 = offs1[L * nc + g * group];
   const int64_t c1 = (g + 1) * group < nc ? offs1[L * nc + (g + 1) * group]
                                           : (L + 1 < (1LL << lo) ? offs1[(L + 1) * nc] : n);
-  rp_scatter8_body<K, KO, MODE, NT, IT>(keys, vals, c0, c1, bits, shift, offs2 + u,
-                                        (1LL << lo) * ng, kout, vout, 0);
+  rp_scatter8_body<K, KO, MODE, NT, IT, NV>(keys, vals, c0, c1, bits, shift, offs2 + u,
+                                            (1LL << lo) * ng, kout, vout, 0, cy);
 }
 
 __global__ void rp_level_offsets(const int64_t* __restrict__ offs2, int64_t ng, int P,
@@ -771,7 +796,8 @@ void launch_units(const K* keys, const int32_t* vals, int64_t n, const LvPlan& p
                   const int64_t* offs1, int bits, int shift, const int64_t* offs2, KO* kout,
                   int32_t* vout, hipStream_t st) {
   rp_scatter8_units<K, KO, MODE, NT, IT><<<(unsigned)((1LL << p.lo) * p.ng), NT, 0, st>>>(
-      keys, vals, n, p.lo, p.nc, p.ng, p.group, offs1, bits, shift, offs2, kout, vout);
+      keys, vals, n, p.lo, p.nc, p.ng, p.group, offs1, bits, shift, offs2, kout, vout,
+      Carry{});
 }
 
 template <typename K, typename KO, int MODE>
@@ -861,7 +887,108 @@ size_t level_ws(int64_t n, int bits) {
          align256((size_t)(n > 0 ? n : 1) * 4);
 }
 
+// ------------------------------------------- partition with carried values --
+//
+// The join's probe side when residual operands ride along: the hashed-out
+// partition (mix64(key), row id) with NV value columns.  A tile of 1024 x 4
+// rows keeps the staged values (8 B a row each) within the LDS next to the
+// hashes and row ids.
+constexpr int kCarryNT = 1024, kCarryIT = 4;
+
+template <int NV>
+int carry_run(const int64_t* keys, int64_t n, int bits, const Carry& first, const Carry& second,
+              uint64_t* tk, int32_t* tv, uint64_t* h_out, int32_t* row_out, int64_t* boff,
+              void* ws, hipStream_t st, int who) {
+  if (bits <= kOnePassMax) {
+    RpWs r = carve_rp(ws, n, bits);
+    const int64_t crows = chunk_rows_of(bits);
+    const int64_t nchunks = n > 0 ? (n + crows - 1) / crows : 1;
+    {
+      KTimer _kt(kHistName[who], st);
+      rp_hist<int64_t, int64_t, RP_HASH_KEY, 8><<<(unsigned)nchunks, RpShape<8>::NT, 0, st>>>(
+          keys, n, crows, bits, 0, nchunks, r.hist);
+    }
+    if (launch_check("radix_partition hist")) return 1;
+    if (scan_i32_to_i64_excl(r.hist, r.offs, nchunks << bits, nullptr, r.scan, st)) return 1;
+    if (n > 0) {
+      KTimer _kt(kScatterName[who], st);
+      rp_scatter8<int64_t, int64_t, RP_HASH_OUT, kCarryNT, kCarryIT, NV>
+          <<<(unsigned)nchunks, kCarryNT, 0, st>>>(keys, nullptr, n, crows, bits, 0, nchunks,
+                                                   r.offs, (int64_t*)h_out, row_out, first);
+    }
+    if (launch_check("radix_partition scatter")) return 1;
+    rp_offsets<<<grid_for((1 << bits) + 1, 256), 256, 0, st>>>(r.offs, nchunks, 1 << bits, n,
+                                                               boff);
+    return launch_check("radix_partition offsets");
+  }
+  const LvPlan p = level_plan(n, bits);
+  const int P = 1 << bits;
+  LvWs w = carve_lv(ws, n, bits);
+  {
+    KTimer _kt(kHistName[who], st);
+    rp_hist<int64_t, int64_t, RP_HASH_KEY, 12, true><<<(unsigned)p.nc, RpShape<12>::NT, 0, st>>>(
+        keys, n, p.crows, bits, 0, p.nc, w.J);
+  }
+  if (launch_check("radix_partition level hist")) return 1;
+  const int64_t e1 = (1LL << p.lo) * p.nc, e2 = (int64_t)P * p.ng;
+  rp_level_counts<<<grid_for(e1 + e2, 256) < 8192 ? grid_for(e1 + e2, 256) : 8192, 256, 0, st>>>(
+      w.J, p.nc, p.lo, p.hi, p.ng, p.group, w.cnt1, w.cnt2);
+  if (launch_check("radix_partition level counts")) return 1;
+  if (scan_i32_to_i64_excl(w.cnt1, w.offs1, e1, nullptr, w.scan, st) ||
+      scan_i32_to_i64_excl(w.cnt2, w.offs2, e2, nullptr, w.scan, st))
+    return 1;
+  if (n > 0) {
+    {
+      KTimer _kt(kScatterName[who], st);
+      rp_scatter8<int64_t, int64_t, RP_HASH_OUT, kCarryNT, kCarryIT, NV>
+          <<<(unsigned)p.nc, kCarryNT, 0, st>>>(keys, nullptr, n, p.crows, p.lo, p.hi, p.nc,
+                                                w.offs1, (int64_t*)tk, tv, first);
+    }
+    KTimer _kt(kScatter2Name[who], st);
+    rp_scatter8_units<int64_t, int64_t, RP_RAW, kCarryNT, kCarryIT, NV>
+        <<<(unsigned)((1LL << p.lo) * p.ng), kCarryNT, 0, st>>>(
+            (const int64_t*)tk, tv, n, p.lo, p.nc, p.ng, p.group, w.offs1, p.hi, 64 - p.hi,
+            w.offs2, (int64_t*)h_out, row_out, second);
+  }
+  if (launch_check("radix_partition level scatter")) return 1;
+  rp_level_offsets<<<grid_for(P + 1, 256), 256, 0, st>>>(w.offs2, p.ng, P, n, boff);
+  return launch_check("radix_partition level offsets");
+}
+
 }  // namespace
+
+// The join's probe side with nv (1 or 2) value columns carried: rows as
+// (mix64(key), row id) by the top `bits` bits of the hash, stable, each
+// row's values src[c] (PF_COL_* type[c], widened to 64 bits) into dst[c]
+// at its output row.  Two-level partitions (9-12 bits) write their first
+// pass into (tk, tv, tx).  ws: radix_partition_ws(n, bits) for bits <= 8,
+// else radix_level_ws(n, bits).
+int radix_partition_carry(const int64_t* keys, int64_t n, int bits, int nv,
+                          const void* const* src, const int32_t* type, int64_t* const* dst,
+                          uint64_t* tk, int32_t* tv, int64_t* const* tx, uint64_t* h_out,
+                          int32_t* row_out, int64_t* boff, void* ws, size_t ws_bytes,
+                          hipStream_t st, int who) {
+  if (bits < 0 || bits > 12) return fail("radix_partition_carry: bits must be in [0, 12]");
+  if (n < 0 || n >= (1LL << 31)) return fail("radix_partition_carry: n must be below 2^31");
+  if (nv < 1 || nv > 2) return fail("radix_partition_carry: one or two carried columns");
+  const size_t need = bits <= kOnePassMax ? carve_rp(nullptr, n, bits).bytes
+                                          : carve_lv(nullptr, n, bits).bytes;
+  if (ws_bytes < need) return fail("radix_partition_carry: workspace too small");
+  Carry first{}, second{};
+  for (int c = 0; c < nv; ++c) {
+    first.src[c] = src[c];
+    first.type[c] = type[c];
+    // one pass: straight to the output; two: through the first pass's
+    // int64 copies
+    first.dst[c] = bits <= kOnePassMax ? dst[c] : tx[c];
+    second.src[c] = tx ? tx[c] : nullptr;
+    second.type[c] = PF_COL_I64;
+    second.dst[c] = dst[c];
+  }
+  if (nv == 1)
+    return carry_run<1>(keys, n, bits, first, second, tk, tv, h_out, row_out, boff, ws, st, who);
+  return carry_run<2>(keys, n, bits, first, second, tk, tv, h_out, row_out, boff, ws, st, who);
+}
 
 // Host-side use by the other tiers of the library.
 int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,

@@ -4,7 +4,11 @@ multi-match join that stands in for the reference's sort/search core
 (operators/hashjoin.py:380-441) on non-unique builds.
 
 ``HashJoin`` keeps its workspace between calls (a serving loop or a
-benchmark reuses it); ``hash_join`` is the one-shot form.
+benchmark reuses it); ``hash_join`` is the one-shot form.  Either takes a
+residual program (``operators.residual.compile_conditions`` over the build
+columns as the left side and the probe columns as the right), evaluated on
+every candidate pair inside the count and write passes
+(``pf_hash_join_residual_*``): only the kept pairs are counted and written.
 """
 from __future__ import annotations
 
@@ -58,35 +62,80 @@ class HashJoin:
         if not 0 <= self.bits <= MAX_BITS:
             raise ValueError(f"bits must be in [0, {MAX_BITS}]")
         self.device = torch.device(device) if device is not None else torch.device("cuda")
+        if self.device.index is None:
+            self.device = torch.device(self.device.type, torch.cuda.current_device())
         lib = _lib.load()
         self.ws_bytes = int(lib.pf_hash_join_workspace_bytes(self.nb, self.np, self.bits,
                                                              self.mode))
         self.ws = _lib.workspace(self.ws_bytes, self.device)
-        self.host = torch.empty(3, dtype=torch.int64, pin_memory=True)
+        self.host = torch.empty(4, dtype=torch.int64, pin_memory=True)
         self.n_items = 0
         self.pairs = 0
+        self.candidates = 0
+        self.residual = None
         self.prepared = False
 
-    def prepare(self, build_keys: torch.Tensor, probe_keys: torch.Tensor) -> Optional[int]:
+    def _program_args(self, prog):
+        flat = [w for ins in prog.ins for w in ins]
+        return (_lib.void_array([t.data_ptr() for t in prog.cols]),
+                ctypes.cast(_lib.i32_array(prog.types), ctypes.c_void_p),
+                ctypes.cast(_lib.i32_array(prog.sides), ctypes.c_void_p),
+                ctypes.cast(_lib.i64_array([t.numel() for t in prog.cols]), ctypes.c_void_p),
+                len(prog.cols), ctypes.cast(_lib.i64_array(flat), ctypes.c_void_p),
+                len(prog.ins))
+
+    def _residual_workspace(self, prog) -> None:
+        if not self.ordered:
+            raise ValueError("HashJoin: a residual program needs an ordered plan")
+        for t in prog.cols:
+            if t.device != self.device:
+                raise ValueError(f"HashJoin: a residual column on {t.device}, the join on "
+                                 f"{self.device}")
+        n_build = sum(1 for sd in prog.sides if sd == 0)
+        n_probe = len(prog.sides) - n_build
+        need = int(_lib.load().pf_hash_join_residual_workspace_bytes(
+            self.nb, self.np, self.bits, self.mode, n_probe, n_build))
+        if need > self.ws.numel():
+            self.ws = _lib.workspace(need, self.device)
+        self.ws_bytes = max(self.ws_bytes, need)
+
+    def prepare(self, build_keys: torch.Tensor, probe_keys: torch.Tensor,
+                residual=None) -> Optional[int]:
         """Partitions both sides and, when ordered, counts the pairs (two
         host waits; three when a multi-match build bucket is larger than the
         LDS sort takes and is sorted in HBM; one when not ordered).  Returns
-        the pair count, or None when not ordered."""
+        the pair count, or None when not ordered.  With a ``residual``
+        program (ordered plans) the count is of the pairs it keeps, and
+        ``candidates`` holds the pairs before it."""
         _check_keys(build_keys, "build keys")
         _check_keys(probe_keys, "probe keys")
         if build_keys.numel() != self.nb or probe_keys.numel() != self.np:
             raise ValueError("HashJoin: sides differ from the plan's sizes")
         lib = _lib.load()
+        self.residual = None
+        if residual is not None:
+            self._residual_workspace(residual)
+            with torch.cuda.device(self.device):
+                _lib.check(lib.pf_hash_join_residual_prepare(
+                    _lib.ptr(build_keys), self.nb, _lib.ptr(probe_keys), self.np, self.bits,
+                    self.mode, *self._program_args(residual),
+                    ctypes.c_void_p(self.host.data_ptr()), _lib.ptr(self.ws), self.ws_bytes,
+                    _lib.stream_of(self.device)))
+            pairs, _, self.n_items, self.candidates = (int(v) for v in self.host.tolist())
+            self.residual = residual
+            self.prepared = True
+            self.pairs = pairs
+            return self.pairs
         with torch.cuda.device(self.device):
             _lib.check(lib.pf_hash_join_prepare(
                 _lib.ptr(build_keys), self.nb, _lib.ptr(probe_keys), self.np, self.bits,
                 self.mode, ctypes.c_void_p(self.host.data_ptr()), _lib.ptr(self.ws),
                 self.ws_bytes, _lib.stream_of(self.device)))
-        pairs, _, self.n_items = (int(v) for v in self.host.tolist())
+        pairs, _, self.n_items = (int(v) for v in self.host.tolist()[:3])
         self.prepared = True
         if not self.ordered:
             return None
-        self.pairs = pairs
+        self.pairs = self.candidates = pairs
         return self.pairs
 
     def write(self, out_probe: torch.Tensor, out_build: torch.Tensor,
@@ -110,6 +159,16 @@ class HashJoin:
             if t.dtype != torch.int64 or t.numel() < need:
                 raise ValueError(f"HashJoin: payload outputs are int64 of {need} rows")
         lib = _lib.load()
+        if self.residual is not None:
+            with torch.cuda.device(self.device):
+                _lib.check(lib.pf_hash_join_residual_write(
+                    self.nb, self.np, self.bits, self.mode, self.n_items,
+                    _lib.void_array([t.data_ptr() for t in payloads]), len(payloads),
+                    _lib.ptr(out_probe), _lib.ptr(out_build),
+                    _lib.void_array([t.data_ptr() for t in pay_out]),
+                    *self._program_args(self.residual), _lib.ptr(self.ws), self.ws_bytes,
+                    _lib.stream_of(self.device)))
+            return self.pairs
         with torch.cuda.device(self.device):
             _lib.check(lib.pf_hash_join_write(
                 self.nb, self.np, self.bits, self.mode, self.n_items,
@@ -125,14 +184,15 @@ class HashJoin:
 
 def hash_join(build_keys: torch.Tensor, probe_keys: torch.Tensor,
               payloads: Sequence[torch.Tensor] = (), unique: bool = True,
-              bits: Optional[int] = None, ordered: bool = True
+              bits: Optional[int] = None, ordered: bool = True, residual=None
               ) -> Tuple[torch.Tensor, torch.Tensor, List[torch.Tensor]]:
     """(probe rows, build rows, gathered build payloads) of an inner
     equi-join of int64 keys, in the deterministic partition order (or, for
-    a unique build with ``ordered=False``, in no fixed order)."""
+    a unique build with ``ordered=False``, in no fixed order); with a
+    ``residual`` program only the pairs it keeps."""
     plan = HashJoin(build_keys.numel(), probe_keys.numel(), unique, bits, build_keys.device,
-                    ordered=ordered)
-    n = plan.prepare(build_keys, probe_keys)
+                    ordered=ordered or residual is not None)
+    n = plan.prepare(build_keys, probe_keys, residual)
     dev = build_keys.device
     rows = n if n is not None else probe_keys.numel()
     out_p = torch.empty(rows, dtype=torch.int64, device=dev)

@@ -21,7 +21,9 @@ three routes:
 ``pairs``    everything else: every matching pair from the partitioned
              multi-match LDS-table join (``pageflow_amd.join.hash_join``),
              which gives the multiset of pairs the reference's sort/search
-             core gives, then the residual on the pairs.
+             core gives.  The residual's top-level AND operands that compile
+             to the device program are evaluated inside the join's count and
+             write passes (only kept pairs are made); the rest on the pairs.
 
 Residuals on pairs run as one compiled device program
 (``operators.residual``) where the conditions compile, else on the
@@ -362,13 +364,31 @@ def _sort_search_pairs(tensors_left, tensors_right, left, right, condition_list,
         none = torch.empty(0, dtype=torch.int64, device=device)
         return none, none.clone()
     lk, rk = _join_keys(left, right)
-    right_rows, left_rows, _ = _join.hash_join(lk.contiguous(), rk.contiguous(), unique=False)
-    if condition_list and left_rows.numel() > INT_ROWS_MAX:
-        raise RuntimeError(f"{name}: a core call of {n_left} left and {n_right} right rows "
-                           f"makes {left_rows.numel()} pairs, over the {INT_ROWS_MAX} its "
-                           f"residual conditions take")
-    return _filter_pairs(tensors_left, tensors_right, condition_list, left_rows, right_rows,
-                         device)
+    lk, rk = lk.contiguous(), rk.contiguous()
+
+    def over_limit(n_pairs: int) -> None:
+        if condition_list and n_pairs > INT_ROWS_MAX:
+            raise RuntimeError(f"{name}: a core call of {n_left} left and {n_right} right rows "
+                               f"makes {n_pairs} pairs, over the {INT_ROWS_MAX} its residual "
+                               f"conditions take")
+
+    parts = _conjuncts(_conjunction(condition_list)) if condition_list else []
+    native = [c for c in parts if _compiles(tensors_left, tensors_right, c)]
+    if not native:
+        right_rows, left_rows, _ = _join.hash_join(lk, rk, unique=False)
+        over_limit(left_rows.numel())
+        return _filter_pairs(tensors_left, tensors_right, condition_list, left_rows,
+                             right_rows, device)
+    # the compiled terms inside the join (left = build, right = probe)
+    prog = _residual.compile_conditions(tensors_left, tensors_right, native)
+    plan = _join.HashJoin(n_left, n_right, unique=False, device=lk.device)
+    kept = plan.prepare(lk, rk, residual=prog)
+    over_limit(plan.candidates)
+    right_rows = torch.empty(kept, dtype=torch.int64, device=lk.device)
+    left_rows = torch.empty(kept, dtype=torch.int64, device=lk.device)
+    plan.write(right_rows, left_rows)
+    other = [c for c in parts if not any(c is n for n in native)]
+    return _filter_pairs(tensors_left, tensors_right, other, left_rows, right_rows, device)
 
 
 # ----------------------------------------------------------------- tails --
