@@ -501,7 +501,7 @@ def bench_skew(args, dev, emit: bool = True):
            "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in prof.items()}}
     log(f"skew: inner {t_inner * 1e3:.2f} ms ({pairs} pairs), semi {t_semi * 1e3:.2f} ms, "
         f"anti {t_anti * 1e3:.2f} ms")
-    res["residual"] = bench_residual(args, dev, ob, op, nb, np_, timed)
+    res["residual"] = bench_residual(args, dev, ob, op, nb, np_, timed, build, probe, t_inner)
     if not args.no_cpu:
         import oracle
         bk = build[:2_500_000].cpu().numpy()
@@ -530,16 +530,26 @@ def bench_skew(args, dev, emit: bool = True):
     return res
 
 
-def bench_residual(args, dev, ob, op, nb: int, np_: int, timed):
+def bench_residual(args, dev, ob, op, nb: int, np_: int, timed, build=None, probe=None,
+                   t_inner=None):
     """A residual condition on the skewed join's pairs: build payload <
     probe payload (int64, values in [0, 1000)), evaluated on every pair and
-    the kept pairs compacted in pair order.  The fused filter (one
+    the kept pairs compacted in pair order.  The pair filter (one
     evaluation pass, a scan, one compaction pass) against the reference's
     torch steps on the same GPU (two gathers, lt, the mask AND, nonzero, two
     index selects: operators/hashjoin.py:428-439).  Algorithmic bytes: the
     evaluation 32 B a pair (two indexes, two operands) + a mask bit, the
     compaction a mask bit + 32 B a kept pair (two indexes read, two
-    written)."""
+    written).
+
+    ``join_fused``: the same condition evaluated inside the join
+    (``HashJoin.prepare(residual=...)``: the probe operand carried through
+    the partition, the build operand staged in LDS, only kept pairs
+    written), timed against the join followed by the filter, and checked
+    equal to the filter's pairs as a multiset (the condition takes the
+    range form: each build run sorted by the build operand, a probe row's
+    kept rows one sub-run).  Algorithmic bytes: keys and operands of both
+    sides once (16 B a row) + 16 B a kept pair."""
     import torch
     from pageflow_amd import _lib
     from pageflow_amd.operators import residual
@@ -588,6 +598,39 @@ def bench_residual(args, dev, ob, op, nb: int, np_: int, timed):
         out["rf_eval_frac"] = out["rf_eval_GBps"] / HBM_PEAK_GBPS
     if "rf_compact" in kms:
         out["rf_compact_GBps"] = b_comp / (kms["rf_compact"] * 1e-3) / 1e9
+    if build is not None:
+        from pageflow_amd.join import HashJoin
+        plan = HashJoin(nb, np_, unique=False, device=dev)
+        fo = {}
+
+        def fused():
+            n = plan.prepare(build, probe, residual=prog)
+            if "p" not in fo or fo["p"].numel() < n:
+                fo["p"] = torch.empty(n, dtype=torch.int64, device=dev)
+                fo["b"] = torch.empty(n, dtype=torch.int64, device=dev)
+            plan.write(fo["p"], fo["b"])
+            return n
+
+        _lib.profile(True)
+        t_join_fused, n_fused = timed(fused, args.steps)
+        prof_f = _lib.profile_read()
+        _lib.profile(False)
+        assert n_fused == kept and plan.candidates == n, (n_fused, kept, plan.candidates, n)
+        # the same pairs (the range form orders a probe row's build rows by
+        # the build operand): sorted (build, probe) keys compared
+        got = torch.sort(fo["b"][:kept] * np_ + fo["p"][:kept]).values
+        want = torch.sort(kb * np_ + kp).values
+        assert torch.equal(got, want), "the residual inside the join differs from the filter"
+        del got, want
+        b_jf = 16.0 * (nb + np_) + 16.0 * kept
+        out["join_fused"] = {
+            "ms": t_join_fused * 1e3, "join_then_filter_ms": (t_inner + t_fused) * 1e3,
+            "speedup": (t_inner + t_fused) / t_join_fused, "kept": kept,
+            "GBps": b_jf / t_join_fused / 1e9, "frac": b_jf / t_join_fused / 8e12,
+            "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in prof_f.items()}}
+        log(f"residual inside the join: {t_join_fused * 1e3:.2f} ms against join + filter "
+            f"{(t_inner + t_fused) * 1e3:.2f} ms")
+        del plan, fo
     del kb, kp, bpay, ppay
     torch.cuda.empty_cache()
     return out

@@ -160,12 +160,13 @@ __global__ __launch_bounds__(1024) void hj_items(const int64_t* __restrict__ bof
 __global__ __launch_bounds__(SORT_NT) void hj_sort_buckets(uint64_t* __restrict__ bh,
                                                            int32_t* __restrict__ brow,
                                                            const int64_t* __restrict__ boff,
-                                                           int cap) {
+                                                           int cap, int min_n) {
   extern __shared__ uint64_t sort_lds[];
   uint64_t* sh = sort_lds;
   int32_t* sr = (int32_t*)(sort_lds + cap);
   const int64_t b0 = boff[blockIdx.x], n = boff[blockIdx.x + 1] - b0;
-  if (n <= 1 || n > cap) return;  // larger buckets are sorted in HBM
+  // larger buckets are sorted in HBM; those up to min_n by another sort
+  if (n <= 1 || n > cap || n <= min_n) return;
   int m = 2;
   while (m < n) m <<= 1;
   for (int i = threadIdx.x; i < m; i += SORT_NT) {
@@ -199,6 +200,65 @@ __global__ __launch_bounds__(SORT_NT) void hj_sort_buckets(uint64_t* __restrict_
   }
   for (int i = threadIdx.x; i < n; i += SORT_NT) {
     bh[b0 + i] = sh[i];
+    brow[b0 + i] = sr[i];
+  }
+}
+
+// Build bucket b's (hash, key, row) triples sorted by hash, then key, then
+// row, for buckets of at most `cap` rows (a power of two <= KEYED_CAP):
+// the residual's range form keeps a probe row's kept build rows as one
+// sub-run of its key's run.
+constexpr int KEYED_CAP = 4096;
+__global__ __launch_bounds__(SORT_NT) void hj_sort_keyed(uint64_t* __restrict__ bh,
+                                                         int64_t* __restrict__ bkey,
+                                                         int32_t* __restrict__ brow,
+                                                         const int64_t* __restrict__ boff,
+                                                         int cap) {
+  extern __shared__ uint64_t keyed_lds[];
+  uint64_t* sh = keyed_lds;
+  int64_t* sk = (int64_t*)(keyed_lds + cap);
+  int32_t* sr = (int32_t*)(keyed_lds + 2 * cap);
+  const int64_t b0 = boff[blockIdx.x], n = boff[blockIdx.x + 1] - b0;
+  if (n <= 1 || n > cap) return;
+  int m = 2;
+  while (m < n) m <<= 1;
+  for (int i = threadIdx.x; i < m; i += SORT_NT) {
+    if (i < n) {
+      sh[i] = bh[b0 + i];
+      sk[i] = bkey[b0 + i];
+      sr[i] = brow[b0 + i];
+    } else {
+      sh[i] = ~0ULL;
+      sk[i] = INT64_MAX;
+      sr[i] = INT32_MAX;
+    }
+  }
+  __syncthreads();
+  for (int k = 2; k <= m; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < (m >> 1); i += SORT_NT) {
+        const int lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+        const int hi = lo + j;
+        const bool up = (lo & k) == 0;
+        const uint64_t a = sh[lo], c = sh[hi];
+        const int64_t ka = sk[lo], kc = sk[hi];
+        const int32_t ra = sr[lo], rc = sr[hi];
+        const bool gt = a > c || (a == c && (ka > kc || (ka == kc && ra > rc)));
+        if (gt == up) {
+          sh[lo] = c;
+          sh[hi] = a;
+          sk[lo] = kc;
+          sk[hi] = ka;
+          sr[lo] = rc;
+          sr[hi] = ra;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < n; i += SORT_NT) {
+    bh[b0 + i] = sh[i];
+    bkey[b0 + i] = sk[i];
     brow[b0 + i] = sr[i];
   }
 }
@@ -515,13 +575,67 @@ struct HjRes {
   int64_t* cand;         // count pass: candidate pairs before the residual (one counter)
 };
 
-// One (round slot)'s candidates of a wave, a window of 64 at a time; WRITE
-// stores the kept ones at base + the kept candidates before them.  Returns
-// the wave's kept count (uniform).  The s_* arrays hold the wave's lanes:
-// candidate offset, first chunk row of the run, probe row, carried values.
-template <bool WRITE>
-__device__ __forceinline__ int res_sweep(const HjParams& p, const JoinResidual& R, PairCtx c,
-                                         int T, int wb, const int32_t* s_ex, const int32_t* s_hi,
+// The program on one pair: its fast form (decoded on the host; c.pv0 and
+// c.pv1 then hold the row's resolved operands, jr_fast_row), or the
+// LDS-decoded interpreter; in the range form, every candidate of a
+// narrowed run (KM_ALL) or "build key rrel row key" (KM_KEY: c.bv holds the
+// build keys, c.pv0 the row's key).
+enum : int { KM_INTERP = 0, KM_FAST = 1, KM_ALL = 2, KM_KEY = 3 };
+template <int KM>
+__device__ __forceinline__ bool res_keep(const JoinResidual& R, const JrLds* d,
+                                         const PairCtx& c) {
+  if constexpr (KM == KM_FAST)
+    return jr_fast_eval(R.fast, (R.fast.a_bv | R.fast.b_bv) ? c.bv[c.li] : 0, c.pv0, c.pv1);
+  else if constexpr (KM == KM_INTERP)
+    return jr_eval_lds(R, d, c);
+  else if constexpr (KM == KM_ALL)
+    return true;
+  else
+    return jr_key_rel(R.fast.rrel, R.fast.f64 != 0, c.bv[c.li], c.pv0);
+}
+
+// Candidates [lo, hi) of a run of n build keys k (ascending) that "build
+// rrel x" keeps, x the row's key: a prefix, a suffix or the equal range;
+// in the float64 domain NaN keys sort last and are never kept.
+__device__ __forceinline__ void key_range(const int64_t* k, int n, int rel, bool f64, int64_t x,
+                                          int* lo, int* hi) {
+  auto bound = [&](int64_t v, bool upper) {
+    int a = 0, b = n;
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (upper ? k[mid] <= v : k[mid] < v)
+        a = mid + 1;
+      else
+        b = mid;
+    }
+    return a;
+  };
+  if (f64 && x == kNanKey) {
+    *lo = *hi = 0;
+    return;
+  }
+  const int end = f64 ? bound(kNanKey, false) : n;
+  switch (rel) {
+    case PF_REL_LT: *lo = 0; *hi = bound(x, false); break;
+    case PF_REL_LE: *lo = 0; *hi = bound(x, true); break;
+    case PF_REL_GT: *lo = bound(x, true); *hi = end; break;
+    case PF_REL_GE: *lo = bound(x, false); *hi = end; break;
+    default: *lo = bound(x, false); *hi = bound(x, true); break;  // EQ
+  }
+  if (*hi < *lo) *hi = *lo;
+}
+
+// One (round slot)'s candidates of a wave, a window of 64 at a time: lane j
+// takes the wave's candidate j (its source lane found by six halving steps
+// over the lanes' candidate offsets staged in LDS, which never decrease) and
+// evaluates the program on it, so a window's 64 lanes are busy however the
+// runs are skewed and the evaluator's uniform work is paid once a window.
+// WRITE stores the kept candidates, compacted by ballot, at base + the kept
+// candidates before them.  Returns the wave's kept count.
+template <int KM, bool WRITE>
+__device__ __forceinline__ int res_sweep(const HjParams& p, const JoinResidual& R,
+                                         const JrLds* d, PairCtx c, int T, int wb,
+                                         const int32_t* s_ex, const int32_t* s_hi,
                                          const int32_t* s_pr, const int64_t* s_p0,
                                          const int64_t* s_p1, const int32_t* rows,
                                          int64_t base) {
@@ -529,22 +643,20 @@ __device__ __forceinline__ int res_sweep(const HjParams& p, const JoinResidual& 
   int kept = 0, lo0 = 0;
   for (int j0 = 0; j0 < T; j0 += kWave) {
     const int j = j0 + lane;
-    bool keep = false;
     int lo = lo0;  // the last lane whose run starts at or before j
+#pragma unroll
+    for (int step = kWave / 2; step > 0; step >>= 1) {
+      const int nx = lo + step;
+      const int ex = s_ex[wb + (nx < kWave ? nx : kWave - 1)];  // branch-free, clamped
+      lo = (nx < kWave && ex <= j) ? nx : lo;
+    }
+    bool keep = false;
     if (j < T) {
-      int hi = kWave;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_ex[wb + mid] <= j)
-          lo = mid;
-        else
-          hi = mid;
-      }
       c.li = s_hi[wb + lo] + (j - s_ex[wb + lo]);
       c.prow = s_pr[wb + lo];
       c.pv0 = s_p0[wb + lo];
       c.pv1 = s_p1[wb + lo];
-      keep = jr_eval(R, c);
+      keep = res_keep<KM>(R, d, c);
     }
     const uint64_t bal = ballot(keep);
     if (WRITE && keep) {
@@ -554,15 +666,42 @@ __device__ __forceinline__ int res_sweep(const HjParams& p, const JoinResidual& 
       for (int q = 0; q < p.npay; ++q) p.pay_out[q][o] = p.pay_src[q][c.cb + c.li];
     }
     kept += __popcll(bal);
-    lo0 = __shfl(lo, kWave - 1, kWave);
+    lo0 = __builtin_amdgcn_readlane(lo, kWave - 1);
   }
   return kept;
 }
 
+// a[k] of a small register array for a wave-uniform k that the compiler
+// cannot fold: a chain of selects, so a loop over k need not be unrolled.
+template <int N, typename T>
+__device__ __forceinline__ T pick(const T (&a)[N], int k) {
+  T v = a[0];
+#pragma unroll
+  for (int q = 1; q < N; ++q) v = k == q ? a[q] : v;
+  return v;
+}
+template <int N, typename T>
+__device__ __forceinline__ void put(T (&a)[N], int k, T v) {
+#pragma unroll
+  for (int q = 0; q < N; ++q) a[q] = k == q ? v : a[q];
+}
+
 // W1 = false: count the kept pairs of each item (and every candidate, into
-// x.cand); true: write them at the scanned item positions.
-template <bool MULTI, bool W1>
+// x.cand); true: write them at the scanned item positions.  FAST: the
+// program's fast form (host-decoded, jr_eval_fast); else the interpreter
+// over the program decoded into LDS.  Unique builds evaluate a row's one
+// candidate in its lane; multi-match builds sweep each (slot, wave)'s
+// candidates in windows (res_sweep): the count pass once, the write pass
+// twice (kept totals for the offsets, then the stores).  The loops over a
+// round's slots are unrolled, which keeps the per-slot values in registers;
+// the evaluators are compact enough for the copies to fit the instruction
+// cache.
+template <bool MULTI, bool W1, int EV>
 __global__ __launch_bounds__(HJ_NT) void hj_probe_res(HjParams p, JoinResidual R, HjRes x) {
+  constexpr bool FAST = EV >= 1;
+  // the window sweeps' evaluation: the interpreter, the fast form, or in
+  // the range form the key comparison (buckets past the keyed sort)
+  constexpr int KM = EV == 0 ? KM_INTERP : EV == 1 ? KM_FAST : KM_KEY;
   constexpr int CAP = Lds<MULTI>::CAP, TS = Lds<MULTI>::TS;
   constexpr int IT = hr_it(MULTI), ROUND = HJ_NT * IT;
   constexpr int MT = MULTI ? HJ_NT : 1;
@@ -576,10 +715,15 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe_res(HjParams p, JoinResidual R
   __shared__ int32_t s_ex[MT], s_hi[MT], s_pr[MT];
   __shared__ int64_t s_p0[MT], s_p1[MT];
   __shared__ int64_t last_tot;
+  __shared__ JrLds dec[FAST ? 1 : 1];
   const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave, wb = w * kWave;
   const int64_t item = blockIdx.x;
   if (item >= p.n_items) return;
+  if (!FAST) jr_decode(R, dec);  // read after the chunk's first barrier
   const Item it = item_of(p, item);
+  // range form: a bucket the keyed sort ordered by (hash, key) narrows each
+  // run to its kept sub-run; larger buckets compare keys per candidate
+  const bool nar = EV == 2 && it.be - it.bs <= KEYED_CAP;
   int64_t pos = W1 ? p.item_pos[item] : 0;
   int64_t kept_n = 0, cand_n = 0;
   const bool has_p0 = x.pv[0] != nullptr, has_p1 = x.pv[1] != nullptr;
@@ -615,35 +759,27 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe_res(HjParams p, JoinResidual R
     c.rows = rows;
     c.cb = cb;
     for (int i0 = 0; i0 < np_item; i0 += ROUND) {
-      // unique: a lane's rows and values are loaded with the hashes; multi:
-      // they are loaded as each slot is staged (fewer live registers)
-      constexpr int QT = MULTI ? 1 : IT;
       uint64_t hv[IT];
-      int32_t pr[QT];
-      int64_t q0[QT], q1[QT];
+      int32_t pr[IT];
+      int64_t q0[IT], q1[IT];
       int hidx[IT], m[IT];
 #pragma unroll
       for (int k = 0; k < IT; ++k) {
         const int i = i0 + k * HJ_NT + tid;
         const int ic = i < np_item ? i : np_item - 1;
         hv[k] = ph[ic];
-        if constexpr (!MULTI) {
-          pr[k] = prw[ic];
-          q0[k] = has_p0 ? pv0[ic] : 0;
-          q1[k] = has_p1 ? pv1[ic] : 0;
+        pr[k] = prw[ic];
+        q0[k] = has_p0 ? pv0[ic] : 0;
+        q1[k] = has_p1 ? pv1[ic] : 0;
+        if constexpr (FAST) {  // from here: the row's resolved operands
+          const int64_t ra = jr_fast_row(R.fast.asel, R.fast.aconv, R.fast.a, q0[k], q1[k]);
+          const int64_t rb = jr_fast_row(R.fast.bsel, R.fast.bconv, R.fast.b, q0[k], q1[k]);
+          q0[k] = ra;
+          q1[k] = rb;
+          if constexpr (EV == 2)  // the range form: the row operand's key
+            q0[k] = jr_key(R.fast.xa ? ra : rb, R.fast.f64 != 0, false);
         }
       }
-      // stages slot k's lanes for the window sweeps (multi)
-      auto stage = [&](int k, int inw_k) {
-        const int i = i0 + k * HJ_NT + tid;
-        const int ic = i < np_item ? i : np_item - 1;
-        s_ex[tid] = inw_k;
-        s_hi[tid] = hidx[k];
-        s_pr[tid] = prw[ic];
-        s_p0[tid] = has_p0 ? pv0[ic] : 0;
-        s_p1[tid] = has_p1 ? pv1[ic] : 0;
-        wave_lds_sync();  // a wave reads only its own lanes' entries
-      };
 #pragma unroll
       for (int k = 0; k < IT; ++k) {
         const int i = i0 + k * HJ_NT + tid;
@@ -658,47 +794,76 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe_res(HjParams p, JoinResidual R
         m[k] = len;
         cand_n += len;
       }
-      int kc[IT];  // kept: unique, this lane's (0/1); multi, the wave's in the slot
-      int inw[IT];
-      if constexpr (!MULTI) {
+      // kept candidates: unique, this lane's (0 or 1); multi, the wave's in
+      // each slot
+      int kl[IT], inw[IT];
 #pragma unroll
-        for (int k = 0; k < IT; ++k) {
-          bool keep = false;
-          if (hidx[k] >= 0) {
-            c.li = hidx[k];
-            c.prow = pr[k];
-            c.pv0 = q0[k];
-            c.pv1 = q1[k];
-            keep = jr_eval(R, c);
+      for (int k = 0; k < IT; ++k) kl[k] = 0;
+      if constexpr (MULTI) {
+        if (nar) {
+#pragma unroll
+          for (int k = 0; k < IT; ++k) {
+            if (m[k] > 0) {
+              int lo, hi;
+              key_range(bv + hidx[k], m[k], R.fast.rrel, R.fast.f64 != 0, q0[k], &lo, &hi);
+              hidx[k] += lo;
+              m[k] = hi - lo;
+            }
           }
-          kc[k] = keep ? 1 : 0;
-          if (!W1) kept_n += kc[k];
         }
-        if (!W1) continue;
-#pragma unroll
-        for (int k = 0; k < IT; ++k) {
-          const uint64_t bal = ballot(kc[k] != 0);
-          inw[k] = mbcnt(bal);
-          if (lane == 0) cnt[k * HJ_W + w] = __popcll(bal);
-        }
-      } else {
 #pragma unroll
         for (int k = 0; k < IT; ++k) {
           const int incl = wave_incl_scan(m[k]);
           inw[k] = incl - m[k];
           m[k] = __shfl(incl, kWave - 1, kWave);  // from here: the wave's candidates
         }
+      }
+      // stages slot k's lanes for its window sweeps
+      auto stage = [&](int k) {
+        s_ex[tid] = pick(inw, k);
+        s_hi[tid] = pick(hidx, k);
+        s_pr[tid] = pick(pr, k);
+        s_p0[tid] = pick(q0, k);
+        s_p1[tid] = pick(q1, k);
+        wave_lds_sync();  // a wave reads only its own lanes' entries
+      };
 #pragma unroll
-        for (int k = 0; k < IT; ++k) {
-          stage(k, inw[k]);
-          kc[k] = res_sweep<false>(p, R, c, m[k], wb, s_ex, s_hi, s_pr, s_p0, s_p1, rows, 0);
-          wave_lds_sync();  // s_* are rewritten for the next slot
-          if (!W1 && lane == 0) kept_n += kc[k];
+      for (int k = 0; k < IT; ++k) {
+        int kk = 0;
+        if constexpr (MULTI) {
+          const int T = __builtin_amdgcn_readfirstlane(pick(m, k));
+          if (T > 0 && nar) {
+            kk = T;  // a narrowed run keeps every candidate
+          } else if (T > 0) {
+            stage(k);
+            kk = res_sweep<KM, false>(p, R, dec, c, T, wb, s_ex, s_hi, s_pr, s_p0, s_p1, rows,
+                                      0);
+            wave_lds_sync();  // s_* are rewritten for the next slot
+          }
+          if (!W1 && lane == 0) kept_n += kk;
+        } else {
+          if (pick(m, k) > 0) {
+            c.li = pick(hidx, k);
+            c.prow = pick(pr, k);
+            c.pv0 = pick(q0, k);
+            c.pv1 = pick(q1, k);
+            kk = res_keep<KM>(R, dec, c) ? 1 : 0;
+          }
+          if (!W1) kept_n += kk;
         }
-        if (!W1) continue;
-        if (lane == 0) {
+        put(kl, k, kk);
+      }
+      if (!W1) continue;
+      // offsets: within a wave by lane (unique); across (slot, wave) by one
+      // wave's scan of the kept totals
 #pragma unroll
-          for (int k = 0; k < IT; ++k) cnt[k * HJ_W + w] = kc[k];
+      for (int k = 0; k < IT; ++k) {
+        if constexpr (MULTI) {
+          if (lane == 0) cnt[k * HJ_W + w] = kl[k];
+        } else {
+          const uint64_t bal = ballot(kl[k] != 0);
+          inw[k] = mbcnt(bal);
+          if (lane == 0) cnt[k * HJ_W + w] = __popcll(bal);
         }
       }
       __syncthreads();
@@ -719,22 +884,27 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe_res(HjParams p, JoinResidual R
         if (lane == kWave - 1) last_tot = ex;
       }
       __syncthreads();
-      if constexpr (!MULTI) {
 #pragma unroll
-        for (int k = 0; k < IT; ++k) {
-          if (!kc[k]) continue;
+      for (int k = 0; k < (MULTI ? IT : 0); ++k) {
+        const int T = __builtin_amdgcn_readfirstlane(pick(m, k));
+        if (T == 0) continue;
+        stage(k);
+        if (EV == 2 && nar)
+          res_sweep<KM_ALL, true>(p, R, dec, c, T, wb, s_ex, s_hi, s_pr, s_p0, s_p1, rows,
+                                  pos + cnt[k * HJ_W + w]);
+        else
+          res_sweep<KM, true>(p, R, dec, c, T, wb, s_ex, s_hi, s_pr, s_p0, s_p1, rows,
+                              pos + cnt[k * HJ_W + w]);
+        wave_lds_sync();  // s_* are rewritten for the next slot
+      }
+#pragma unroll
+      for (int k = 0; k < IT; ++k) {
+        if constexpr (!MULTI) {
+          if (!kl[k]) continue;
           const int64_t o = pos + cnt[k * HJ_W + w] + inw[k];
           p.out_probe[o] = (int64_t)pr[k];
           p.out_build[o] = (int64_t)rows[hidx[k]];
           for (int q = 0; q < p.npay; ++q) p.pay_out[q][o] = p.pay_src[q][cb + hidx[k]];
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < IT; ++k) {
-          stage(k, inw[k]);
-          res_sweep<true>(p, R, c, m[k], wb, s_ex, s_hi, s_pr, s_p0, s_p1, rows,
-                          pos + cnt[k * HJ_W + w]);
-          wave_lds_sync();
         }
       }
       pos += last_tot;
@@ -768,6 +938,16 @@ __global__ void hj_gather_res(const void* __restrict__ src, int type,
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
     dst[i] = rf_widen(src, type, idx[i]);
+}
+
+// dst[i] = the comparison key (jr_key) of a build column at idx[i]: the
+// range form's build operand in partition order.
+__global__ void hj_gather_key(const void* __restrict__ src, int type,
+                              const int32_t* __restrict__ idx, int64_t n, int f64, int conv,
+                              int64_t* __restrict__ dst) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = jr_key(rf_widen(src, type, idx[i]), f64 != 0, conv != 0);
 }
 
 // dst[i] = src[idx[i]]: a build payload column in partition order.
@@ -950,13 +1130,41 @@ struct ProbeCarry {
   int64_t* tx[2] = {nullptr, nullptr};
 };
 
+// Multi-match build buckets of more than min_n rows sorted by (hash, row):
+// in HBM past the LDS sort's SORT_CAP rows, in LDS below.  *all_sorted: one
+// global sort ordered every bucket.
+int sort_buckets(const HjWs& w, int64_t nb, int bits, int64_t largest, int min_n,
+                 hipStream_t st, bool* all_sorted) {
+  const int P = 1 << bits;
+  *all_sorted = false;
+  if (largest > SORT_CAP && sort_big_buckets(w, nb, bits, st, all_sorted)) return 1;
+  if (nb > 0 && !*all_sorted && largest > min_n) {
+    {
+      KTimer _kt("hj_sort_buckets", st);
+      int cap = 2;  // the largest bucket sorted in LDS, to a power of two
+      while (cap < largest && cap < SORT_CAP) cap <<= 1;
+      const size_t lds = (size_t)cap * (sizeof(uint64_t) + sizeof(int32_t));
+      if (hip_check(hipFuncSetAttribute((const void*)hj_sort_buckets,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)(SORT_CAP * (sizeof(uint64_t) + sizeof(int32_t)))),
+                    "hash_join sort LDS"))
+        return 1;
+      hj_sort_buckets<<<(unsigned)P, SORT_NT, lds, st>>>(w.bh, w.brow, w.boff, cap, min_n);
+    }
+    if (launch_check("hash_join sort")) return 1;
+  }
+  return 0;
+}
+
 // Both sides partitioned as (mix64(key), row) by the top `bits` bits of the
 // hash (the probe side with its carried columns when pc has any), the work
-// items built and read back (one wait), and for multi-match builds every
-// bucket sorted by (hash, row).  *n_items and *largest from the read-back.
+// items built and read back (one wait), and with `sort`, for multi-match
+// builds, every bucket sorted by (hash, row).  *n_items and *largest from
+// the read-back.
 int prepare_sides(const int64_t* bkeys, int64_t nb, const int64_t* pkeys, int64_t np, int bits,
                   bool multi, const HjWs& w, const ProbeCarry& pc, int64_t* host_out,
-                  void* hdev, hipStream_t st, int64_t* n_items, int64_t* largest) {
+                  void* hdev, hipStream_t st, int64_t* n_items, int64_t* largest,
+                  bool sort = true) {
   const int P = 1 << bits;
   const size_t pws = part_ws_bytes(nb, np, bits);
   if (partition_side(bkeys, nb, bits, w.bh, w.brow, w.boff, w.th, w.trow, w.part_ws, pws, st, 1))
@@ -977,23 +1185,8 @@ int prepare_sides(const int64_t* bkeys, int64_t nb, const int64_t* pkeys, int64_
   *n_items = host_out[0];
   *largest = host_out[1];
   if (*n_items > max_items(np, bits, multi)) return fail("hash_join: work items overflow");
-  bool all_sorted = false;  // every bucket sorted in HBM already
-  if (multi && *largest > SORT_CAP && sort_big_buckets(w, nb, bits, st, &all_sorted)) return 1;
-  if (multi && nb > 0 && !all_sorted) {
-    {
-      KTimer _kt("hj_sort_buckets", st);
-      int cap = 2;  // the largest bucket sorted in LDS, to a power of two
-      while (cap < *largest && cap < SORT_CAP) cap <<= 1;
-      const size_t lds = (size_t)cap * (sizeof(uint64_t) + sizeof(int32_t));
-      if (hip_check(hipFuncSetAttribute((const void*)hj_sort_buckets,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)(SORT_CAP * (sizeof(uint64_t) + sizeof(int32_t)))),
-                    "hash_join sort LDS"))
-        return 1;
-      hj_sort_buckets<<<(unsigned)P, SORT_NT, lds, st>>>(w.bh, w.brow, w.boff, cap);
-    }
-    if (launch_check("hash_join sort")) return 1;
-  }
+  bool all_sorted;
+  if (multi && sort && sort_buckets(w, nb, bits, *largest, 0, st, &all_sorted)) return 1;
   return 0;
 }
 
@@ -1027,8 +1220,8 @@ HrWs carve_hr(void* ws, size_t base, int64_t nb, int64_t np, int bits, int nv, i
 // column is staged in LDS and the second read in partition order, and any
 // other is read at the pair's row.  Every column must cover its side's
 // rows.
-int residual_plan(const RfProgram& P, int n_cols, int64_t nb, int64_t np, JoinResidual* R,
-                  int* nv, int* nbv, int pslot[2], int bslot[2]) {
+int residual_plan(const RfProgram& P, int n_cols, int64_t nb, int64_t np, bool multi,
+                  JoinResidual* R, int* nv, int* nbv, int pslot[2], int bslot[2]) {
   R->P = P;
   R->bpart1 = nullptr;
   for (int s = 0; s < PF_RES_MAX_COLS; ++s) R->src[s] = RS_ROW;
@@ -1057,7 +1250,20 @@ int residual_plan(const RfProgram& P, int n_cols, int64_t nb, int64_t np, JoinRe
     }
   }
   (void)n_cols;
+  R->fast = jr_fast_of(*R);
+  if (!multi) R->fast.range = 0;  // a unique build's runs are one row
   return 0;
+}
+
+template <bool MULTI, bool W1>
+void launch_res(int64_t n_items, const HjParams& p, const JoinResidual& R, const HjRes& x,
+                hipStream_t st) {
+  if (MULTI && R.fast.range)
+    hj_probe_res<MULTI, W1, MULTI ? 2 : 1><<<(unsigned)n_items, HJ_NT, 0, st>>>(p, R, x);
+  else if (R.fast.on)
+    hj_probe_res<MULTI, W1, 1><<<(unsigned)n_items, HJ_NT, 0, st>>>(p, R, x);
+  else
+    hj_probe_res<MULTI, W1, 0><<<(unsigned)n_items, HJ_NT, 0, st>>>(p, R, x);
 }
 
 HjParams params_of(const HjWs& w, int bits, bool multi, int64_t n_items) {
@@ -1221,7 +1427,7 @@ int pf_hash_join_residual_prepare(const int64_t* bkeys, int64_t nb, const int64_
   if (rf_program_of(cols, col_types, col_sides, col_rows, n_cols, ins, n_ins, &P)) return 1;
   JoinResidual R;
   int nv, nbv, pslot[2], bslot[2];
-  if (residual_plan(P, n_cols, nb, np, &R, &nv, &nbv, pslot, bslot)) return 1;
+  if (residual_plan(P, n_cols, nb, np, multi != 0, &R, &nv, &nbv, pslot, bslot)) return 1;
   HjWs w = carve_hj(workspace, nb, np, bits, multi != 0);
   HrWs r = carve_hr(workspace, w.bytes, nb, np, bits, nv, nbv);
   if (workspace_bytes < r.bytes) return fail("hash_join: workspace too small");
@@ -1236,14 +1442,39 @@ int pf_hash_join_residual_prepare(const int64_t* bkeys, int64_t nb, const int64_
     pc.tx[c] = r.tx[c];
   }
   int64_t n_items = 0, largest = 0;
+  const bool range = R.fast.range != 0;
   if (prepare_sides(bkeys, nb, pkeys, np, bits, multi != 0, w, pc, host_out, hdev, st, &n_items,
-                    &largest))
+                    &largest, !range))
     return 1;
-  // the build operands in the (sorted) partition order
-  for (int c = 0; c < nbv; ++c) {
-    hj_gather_res<<<grid_for(nb, 256) < 16384 ? grid_for(nb, 256) : 16384, 256, 0, st>>>(
-        P.col[bslot[c]], P.type[bslot[c]], w.brow, nb, r.bx[c]);
-    if (launch_check("hash_join residual operands")) return 1;
+  const unsigned gg = grid_for(nb, 256) < 16384 ? grid_for(nb, 256) : 16384;
+  if (range) {
+    // the range form: buckets past the keyed sort by (hash, row), the build
+    // operand's keys gathered, then the smaller buckets by (hash, key, row)
+    bool all_sorted;
+    if (sort_buckets(w, nb, bits, largest, KEYED_CAP, st, &all_sorted)) return 1;
+    hj_gather_key<<<gg, 256, 0, st>>>(P.col[bslot[0]], P.type[bslot[0]], w.brow, nb,
+                                      R.fast.f64, R.fast.bv_conv, r.bx[0]);
+    if (launch_check("hash_join residual keys")) return 1;
+    if (nb > 1) {
+      KTimer _kt("hj_sort_keyed", st);
+      int cap = 2;
+      while (cap < largest && cap < KEYED_CAP) cap <<= 1;
+      const size_t per = sizeof(uint64_t) + sizeof(int64_t) + sizeof(int32_t);
+      if (hip_check(hipFuncSetAttribute((const void*)hj_sort_keyed,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)(KEYED_CAP * per)),
+                    "hash_join keyed sort LDS"))
+        return 1;
+      hj_sort_keyed<<<(unsigned)(1 << bits), SORT_NT, cap * per, st>>>(w.bh, r.bx[0], w.brow,
+                                                                      w.boff, cap);
+    }
+    if (launch_check("hash_join keyed sort")) return 1;
+  } else {
+    // the build operands in the (sorted) partition order
+    for (int c = 0; c < nbv; ++c) {
+      hj_gather_res<<<gg, 256, 0, st>>>(P.col[bslot[c]], P.type[bslot[c]], w.brow, nb, r.bx[c]);
+      if (launch_check("hash_join residual operands")) return 1;
+    }
   }
   if (fill_i64(r.dev, 2, 0, st)) return 1;
   HjParams p = params_of(w, bits, multi != 0, n_items);
@@ -1254,9 +1485,9 @@ int pf_hash_join_residual_prepare(const int64_t* bkeys, int64_t nb, const int64_
     {
       KTimer _kt("hj_count_res", st);
       if (multi)
-        hj_probe_res<true, false><<<(unsigned)n_items, HJ_NT, 0, st>>>(p, R, x);
+        launch_res<true, false>(n_items, p, R, x, st);
       else
-        hj_probe_res<false, false><<<(unsigned)n_items, HJ_NT, 0, st>>>(p, R, x);
+        launch_res<false, false>(n_items, p, R, x, st);
     }
     if (launch_check("hash_join residual count")) return 1;
   }
@@ -1288,7 +1519,7 @@ int pf_hash_join_residual_write(int64_t nb, int64_t np, int32_t bits, int32_t mu
   if (rf_program_of(cols, col_types, col_sides, col_rows, n_cols, ins, n_ins, &P)) return 1;
   JoinResidual R;
   int nv, nbv, pslot[2], bslot[2];
-  if (residual_plan(P, n_cols, nb, np, &R, &nv, &nbv, pslot, bslot)) return 1;
+  if (residual_plan(P, n_cols, nb, np, multi != 0, &R, &nv, &nbv, pslot, bslot)) return 1;
   HjWs w = carve_hj(workspace, nb, np, bits, multi != 0);
   HrWs r = carve_hr(workspace, w.bytes, nb, np, bits, nv, nbv);
   if (workspace_bytes < r.bytes) return fail("hash_join: workspace too small");
@@ -1314,9 +1545,9 @@ int pf_hash_join_residual_write(int64_t nb, int64_t np, int32_t bits, int32_t mu
   {
     KTimer _kt("hj_write_res", st);
     if (multi)
-      hj_probe_res<true, true><<<(unsigned)n_items, HJ_NT, 0, st>>>(p, R, x);
+      launch_res<true, true>(n_items, p, R, x, st);
     else
-      hj_probe_res<false, true><<<(unsigned)n_items, HJ_NT, 0, st>>>(p, R, x);
+      launch_res<false, true>(n_items, p, R, x, st);
   }
   if (launch_check("hash_join residual write")) return 1;
   return ok();

@@ -5,11 +5,17 @@ On the pair filter's random condition trees (every column dtype, every
 literal kind) the kept pairs equal the oracle's residual filter applied to
 the oracle's join pairs, as multisets, for unique and multi-match builds,
 one-pass and two-level partitions; in order they equal the unfused path
-(the join's pairs, then ``residual.filter_pairs``); ``candidates`` is the
-unfused pair count.  Programs naming more columns than the join carries
-(three a side: the rest read at their rows), payloads gathered for kept
-pairs, build buckets past the LDS chunk and probe buckets of several work
-items, and a column shorter than its side (raises) are covered."""
+(the join's pairs, then ``residual.filter_pairs``) except in the range form
+(one order or equality between the build column and a row operand, on a
+multi-match build), which orders a probe row's build rows by the build
+operand; ``candidates`` is the unfused pair count.  The range form is
+checked for every relation, both operand orders, int and float columns
+(NaN, -0.0, infinities, integer against float), literals, and buckets
+past the keyed sort and the LDS sort.  Programs naming more columns than
+the join carries (three a side: the rest read at their rows), payloads
+gathered for kept pairs, build buckets past the LDS chunk and probe
+buckets of several work items, and a column shorter than its side
+(raises) are covered."""
 from __future__ import annotations
 
 import numpy as np
@@ -76,9 +82,14 @@ def test_fused_matches_oracle_on_random_conditions(unique, bits):
     assert ran >= 10
 
 
+def _pair_keys(b, p, n_probe):
+    return torch.sort(b * n_probe + p).values
+
+
 @pytest.mark.parametrize("unique", [True, False])
 @pytest.mark.parametrize("bits", [2, 10])
-def test_fused_equals_the_filtered_join_in_order(unique, bits):
+@pytest.mark.parametrize("op", ['<', '<>'])
+def test_fused_equals_the_filtered_join_in_order(unique, bits, op):
     rng = np.random.default_rng(5 + bits + unique)
     nb, np_ = 20_000, 150_000
     bk = torch.from_numpy(_keys(rng, nb, 5000, unique)).cuda()
@@ -86,12 +97,69 @@ def test_fused_equals_the_filtered_join_in_order(unique, bits):
     pk[::9] = -5
     bv = torch.from_numpy(rng.integers(0, 100, nb)).cuda()
     pv = torch.from_numpy(rng.integers(0, 100, np_).astype(np.int32)).cuda()
-    prog = residual.compile_conditions({1: bv}, {2: pv}, [['#1', '<', '#2']])
-    op, ob, _ = hash_join(bk, pk, unique=unique, bits=bits)
-    kb, kp = residual.filter_pairs(prog, ob, op)
+    prog = residual.compile_conditions({1: bv}, {2: pv}, [['#1', op, '#2']])
+    op_, ob, _ = hash_join(bk, pk, unique=unique, bits=bits)
+    kb, kp = residual.filter_pairs(prog, ob, op_)
     plan, fb, fp, _ = _fused(bk, pk, prog, unique, bits)
     assert plan.candidates == ob.numel()
-    assert torch.equal(fb, kb) and torch.equal(fp, kp)
+    if unique or op == '<>':
+        assert torch.equal(fb, kb) and torch.equal(fp, kp)
+    else:  # the range form: a probe row's build rows by the build operand
+        assert torch.equal(_pair_keys(fb, fp, np_), _pair_keys(kb, kp, np_))
+
+
+RANGE_OPS = ['=', '<', '<=', '>', '>=', '<>']
+
+
+def _range_columns(rng, nb, np_, kind):
+    if kind == "int":
+        return rng.integers(-5, 6, nb), rng.integers(-5, 6, np_).astype(np.int32)
+    if kind == "float":
+        b = rng.integers(-3, 4, nb).astype(np.float64) * 0.5
+        p = rng.integers(-3, 4, np_).astype(np.float64) * 0.5
+        for x, n in ((b, nb), (p, np_)):
+            x[rng.random(n) < 0.03] = np.nan
+            x[rng.random(n) < 0.03] = -0.0
+            x[rng.random(n) < 0.02] = np.inf
+            x[rng.random(n) < 0.02] = -np.inf
+        return b, p
+    # an integer build column against a float probe column
+    return rng.integers(-3, 4, nb), rng.integers(-6, 7, np_).astype(np.float64) * 0.5
+
+
+@pytest.mark.parametrize("kind", ["int", "float", "int_vs_float"])
+@pytest.mark.parametrize("build_left", [True, False])
+def test_range_form_every_relation(kind, build_left):
+    rng = np.random.default_rng(31 + len(kind) + build_left)
+    nb, np_ = 8000, 60_000
+    bk = _keys(rng, nb, 1500, False)
+    pk = np.where(rng.random(np_) < 0.85, bk[rng.integers(0, nb, np_)], -77)
+    bv, pv = _range_columns(rng, nb, np_, kind)
+    left, right = {1: bv}, {2: pv}
+    dbk, dpk = torch.from_numpy(bk).cuda(), torch.from_numpy(pk).cuda()
+    dl, dr = _dev(left), _dev(right)
+    lm, rm = oracle.sort_search_pairs(bk, pk)
+    for op in RANGE_OPS:
+        if kind != "int" and op == '=':
+            continue  # a float '=' is isclose, which the range form does not take
+        cond = [['#1', op, '#2'] if build_left else ['#2', op, '#1']]
+        try:
+            prog = residual.compile_conditions(dl, dr, cond)
+        except NotImplementedError:
+            continue
+        wl, wr = oracle.residual_filter(left, right, cond, lm, rm)
+        plan, ob, op_, _ = _fused(dbk, dpk, prog, False, None)
+        assert plan.candidates == lm.shape[0]
+        assert _sorted_pairs(ob.cpu(), op_.cpu()) == _sorted_pairs(wl, wr), (kind, cond)
+    for lit in ['2', '-1.5']:
+        cond = [['#1', '<=', lit]]
+        try:
+            prog = residual.compile_conditions(dl, dr, cond)
+        except NotImplementedError:
+            continue
+        wl, wr = oracle.residual_filter(left, right, cond, lm, rm)
+        _, ob, op_, _ = _fused(dbk, dpk, prog, False, None)
+        assert _sorted_pairs(ob.cpu(), op_.cpu()) == _sorted_pairs(wl, wr), (kind, cond)
 
 
 @pytest.mark.parametrize("unique", [True, False])
@@ -136,11 +204,12 @@ def test_large_build_buckets_and_many_items():
     prog = residual.compile_conditions({1: dbv}, {2: dpv}, [['#1', '>', '#2']])
     op, ob, _ = hash_join(dbk, dpk, unique=False, bits=2)
     kb, kp = residual.filter_pairs(prog, ob, op)
-    plan, fb, fp, _ = _fused(dbk, dpk, prog, False, 2)
-    assert plan.candidates == ob.numel() and plan.n_items > 4
-    assert torch.equal(fb, kb) and torch.equal(fp, kp)
-    keep = bv[fb.cpu().numpy()] > pv[fp.cpu().numpy()]
-    assert keep.all() and bool((dbk[fb] == dpk[fp]).all())
+    for bits in (2, 4):  # 4: buckets of both sizes, narrowed and compared per candidate
+        plan, fb, fp, _ = _fused(dbk, dpk, prog, False, bits)
+        assert plan.candidates == ob.numel() and plan.n_items > 4
+        assert torch.equal(_pair_keys(fb, fp, np_), _pair_keys(kb, kp, np_))
+        keep = bv[fb.cpu().numpy()] > pv[fp.cpu().numpy()]
+        assert keep.all() and bool((dbk[fb] == dpk[fp]).all())
 
 
 def test_empty_sides_and_nothing_kept():
