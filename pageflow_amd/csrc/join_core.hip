@@ -331,9 +331,14 @@ __global__ __launch_bounds__(CT_NT) void k_set_probe(
   for (int j = 0; j < CT_IT; ++j) {
     const int64_t i = base + j * CT_NT + threadIdx.x;
     if (i >= np) break;
-    const bool hit = set_hit(pk[i], skey, mn, mx, cap, op, op ? pv[i] : 0);
+    // the probe stream is read and the mask written past L2 (nontemporal),
+    // which keeps L2 for the set's lines: a skewed probe's hot keys then
+    // stay resident
+    const int64_t k = __builtin_nontemporal_load(&pk[i]);
+    const int64_t v = op ? __builtin_nontemporal_load(&pv[i]) : 0;
+    const bool hit = set_hit(k, skey, mn, mx, cap, op, v);
     const bool keep = hit != (anti != 0);
-    keepv[i] = keep;
+    __builtin_nontemporal_store((uint8_t)keep, &keepv[i]);
     c += keep;
   }
   if (tilecnt) {
