@@ -573,7 +573,14 @@ struct HjRes {
   const int64_t* pv[2];  // carried probe values in partition order (nullptr: none)
   const int64_t* bv0;    // the LDS-staged build column in partition order (nullptr: none)
   int64_t* cand;         // count pass: candidate pairs before the residual (one counter)
+  int32_t* kt;           // multi: kept totals a (round, slot, wave), KT_ITEM an item
 };
+
+// A multi-match item's kept totals: at most PCH_M / (HJ_NT * 4) rounds of
+// 4 slots x HJ_W waves.  The count pass stores them for items of one build
+// chunk, and the write pass reads them instead of sweeping the candidates
+// twice.
+constexpr int KT_ITEM = PCH_M / (HJ_NT * 4) * 4 * HJ_W;
 
 // The program on one pair: its fast form (decoded on the host; c.pv0 and
 // c.pv1 then hold the row's resolved operands, jr_fast_row), or the
@@ -724,6 +731,10 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe_res(HjParams p, JoinResidual R
   // range form: a bucket the keyed sort ordered by (hash, key) narrows each
   // run to its kept sub-run; larger buckets compare keys per candidate
   const bool nar = EV == 2 && it.be - it.bs <= KEYED_CAP;
+  // kept totals kept between the passes: multi-match items of one build
+  // chunk that sweep their candidates
+  const bool kt_use = MULTI && !nar && x.kt != nullptr && it.be - it.bs <= CAP;
+  int32_t* const kt_item = kt_use ? x.kt + item * KT_ITEM : nullptr;
   int64_t pos = W1 ? p.item_pos[item] : 0;
   int64_t kept_n = 0, cand_n = 0;
   const bool has_p0 = x.pv[0] != nullptr, has_p1 = x.pv[1] != nullptr;
@@ -834,13 +845,18 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe_res(HjParams p, JoinResidual R
           const int T = __builtin_amdgcn_readfirstlane(pick(m, k));
           if (T > 0 && nar) {
             kk = T;  // a narrowed run keeps every candidate
+          } else if (W1 && kt_use) {
+            kk = kt_item[(i0 / ROUND) * (IT * HJ_W) + k * HJ_W + w];  // the count pass's
           } else if (T > 0) {
             stage(k);
             kk = res_sweep<KM, false>(p, R, dec, c, T, wb, s_ex, s_hi, s_pr, s_p0, s_p1, rows,
                                       0);
             wave_lds_sync();  // s_* are rewritten for the next slot
           }
-          if (!W1 && lane == 0) kept_n += kk;
+          if (!W1 && lane == 0) {
+            kept_n += kk;
+            if (kt_use) kt_item[(i0 / ROUND) * (IT * HJ_W) + k * HJ_W + w] = kk;
+          }
         } else {
           if (pick(m, k) > 0) {
             c.li = pick(hidx, k);
@@ -1199,17 +1215,20 @@ struct HrWs {
   int64_t* px[2] = {nullptr, nullptr};
   int64_t* tx[2] = {nullptr, nullptr};
   int64_t* bx[2] = {nullptr, nullptr};
+  int32_t* kt = nullptr;
   int64_t* dev = nullptr;
   size_t bytes = 0;
 };
 
-HrWs carve_hr(void* ws, size_t base, int64_t nb, int64_t np, int bits, int nv, int nbv) {
+HrWs carve_hr(void* ws, size_t base, int64_t nb, int64_t np, int bits, bool multi, int nv,
+              int nbv) {
   Carve cv(ws ? (char*)ws + base : nullptr);
   HrWs r;
   for (int c = 0; c < nv; ++c) r.px[c] = cv.take<int64_t>(np > 0 ? np : 1);
   if (bits > kOnePassBits)
     for (int c = 0; c < nv; ++c) r.tx[c] = cv.take<int64_t>(np > 0 ? np : 1);
   for (int c = 0; c < nbv; ++c) r.bx[c] = cv.take<int64_t>(nb > 0 ? nb : 1);
+  if (multi) r.kt = cv.take<int32_t>(max_items(np, bits, true) * KT_ITEM);
   r.dev = cv.take<int64_t>(2);
   r.bytes = base + cv.bytes();
   return r;
@@ -1407,7 +1426,7 @@ size_t pf_hash_join_residual_workspace_bytes(int64_t nb, int64_t np, int32_t bit
   const int nv = probe_cols < 0 ? 0 : probe_cols > 2 ? 2 : probe_cols;
   const int nbv = build_cols < 0 ? 0 : build_cols > 2 ? 2 : build_cols;
   const size_t base = pf::carve_hj(nullptr, nb, np, bits, multi == 1).bytes;
-  return pf::carve_hr(nullptr, base, nb, np, bits, nv, nbv).bytes;
+  return pf::carve_hr(nullptr, base, nb, np, bits, multi == 1, nv, nbv).bytes;
 }
 
 int pf_hash_join_residual_prepare(const int64_t* bkeys, int64_t nb, const int64_t* pkeys,
@@ -1429,7 +1448,7 @@ int pf_hash_join_residual_prepare(const int64_t* bkeys, int64_t nb, const int64_
   int nv, nbv, pslot[2], bslot[2];
   if (residual_plan(P, n_cols, nb, np, multi != 0, &R, &nv, &nbv, pslot, bslot)) return 1;
   HjWs w = carve_hj(workspace, nb, np, bits, multi != 0);
-  HrWs r = carve_hr(workspace, w.bytes, nb, np, bits, nv, nbv);
+  HrWs r = carve_hr(workspace, w.bytes, nb, np, bits, multi != 0, nv, nbv);
   if (workspace_bytes < r.bytes) return fail("hash_join: workspace too small");
   void* hdev = nullptr;
   if (host_alias(host_out, &hdev)) return 1;
@@ -1480,7 +1499,7 @@ int pf_hash_join_residual_prepare(const int64_t* bkeys, int64_t nb, const int64_
   HjParams p = params_of(w, bits, multi != 0, n_items);
   R.bpart1 = nbv > 1 ? r.bx[1] : nullptr;
   HjRes x{{nv > 0 ? r.px[0] : nullptr, nv > 1 ? r.px[1] : nullptr},
-          nbv > 0 ? r.bx[0] : nullptr, r.dev};
+          nbv > 0 ? r.bx[0] : nullptr, r.dev, r.kt};
   if (n_items > 0) {
     {
       KTimer _kt("hj_count_res", st);
@@ -1521,7 +1540,7 @@ int pf_hash_join_residual_write(int64_t nb, int64_t np, int32_t bits, int32_t mu
   int nv, nbv, pslot[2], bslot[2];
   if (residual_plan(P, n_cols, nb, np, multi != 0, &R, &nv, &nbv, pslot, bslot)) return 1;
   HjWs w = carve_hj(workspace, nb, np, bits, multi != 0);
-  HrWs r = carve_hr(workspace, w.bytes, nb, np, bits, nv, nbv);
+  HrWs r = carve_hr(workspace, w.bytes, nb, np, bits, multi != 0, nv, nbv);
   if (workspace_bytes < r.bytes) return fail("hash_join: workspace too small");
   if (n_items <= 0) return ok();
   if (n_items > max_items(np, bits, multi != 0)) return fail("hash_join: work items overflow");
@@ -1541,7 +1560,7 @@ int pf_hash_join_residual_write(int64_t nb, int64_t np, int32_t bits, int32_t mu
   }
   R.bpart1 = nbv > 1 ? r.bx[1] : nullptr;
   HjRes x{{nv > 0 ? r.px[0] : nullptr, nv > 1 ? r.px[1] : nullptr},
-          nbv > 0 ? r.bx[0] : nullptr, r.dev};
+          nbv > 0 ? r.bx[0] : nullptr, r.dev, r.kt};
   {
     KTimer _kt("hj_write_res", st);
     if (multi)

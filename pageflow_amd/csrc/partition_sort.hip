@@ -50,7 +50,7 @@ constexpr int MS_NT = 256;         // select-first prologue: 16 mask bytes a thr
 constexpr int MS_ROWS = MS_NT * 16;
 constexpr int SC_NT = 1024;        // one-block count scan
 
-struct Cols {
+struct ColumnSet {
   const uint8_t* in[MAXC];
   uint8_t* out[MAXC];
   int rb[MAXC];
@@ -59,8 +59,8 @@ struct Cols {
   int ncols;
 };
 
-struct Params {
-  Cols c;
+struct PsArgs {
+  ColumnSet c;
   int64_t* gout;          // null: no gidx output
   const int32_t* bucket;  // the rows' buckets given (pf_partition_by_bucket), or null
   int gcls;
@@ -85,10 +85,11 @@ __device__ __forceinline__ int bucket_of(int64_t key, int logB) {
   return logB ? (int)(mix64((uint64_t)key) >> (64 - logB)) : 0;
 }
 
-// Where domain row i of a column lives: compacted columns at the compact
-// index, raw ones at the pre-mask row (select-first: the kept row's id).
+// The row of a column that holds domain row i: a compacted column's is its
+// position among the kept rows, a full column's the row before the mask
+// (after select-first, the id the prologue recorded for kept row i).
 template <bool SRC>
-__device__ __forceinline__ int64_t col_index(const Params& p, bool comp, int64_t i, int jl,
+__device__ __forceinline__ int64_t domain_row(const PsArgs& p, bool comp, int64_t i, int jl,
                                              int64_t coff) {
   if (SRC) return comp ? i : (int64_t)p.src[i];
   return comp ? coff + jl : i;
@@ -110,7 +111,7 @@ __device__ __forceinline__ uint64_t match_bucket(int b, int logB) {
 }
 
 template <bool SRC>
-__device__ __forceinline__ void load_valid(const Params& p, int64_t seg, bool (&valid)[IT]) {
+__device__ __forceinline__ void load_valid(const PsArgs& p, int64_t seg, bool (&valid)[IT]) {
   const int lane = lane_id();
 #pragma unroll
   for (int k = 0; k < IT; ++k) {
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(MS_NT) void ps_mask_select(const uint8_t* __restric
 // ---------------------------------------------------------------- K1 --
 
 template <bool SRC>
-__global__ __launch_bounds__(NT) void ps_hist(Params p) {
+__global__ __launch_bounds__(NT) void ps_hist(PsArgs p) {
   __shared__ int32_t h[BMAX];
   __shared__ int32_t wv[NW];
   const int tid = threadIdx.x, w = tid / kWave;
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(NT) void ps_hist(Params p) {
           continue;
         }
       } else {
-        b = bucket_of(p.key[col_index<SRC>(p, p.key_comp, i, jl[k], coff)], p.logB);
+        b = bucket_of(p.key[domain_row<SRC>(p, p.key_comp, i, jl[k], coff)], p.logB);
       }
       atomicAdd(&h[b], 1);
     }
@@ -368,18 +369,18 @@ __global__ __launch_bounds__(NT) void ps_finish(const int64_t* __restrict__ coun
 // ---------------------------------------------------------------- K3 --
 
 template <int ES>
-struct VT;
+struct WordOf;
 template <>
-struct VT<1> { typedef uint8_t T; };
+struct WordOf<1> { typedef uint8_t T; };
 template <>
-struct VT<2> { typedef uint16_t T; };
+struct WordOf<2> { typedef uint16_t T; };
 template <>
-struct VT<4> { typedef uint32_t T; };
+struct WordOf<4> { typedef uint32_t T; };
 template <>
-struct VT<8> { typedef uint64_t T; };
+struct WordOf<8> { typedef uint64_t T; };
 
 template <bool SRC>
-__global__ __launch_bounds__(NT) void ps_scatter(Params p) {
+__global__ __launch_bounds__(NT) void ps_scatter(PsArgs p) {
   if (((p.check & 1) && p.flags[1]) || ((p.check & 2) && p.flags[2])) return;
   __shared__ uint64_t stage[TILE];
   __shared__ int64_t gb[MAXCLS * BMAX];
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(NT) void ps_scatter(Params p) {
       const int b = p.bucket[i];
       bk[k] = (unsigned)b < (unsigned)B ? b : -1;
     } else {
-      bk[k] = bucket_of(p.key[col_index<SRC>(p, p.key_comp, i, jl[k], coff)], p.logB);
+      bk[k] = bucket_of(p.key[domain_row<SRC>(p, p.key_comp, i, jl[k], coff)], p.logB);
     }
   }
   // stable rank inside (wave, bucket): rounds in row order, lanes in order
@@ -494,10 +495,10 @@ __global__ __launch_bounds__(NT) void ps_scatter(Params p) {
     const int64_t* gbc = gb + p.c.cls[c] * B;
 #define PF_STAGE(ES)                                                                   \
   {                                                                                    \
-    typedef VT<ES>::T T;                                                               \
+    typedef WordOf<ES>::T T;                                                               \
     _Pragma("unroll") for (int k = 0; k < IT; ++k) {                                   \
       if (pos[k] >= 0) {                                                               \
-        const int64_t i = col_index<SRC>(p, comp, seg + k * kWave + lane, jl[k], coff); \
+        const int64_t i = domain_row<SRC>(p, comp, seg + k * kWave + lane, jl[k], coff); \
         ((T*)stage)[pos[k]] = ((const T*)in)[i];                                       \
       }                                                                                \
     }                                                                                  \
@@ -520,7 +521,7 @@ __global__ __launch_bounds__(NT) void ps_scatter(Params p) {
       for (int64_t e = tid; e < total; e += NT) {
         const int s = (int)(e / per), q = (int)(e - (int64_t)s * per);
         const int64_t dom = tile_base + sidx[s];
-        const int64_t i = col_index<SRC>(p, comp, dom, sjl[s], coff);
+        const int64_t i = domain_row<SRC>(p, comp, dom, sjl[s], coff);
         const int64_t o = gbc[sb[s]] + s;
         if (words)
           ((uint32_t*)out)[o * per + q] = ((const uint32_t*)in)[i * per + q];
@@ -610,7 +611,7 @@ int partition_impl(const void* const* cols_in, void* const* cols_out, const int3
     return fail("partition_sort: n_valid " + std::to_string(n_valid) + " outside [0, " +
                 std::to_string(ndom) + "]");
   const bool has_mask = mask != nullptr;
-  Params p{};
+  PsArgs p{};
   int rb_cls[MAXCLS];
   int ncls = 0;
   bool bad_cls = false;
