@@ -11,7 +11,8 @@ requests of the batch and the serving stream).
   C2   16M unique int64 build (+ payload) x 640M probe, 10% absent
   C2b  the standalone radix partition of C2's 640M keys at 8 and 12 bits
   C3   10M build over 2.5M keys (geometric duplicates) x 400M Zipf(1.1)
-       probe, 20% absent: inner pairs, semi and anti masks
+       probe, 20% absent: inner pairs, semi and anti masks, and a residual
+       condition evaluated inside the join
   C4   256 uneven joins (1K-4M unique int32 build, 8x probe, 15% absent)
   C5   the 2048-request serving stream over 8 resident dimension tables
 """
@@ -168,6 +169,61 @@ def test_c3_skewed_join_and_masks_at_full_size():
     del semi
     anti = semi_anti_mask(build, probe, anti=True)
     assert bool((anti == ~present).all())
+
+
+@pytest.mark.parametrize("op", ['<', '<>'])
+def test_c3_residual_inside_the_join_at_full_size(op):
+    """C3 with the residual b_pay OP p_pay (int64 payloads in [0, 1000))
+    evaluated inside the join ('<' takes the range form, '<>' the fast form
+    with window sweeps): the same pairs as the join followed by the filter
+    (count and sorted (build, probe) keys at full size), every kept pair's
+    keys equal and condition true, and the oracle's filtered pairs on the
+    first 2M probe rows."""
+    from pageflow_amd.operators import residual
+    dev = torch.device("cuda")
+    nb, nd, np_ = 10_000_000, 2_500_000, 400_000_000
+    say(f"C3 residual {op}: generate")
+    build, probe, want = make_skewed_join(nb, nd, np_, 3, dev)
+    g = torch.Generator(device=dev).manual_seed(11)
+    bpay = torch.randint(0, 1000, (nb,), generator=g, device=dev)
+    ppay = torch.randint(0, 1000, (np_,), generator=g, device=dev)
+    cond = [['b_pay#11', op, 'p_pay#12']]
+    prog = residual.compile_conditions({11: bpay}, {12: ppay}, cond)
+    plan = HashJoin(nb, np_, unique=False, device=dev)
+    kept = plan.prepare(build, probe, residual=prog)
+    assert plan.candidates == want["pairs"]
+    fp = torch.empty(kept, dtype=torch.int64, device=dev)
+    fb = torch.empty_like(fp)
+    plan.write(fp, fb)
+    del plan
+    say(f"C3 residual {op}: the join, then the filter")
+    plan = HashJoin(nb, np_, unique=False, device=dev)
+    n = plan.prepare(build, probe)
+    op_ = torch.empty(n, dtype=torch.int64, device=dev)
+    ob = torch.empty_like(op_)
+    plan.write(op_, ob)
+    del plan
+    kb, kp = residual.filter_pairs(prog, ob, op_)
+    del op_, ob
+    assert kb.numel() == kept
+    assert torch.equal(torch.sort(fb * np_ + fp).values, torch.sort(kb * np_ + kp).values)
+    del kb, kp
+    say(f"C3 residual {op}: properties of {kept} pairs")
+    cmp = (lambda a, b: a < b) if op == '<' else (lambda a, b: a != b)
+    assert _all_equal_in_chunks(lambda a, b: build[fb[a:b]] == probe[fp[a:b]], kept)
+    assert _all_equal_in_chunks(lambda a, b: cmp(bpay[fb[a:b]], ppay[fp[a:b]]), kept)
+    m = 2_000_000
+    sel = fp < m
+    got = torch.stack([fb[sel], fp[sel]], 1).cpu().numpy()
+    del fb, fp
+    say(f"C3 residual {op}: oracle slice")
+    bk, pk = build.cpu().numpy(), probe[:m].cpu().numpy()
+    lm, rm = oracle.sort_search_pairs(bk, pk)
+    wl, wr = oracle.residual_filter({11: bpay.cpu().numpy()}, {12: ppay[:m].cpu().numpy()},
+                                    cond, lm, rm)
+    want_pairs = np.stack([wl, wr], 1)
+    want_pairs = want_pairs[np.lexsort((want_pairs[:, 1], want_pairs[:, 0]))]
+    assert np.array_equal(got[np.lexsort((got[:, 1], got[:, 0]))], want_pairs)
 
 
 def test_c4_batch_of_256_joins_at_full_size():
