@@ -163,7 +163,7 @@ def partition_sweep(args, dev, probe):
     vout = torch.empty_like(rows)
     sweep = {}
     cases = [(int(b), int(sh)) for b in args.part_bits.split(",")
-             for sh in args.part_shapes.split(",") if int(sh) < 0 or int(b) <= 8]
+             for sh in args.part_shapes.split(",") if int(sh) != 4 or int(b) > 8]
     for bits, shape in cases:
         offs = torch.empty((1 << bits) + 1, dtype=torch.int64, device=dev)
         ws = _lib.workspace(radix.workspace_bytes(np_, bits), dev)
