@@ -40,6 +40,11 @@ Parity pinning (what checks each function besides the formula itself):
   the golden-pinned ``bucket_of``.
 * ``partition_outputs``: each bucket's rows in source order (brute force per
   bucket) at the starts ``padded_layout`` gives, padding unused.
+* ``fmix32_np`` (the batch tier's table slots; placement only, no result
+  depends on it): Murmur3's published 32-bit finalizer, pinned as a
+  bijection by its algebraic inverse (the multipliers' inverses mod 2^32
+  computed by Python), by fmix32(0) = 0, and against the same steps in
+  Python integers.
 * ``zero_copy_rearrange`` / ``zero_copy_mask``: the reference test's closed
   forms (``tests/IO/test_scan_regions.py:187-218``: ``x[lo:hi][idx]`` with
   null rows -64, ``x[lo:hi][mask]``) over the same column cut into
@@ -111,6 +116,24 @@ def mix64_np(keys) -> np.ndarray:
         k = k * _M2
         k = k ^ (k >> _S33)
     return k
+
+
+_F1 = np.uint32(0x85ebca6b)
+_F2 = np.uint32(0xc2b2ae35)
+
+
+def fmix32_np(keys) -> np.ndarray:
+    """Murmur3's 32-bit finalizer of each key's 32 bits, as uint32 (the
+    batch tier's table slot hash; the reference hashes with mix64 only, and
+    no join result depends on the slot)."""
+    h = np.ascontiguousarray(np.asarray(keys)).astype(np.int64).astype(np.uint32)
+    with np.errstate(over="ignore"):
+        h = h ^ (h >> np.uint32(16))
+        h = h * _F1
+        h = h ^ (h >> np.uint32(13))
+        h = h * _F2
+        h = h ^ (h >> np.uint32(16))
+    return h
 
 
 def mix64(keys) -> np.ndarray:

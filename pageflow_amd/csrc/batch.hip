@@ -145,8 +145,10 @@ __device__ __forceinline__ int64_t uniform_item(int64_t v) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+// A key's home slot in a join's table of mask + 1 slots: the low bits of
+// fmix32(key) (the slice partition takes the top bits of the same slot).
 __device__ __forceinline__ uint32_t bslot(int32_t key, uint32_t mask) {
-  return (uint32_t)mix64((uint64_t)(int64_t)key) & mask;
+  return fmix32((uint32_t)key) & mask;
 }
 
 __global__ void bj_fill_tables(uint64_t* table, int64_t n, int32_t* queue, int nq) {

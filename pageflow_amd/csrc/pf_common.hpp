@@ -28,6 +28,18 @@ __device__ __forceinline__ uint64_t mix64(uint64_t k) {
   return k;
 }
 
+// Murmur3's 32-bit finalizer (fmix32), a bijection on 32-bit values: the
+// batch tier's table slots of int32 keys (a third of mix64's instructions on
+// 64-bit values).
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
 // The hash of a 2-key tuple: the first key's mix64 combined with the
 // second's (boost-style combine), finalised once more.
 __device__ __forceinline__ uint64_t tuple_hash(int64_t k0, int64_t k1, bool two) {
@@ -202,8 +214,9 @@ int scan_i32_to_i64_excl(const int32_t* in, int64_t* out, int64_t n, int64_t* to
 // one launch each of hist, scan and scatter): its blocks start at
 // blk_start; its keys at in_off of the key buffer (n of them, int32); its
 // histogram (bucket-major, nchunks a bucket) at hist_off; its outputs at
-// the scanned position + delta; its digit is `bits` bits of mix64 below the
-// top `shift` bits.  Row ids are segment-local.
+// the scanned position + delta; its digit is `bits` bits of fmix32(key),
+// held as a 64-bit value, below its top `shift` bits.  Row ids are
+// segment-local.
 struct RpSeg {
   int64_t blk_start, in_off, n, nchunks, hist_off, delta;
   int32_t bits, shift;

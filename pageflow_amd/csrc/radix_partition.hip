@@ -38,7 +38,9 @@ int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_
 
 namespace {
 
-enum RpMode : int { RP_HASH_KEY = 0, RP_HASH_OUT = 1, RP_RAW = 2 };
+// RP_HASH32: the digit of fmix32 of an int32 key (the batch tier's slots),
+// taken as the low 32 bits of a 64-bit value
+enum RpMode : int { RP_HASH_KEY = 0, RP_HASH_OUT = 1, RP_RAW = 2, RP_HASH32 = 3 };
 
 // Up to two value columns that travel with the rows of a partition (the
 // join's probe-side residual operands): src[c] is read at the row's input
@@ -65,7 +67,8 @@ __device__ __forceinline__ uint32_t rp_digit(K key, int bits, int shift, KO& out
     outv = (KO)key;
     return (uint32_t)((u >> shift) & ((1ULL << bits) - 1));
   }
-  const uint64_t h = mix64((uint64_t)(int64_t)key);
+  const uint64_t h = MODE == RP_HASH32 ? (uint64_t)fmix32((uint32_t)key)
+                                       : mix64((uint64_t)(int64_t)key);
   outv = MODE == RP_HASH_OUT ? (KO)h : (KO)key;
   return bits ? (uint32_t)((h << shift) >> (64 - bits)) : 0u;
 }
@@ -149,9 +152,9 @@ __global__ __launch_bounds__(RpShape<8>::NT) void rp_hist_seg(const int32_t* __r
                                                              int nseg,
                                                              int32_t* __restrict__ hist) {
   const RpSeg& g = segs[seg_of_block(segs, nseg)];
-  rp_hist_body<int32_t, int32_t, RP_HASH_KEY, 8>(keys + g.in_off, g.n, kSegChunkRows, g.bits,
-                                                 g.shift, g.nchunks, hist + g.hist_off,
-                                                 (int64_t)blockIdx.x - g.blk_start);
+  rp_hist_body<int32_t, int32_t, RP_HASH32, 8>(keys + g.in_off, g.n, kSegChunkRows, g.bits,
+                                               g.shift, g.nchunks, hist + g.hist_off,
+                                               (int64_t)blockIdx.x - g.blk_start);
 }
 
 // Lanes of the wave holding the same digit (d == ~0u: not a row).
@@ -450,7 +453,7 @@ __global__ __launch_bounds__(NT) void rp_scatter8_seg(const int32_t* __restrict_
   const RpSeg& g = segs[seg_of_block(segs, nseg)];
   const int64_t chunk = (int64_t)blockIdx.x - g.blk_start;
   const int64_t c0 = chunk * kSegChunkRows;
-  rp_scatter8_body<int32_t, int32_t, RP_HASH_KEY, NT, IT>(
+  rp_scatter8_body<int32_t, int32_t, RP_HASH32, NT, IT>(
       keys + g.in_off, nullptr, c0, c0 + kSegChunkRows < g.n ? c0 + kSegChunkRows : g.n, g.bits,
       g.shift, offs + g.hist_off + chunk, g.nchunks, kout, vout, g.delta, Carry{});
 }

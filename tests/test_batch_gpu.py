@@ -122,7 +122,7 @@ def test_sliced_join_order(per_join, slice_kb, sbits):
     p, _ = bj.pairs(0)
     p = p.cpu().numpy()
     cap = 1 << 17  # pow2 >= 2 * 40000 slots (1 MiB)
-    h = oracle.mix64(pkeys.numpy().astype(np.int64)[p])
+    h = oracle.fmix32_np(pkeys.numpy()[p]).astype(np.uint64)  # the batch's slot hash
     sl = (h & np.uint64(cap - 1)) >> np.uint64(17 - sbits)
     key = sl.astype(np.int64) * (1 << 32) + p
     assert np.all(np.diff(key) > 0)  # slices ascending, probe rows ascending in a slice
@@ -138,7 +138,7 @@ def test_lds_table_passes(per_join, slice_kb, n_hot, hot_slots):
     # hot keys' clusters also wrap around the table's end
     cap, mask = 1 << 17, (1 << 17) - 1
     cand = np.arange(1, 1 << 22, dtype=np.int64)
-    slot = (oracle.mix64(cand) & np.uint64(mask)).astype(np.int64)
+    slot = (oracle.fmix32_np(cand).astype(np.int64) & mask)  # the batch's slot hash
     hot = cand[slot < hot_slots][:n_hot]
     rest = np.random.default_rng(5).permutation(cand[slot >= hot_slots][:400_000])[:40_000 - n_hot]
     bk = np.random.default_rng(7).permutation(np.concatenate([hot, rest])).astype(np.int32)

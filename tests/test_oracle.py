@@ -574,3 +574,32 @@ def test_residual_constants_mirror_the_header():
         assert int(d[f"PF_COL_{name}"]) == residual.COL_TYPE[dt]
     assert int(d["PF_RES_MAX_COLS"]) == residual.MAX_COLS
     assert int(d["PF_RES_MAX_INS"]) == residual.MAX_INS
+
+
+def test_fmix32_is_murmur3s_finalizer_and_a_bijection():
+    """fmix32 against its steps in Python integers, its inverse (the
+    multipliers' inverses mod 2^32), and 0 -> 0."""
+    rng = np.random.default_rng(32)
+    x = rng.integers(-(1 << 31), 1 << 31, 5000, dtype=np.int64)
+    h = oracle.fmix32_np(x)
+    M = (1 << 32) - 1
+
+    def steps(v):
+        v &= M
+        v ^= v >> 16
+        v = (v * 0x85ebca6b) & M
+        v ^= v >> 13
+        v = (v * 0xc2b2ae35) & M
+        return v ^ (v >> 16)
+
+    def inverse(v):
+        v ^= v >> 16
+        v = (v * pow(0xc2b2ae35, -1, 1 << 32)) & M
+        v ^= (v >> 13) ^ (v >> 26)
+        v = (v * pow(0x85ebca6b, -1, 1 << 32)) & M
+        return v ^ (v >> 16)
+
+    assert [int(v) for v in h] == [steps(int(v)) for v in x]
+    assert [inverse(int(v)) for v in h] == [int(v) & M for v in x]
+    assert int(oracle.fmix32_np(np.array([0]))[0]) == 0
+    assert np.unique(oracle.fmix32_np(np.arange(-50000, 50000))).size == 100000
