@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "pf_common.hpp"
@@ -371,7 +372,9 @@ __global__ __launch_bounds__(NT) void bj_compact(const int64_t* __restrict__ p_o
 // whose slot falls in its sub-range.  An LDS table holds 7/8 of its slots;
 // a sub-range with more build rows (keys clustered by their hash) is joined
 // in several passes of build rows in slice order, a probe row keeping the
-// first (lowest) build row that answers it.
+// first (lowest) build row that answers it.  The partition writes each key
+// as fmix32(key), a bijection: a slot is the low bits of that value and the
+// LDS tables hold and compare it in place of the key, so no lookup hashes.
 //
 // Items of slice s of join j go to queue (s + j) % 8, and workgroup b starts
 // on queue b % 8: workgroups are dealt to the eight XCDs round robin, so the
@@ -446,6 +449,25 @@ __device__ __forceinline__ void lds_insert(uint64_t* T, uint32_t pos, uint32_t m
   }
 }
 
+// A lookup of key k in the aligned slot pair e of an LDS table, starting at
+// its second slot when `odd`: the matching slot's row, -1 when an empty slot
+// ends the probe, or kOpen when both slots are taken by other keys.  An
+// empty slot's row half is all ones (rows are non-negative); a slot before
+// the key's home cannot hold it (the table is never full, so no probe wraps
+// all the way round).
+constexpr int kOpen = -2;
+__device__ __forceinline__ int pair_lookup(uint64_t e0, uint64_t e1, uint32_t odd, int32_t k) {
+  // branch-free: both slots are tested, the first that decides wins
+  const uint32_t r0 = (uint32_t)e0, r1 = (uint32_t)e1;
+  const bool z0 = !odd && r0 == ~0u, z1 = r1 == ~0u;
+  const bool m0 = !odd && r0 != ~0u && (uint32_t)(e0 >> 32) == (uint32_t)k;
+  const bool m1 = !z1 && (uint32_t)(e1 >> 32) == (uint32_t)k;
+  int r = z1 ? -1 : kOpen;
+  r = m1 ? (int)r1 : r;
+  r = z0 ? -1 : r;
+  return m0 ? (int)r0 : r;
+}
+
 __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
     const int32_t* __restrict__ pk_part, const int64_t* __restrict__ p_off,
     const int32_t* __restrict__ bk_part, const int32_t* __restrict__ bv_part,
@@ -457,7 +479,8 @@ __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
     const int64_t* __restrict__ first, const int32_t* __restrict__ lgchunk,
     int32_t* __restrict__ partner, int64_t* __restrict__ item_cnt, int64_t* overflow,
     int32_t* xqueue) {
-  __shared__ uint64_t T[PL_SR];
+  __shared__ __align__(16) uint64_t T[PL_SR];
+  ulonglong2* const T2 = reinterpret_cast<ulonglong2*>(T);  // aligned slot pairs
   // matches per compaction item (chunks of 2^lgchunk partitioned rows, at
   // least 256) that this item's rows fall in
   constexpr int NC = PL_ROWS / 256 + 1;
@@ -469,24 +492,31 @@ __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
   int qi = 0;
   int x = (int)(blockIdx.x % NXQ);
   for (;;) {
-    if (tid == 0) {
+    if (w == 0) {  // wave 0 fetches the next item of queue x
       const int lo = q.lo[x], hi = q.hi[x];
       const int n = xstart[hi];
-      const int it = atomicAdd(&xqueue[x], 1);
+      int it = 0;
+      if (lane == 0) it = atomicAdd(&xqueue[x], 1);
+      it = __shfl(it, 0, kWave);
       int p = -1;
-      if (it < n) {  // the pair holding item `it`: the last with xstart[p] <= it
+      if (it < n) {
+        // the pair holding item `it`: the last p in [lo, hi) with
+        // xstart[p] <= it (xstart ascends; xstart[lo] <= it < xstart[hi]),
+        // narrowed 64 ways a step: lane l tests a + l * step
         int a = lo, b = hi;
         while (b - a > 1) {
-          const int mid = (a + b) >> 1;
-          if (xstart[mid] <= it)
-            a = mid;
-          else
-            b = mid;
+          const int step = (b - a + kWave - 1) / kWave;
+          const int c = a + lane * step;
+          const uint64_t le = ballot(c < b && xstart[c] <= it);
+          a += (63 - (int)__clzll(le)) * step;  // lane 0 (c = a) is always in
+          b = b < a + step ? b : a + step;
         }
         p = a;
       }
-      s_w[0] = it;
-      s_w[1] = p;
+      if (lane == 0) {
+        s_w[0] = it;
+        s_w[1] = p;
+      }
     }
     __syncthreads();  // also: the previous item's lookups and counts are done
     const int64_t it = uniform_item(s_w[0]);
@@ -521,7 +551,7 @@ __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
     int nin = nbs;
     if (lsub > 0) {
       int c = 0;
-      for (int r = tid; r < nbs; r += PL_NT) c += (bslot(bk[r], m) - base) < (uint32_t)SR;
+      for (int r = tid; r < nbs; r += PL_NT) c += (((uint32_t)bk[r] & m) - base) < (uint32_t)SR;
       c = wave_sum(c);
       if (lane == 0) s_red[w] = c;
       __syncthreads();
@@ -544,10 +574,23 @@ __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
       for (int i = tid; i < SR; i += PL_NT) T[i] = kEmpty;
       __syncthreads();
       if (npass == 1) {
-        for (int r = tid; r < nbs; r += PL_NT) {
-          const int32_t k = bk[r];
-          const uint32_t pos = bslot(k, m) - base;
-          if (pos < (uint32_t)SR) lds_insert(T, pos, SR - 1, k, bv[r]);
+        // BU rows a thread loaded before their inserts (one round for a
+        // slice of up to 16K build rows)
+        constexpr int BU = 16;
+        for (int rb = 0; rb < nbs; rb += PL_NT * BU) {
+          int32_t hk[BU], hv[BU];
+#pragma unroll
+          for (int u = 0; u < BU; ++u) {
+            const int r = rb + tid + u * PL_NT;
+            hk[u] = r < nbs ? bk[r] : 0;
+            hv[u] = r < nbs ? bv[r] : 0;
+          }
+#pragma unroll
+          for (int u = 0; u < BU; ++u) {
+            const uint32_t pos = ((uint32_t)hk[u] & m) - base;
+            if (rb + tid + u * PL_NT < nbs && pos < (uint32_t)SR)
+              lds_insert(T, pos, SR - 1, hk[u], hv[u]);
+          }
         }
       } else {
         // pass `pass` takes the sub-range's build rows of rank [pass * cap,
@@ -556,7 +599,7 @@ __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
         for (int rb = 0; rb < nbs; rb += PL_NT) {
           const int r = rb + tid;
           const int32_t k = r < nbs ? bk[r] : 0;
-          const uint32_t pos = bslot(k, m) - base;
+          const uint32_t pos = ((uint32_t)k & m) - base;
           const bool in = r < nbs && pos < (uint32_t)SR;
           const uint64_t bal = ballot(in);
           if (lane == 0) s_red[w] = (int)__popcll(bal);
@@ -577,56 +620,103 @@ __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
       }
       __syncthreads();
       // U rows a thread a round; the next round's keys are loaded while this
-      // round's are looked up
+      // round's are looked up.  A lookup reads its home slot's aligned pair
+      // of slots (16 bytes); the U first reads of a round are in flight
+      // together.  Matches are counted per wave into the compaction item
+      // of its rows (scalar sums, flushed to ccnt when the item changes).
       constexpr int U = 8;
-      int32_t k[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int i = tid + u * PL_NT;
-        k[u] = pk[i < n ? i : n - 1];
-      }
-      for (int i0 = tid; i0 < n; i0 += PL_NT * U) {
-        int32_t kn[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int i = i0 + (U + u) * PL_NT;
-          kn[u] = pk[i < n ? i : n - 1];
+      int acc_c = -1, acc_n = 0;  // wave-uniform: an item and its uncounted matches
+      auto count = [&](int c, int cnt) {
+        if (cnt == 0) return;
+        if (c != acc_c) {
+          if (acc_n > 0 && lane == 0) atomicAdd(&ccnt[acc_c], acc_n);
+          acc_c = c;
+          acc_n = 0;
         }
+        acc_n += cnt;
+      };
+      auto sweep = [&](auto first_pass) {
+        constexpr bool FIRST = decltype(first_pass)::value;
+        int32_t k[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int i = i0 + u * PL_NT;
-          uint32_t pos = bslot(k[u], m) - base;
-          const bool mine = i < n && pos < (uint32_t)SR;
-          int32_t hit = -1;  // a match new in this pass
-          if (mine) {
-            uint64_t v = T[pos];
-            while (v != kEmpty && (uint32_t)(v >> 32) != (uint32_t)k[u]) {
-              pos = (pos + 1) & (uint32_t)(SR - 1);
-              v = T[pos];
+          const int i = tid + u * PL_NT;
+          k[u] = pk[i < n ? i : n - 1];
+        }
+        for (int i0 = tid; i0 < n; i0 += PL_NT * U) {
+          int32_t kn[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int i = i0 + (U + u) * PL_NT;
+            kn[u] = pk[i < n ? i : n - 1];
+          }
+          uint32_t pos[U];
+          bool mine[U];
+          uint64_t g0[U], g1[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            pos[u] = ((uint32_t)k[u] & m) - base;
+            mine[u] = i0 + u * PL_NT < n && pos[u] < (uint32_t)SR;
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const ulonglong2 t = T2[(mine[u] ? pos[u] : 0u) >> 1];
+            g0[u] = t.x;
+            g1[u] = t.y;
+          }
+          int32_t hit[U];  // a match new in this pass, or -1
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            int r = pair_lookup(g0[u], g1[u], pos[u] & 1u, k[u]);
+            r = mine[u] ? r : -1;
+            uint32_t q = pos[u];
+            while (r == kOpen) {
+              q = ((q | 1u) + 1u) & (uint32_t)(SR - 1);
+              const ulonglong2 t = T2[q >> 1];
+              r = pair_lookup(t.x, t.y, 0u, k[u]);
             }
-            hit = v != kEmpty ? (int32_t)(uint32_t)v : -1;
-            if (pass == 0)
-              pt[i] = hit;
-            else if (hit >= 0 && pt[i] < 0)  // this thread's own earlier write
-              pt[i] = hit;
-            else
-              hit = -1;
+            hit[u] = r;
           }
-          // a wave's 64 consecutive rows fall in at most two compaction
-          // items (they hold at least 256 rows); lane 0 holds the wave's
-          // first row
-          const int cw = (__shfl(i, 0, kWave) + coff) >> lg;
-          const int ci = (i + coff) >> lg;
-          const uint64_t b_lo = ballot(hit >= 0 && ci == cw);
-          const uint64_t b_hi = ballot(hit >= 0 && ci != cw);
-          if (lane == 0) {
-            if (b_lo) atomicAdd(&ccnt[cw], (int)__popcll(b_lo));
-            if (b_hi) atomicAdd(&ccnt[cw + 1], (int)__popcll(b_hi));
-          }
-        }
 #pragma unroll
-        for (int u = 0; u < U; ++u) k[u] = kn[u];
-      }
+          for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * PL_NT;
+            if (FIRST) {
+              if (mine[u]) pt[i] = hit[u];
+            } else if (hit[u] >= 0) {
+              if (pt[i] < 0)  // this thread's own earlier write
+                pt[i] = hit[u];
+              else
+                hit[u] = -1;
+            }
+          }
+          // the wave's rows of step u are [first, first + 64): in one
+          // compaction item, or two when an item starts at lane `split`
+          const int wfirst = __builtin_amdgcn_readfirstlane(i0 - lane) + coff;
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint64_t bh = ballot(hit[u] >= 0);
+            if (bh) {
+              const int first = wfirst + u * PL_NT;
+              const int cw = first >> lg;
+              const int split = ((cw + 1) << lg) - first;
+              if (split >= kWave) {
+                count(cw, (int)__popcll(bh));
+              } else {
+                const uint64_t lo = bh & ((1ULL << split) - 1);
+                count(cw, (int)__popcll(lo));
+                count(cw + 1, (int)__popcll(bh ^ lo));
+              }
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) k[u] = kn[u];
+        }
+      };
+      if (pass == 0)
+        sweep(std::true_type{});
+      else
+        sweep(std::false_type{});
+      if (acc_n > 0 && lane == 0) atomicAdd(&ccnt[acc_c], acc_n);
       __syncthreads();  // the table is rebuilt by the next pass
     }
     for (int i = tid; i < nc; i += PL_NT)

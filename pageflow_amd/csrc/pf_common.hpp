@@ -215,8 +215,8 @@ int scan_i32_to_i64_excl(const int32_t* in, int64_t* out, int64_t n, int64_t* to
 // blk_start; its keys at in_off of the key buffer (n of them, int32); its
 // histogram (bucket-major, nchunks a bucket) at hist_off; its outputs at
 // the scanned position + delta; its digit is `bits` bits of fmix32(key),
-// held as a 64-bit value, below its top `shift` bits.  Row ids are
-// segment-local.
+// held as a 64-bit value, below its top `shift` bits.  The key output is
+// fmix32(key) (a bijection of the key); row ids are segment-local.
 struct RpSeg {
   int64_t blk_start, in_off, n, nchunks, hist_off, delta;
   int32_t bits, shift;
@@ -225,7 +225,8 @@ constexpr int64_t kSegChunkRows = 32768;  // rows a chunk, every segment
 
 // Stable segmented partition: one hist, one scan and one scatter launch
 // over the nseg segments in d_segs (nblocks chunks in all, hist_entries
-// histogram entries); hist/offs/scan_ws sized for hist_entries.
+// histogram entries); hist/offs/scan_ws sized for hist_entries.  kout gets
+// each row's fmix32(key), vout its segment-local row id.
 int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
                              int64_t nblocks, int64_t hist_entries, int32_t* hist, int64_t* offs,
                              void* scan_ws, int32_t* kout, int32_t* vout, hipStream_t st,

@@ -39,7 +39,8 @@ int radix_partition(const void* keys, int key_bytes, const int32_t* vals, int64_
 namespace {
 
 // RP_HASH32: the digit of fmix32 of an int32 key (the batch tier's slots),
-// taken as the low 32 bits of a 64-bit value
+// taken as the low 32 bits of a 64-bit value, and fmix32 of the key written
+// in its place (a bijection: equal outputs are equal keys)
 enum RpMode : int { RP_HASH_KEY = 0, RP_HASH_OUT = 1, RP_RAW = 2, RP_HASH32 = 3 };
 
 // Up to two value columns that travel with the rows of a partition (the
@@ -69,7 +70,7 @@ __device__ __forceinline__ uint32_t rp_digit(K key, int bits, int shift, KO& out
   }
   const uint64_t h = MODE == RP_HASH32 ? (uint64_t)fmix32((uint32_t)key)
                                        : mix64((uint64_t)(int64_t)key);
-  outv = MODE == RP_HASH_OUT ? (KO)h : (KO)key;
+  outv = (MODE == RP_HASH_OUT || MODE == RP_HASH32) ? (KO)h : (KO)key;
   return bits ? (uint32_t)((h << shift) >> (64 - bits)) : 0u;
 }
 
