@@ -316,10 +316,12 @@ __global__ __launch_bounds__(NT) void bj_compact(const int64_t* __restrict__ p_o
                                                  const int64_t* __restrict__ item_pos,
                                                  int64_t* __restrict__ out_probe,
                                                  int64_t* __restrict__ out_build, int32_t* queue) {
+  // U rows a thread a round (row r0 + u * NT + thread): one barrier pair
+  // a round of NT * U rows
+  constexpr int W = NT / kWave, U = 4;
   __shared__ int64_t s_item;
-  __shared__ int cnt[NT / kWave];
-  __shared__ int64_t s_base;
-  constexpr int W = NT / kWave;
+  __shared__ int cnt[U][W];
+  const int w = threadIdx.x / kWave, lane = lane_id();
   for (;;) {
     if (threadIdx.x == 0) s_item = item_lo + atomicAdd(queue, 1);
     __syncthreads();
@@ -334,28 +336,39 @@ __global__ __launch_bounds__(NT) void bj_compact(const int64_t* __restrict__ p_o
     // from the partition's row ids
     const int32_t* rmap = sliced[j] ? rows_part + base : nullptr;
     int64_t o = p_off[j] + item_pos[item] - item_pos[first[j]];
-    for (int r0 = 0; r0 < len; r0 += NT) {
-      const int r = r0 + threadIdx.x;
-      const int32_t pr = r < len ? partner[base + r] : -1;
-      const uint64_t bal = ballot(pr >= 0);
-      const int w = threadIdx.x / kWave;
-      if (lane_id() == 0) cnt[w] = __popcll(bal);
-      __syncthreads();
-      int pre = 0, tot = 0;
+    for (int r0 = 0; r0 < len; r0 += NT * U) {
+      int32_t pr[U];
+      uint64_t bal[U];
 #pragma unroll
-      for (int q = 0; q < W; ++q) {
-        pre += q < w ? cnt[q] : 0;
-        tot += cnt[q];
+      for (int u = 0; u < U; ++u) {
+        const int r = r0 + u * NT + (int)threadIdx.x;
+        pr[u] = r < len ? partner[base + r] : -1;
       }
-      if (pr >= 0) {
-        const int64_t pos = o + pre + mbcnt(bal);
-        out_probe[pos] = rmap ? (int64_t)rmap[r] : r00 + r;
-        out_build[pos] = (int64_t)pr;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        bal[u] = ballot(pr[u] >= 0);
+        if (lane == 0) cnt[u][w] = __popcll(bal[u]);
       }
-      o += tot;
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        int pre = 0, tot = 0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+          const int c = cnt[u][q];
+          pre += q < w ? c : 0;
+          tot += c;
+        }
+        if (pr[u] >= 0) {
+          const int r = r0 + u * NT + (int)threadIdx.x;
+          const int64_t pos = o + pre + mbcnt(bal[u]);
+          out_probe[pos] = rmap ? (int64_t)rmap[r] : r00 + r;
+          out_build[pos] = (int64_t)pr[u];
+        }
+        o += tot;
+      }
       __syncthreads();
     }
-    (void)s_base;
   }
 }
 
@@ -491,12 +504,14 @@ __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
   const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   int qi = 0;
   int x = (int)(blockIdx.x % NXQ);
+  int pre = -1;  // thread 0: the next item of queue x, taken during this item's lookups
   for (;;) {
     if (w == 0) {  // wave 0 fetches the next item of queue x
       const int lo = q.lo[x], hi = q.hi[x];
       const int n = xstart[hi];
       int it = 0;
-      if (lane == 0) it = atomicAdd(&xqueue[x], 1);
+      if (lane == 0) it = pre >= 0 ? pre : atomicAdd(&xqueue[x], 1);
+      pre = -1;
       it = __shfl(it, 0, kWave);
       int p = -1;
       if (it < n) {
@@ -547,11 +562,23 @@ __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
     const int32_t* __restrict__ bk = bk_part + b_off[j] + bso[sl];
     const int32_t* __restrict__ bv = bv_part + b_off[j] + bso[sl];
     const int nbs = __builtin_amdgcn_readfirstlane((int)(bso[sl + 1] - bso[sl]));
-    // the build rows of this sub-range (all of the slice's when it is one)
+    // the build rows of this sub-range (all of the slice's when it is one),
+    // counted with BU loads a thread in flight together
+    constexpr int BU = 16;
     int nin = nbs;
     if (lsub > 0) {
       int c = 0;
-      for (int r = tid; r < nbs; r += PL_NT) c += (((uint32_t)bk[r] & m) - base) < (uint32_t)SR;
+      for (int rb = 0; rb < nbs; rb += PL_NT * BU) {
+        int32_t t[BU];
+#pragma unroll
+        for (int u = 0; u < BU; ++u) {
+          const int r = rb + tid + u * PL_NT;
+          t[u] = r < nbs ? bk[r] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < BU; ++u)
+          c += rb + tid + u * PL_NT < nbs && (((uint32_t)t[u] & m) - base) < (uint32_t)SR;
+      }
       c = wave_sum(c);
       if (lane == 0) s_red[w] = c;
       __syncthreads();
@@ -570,26 +597,25 @@ __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
     const int32_t* __restrict__ pk = pk_part + p_off[j] + r0;
     int32_t* __restrict__ pt = partner + p_off[j] + r0;
     const int coff = (int)(r0 & ((1 << lg) - 1));  // r0's offset in its compaction item
+    // the next item's index: its atomic returns while this item is probed
+    if (tid == 0) pre = atomicAdd(&xqueue[x], 1);
     for (int pass = 0; pass < npass; ++pass) {
       for (int i = tid; i < SR; i += PL_NT) T[i] = kEmpty;
       __syncthreads();
-      if (npass == 1) {
-        // BU rows a thread loaded before their inserts (one round for a
-        // slice of up to 16K build rows)
-        constexpr int BU = 16;
+      if (npass == 1) {  // BU rows a thread loaded, then inserted
         for (int rb = 0; rb < nbs; rb += PL_NT * BU) {
-          int32_t hk[BU], hv[BU];
+          int32_t tk[BU], tv[BU];
 #pragma unroll
           for (int u = 0; u < BU; ++u) {
             const int r = rb + tid + u * PL_NT;
-            hk[u] = r < nbs ? bk[r] : 0;
-            hv[u] = r < nbs ? bv[r] : 0;
+            tk[u] = r < nbs ? bk[r] : 0;
+            tv[u] = r < nbs ? bv[r] : 0;
           }
 #pragma unroll
           for (int u = 0; u < BU; ++u) {
-            const uint32_t pos = ((uint32_t)hk[u] & m) - base;
+            const uint32_t pos = ((uint32_t)tk[u] & m) - base;
             if (rb + tid + u * PL_NT < nbs && pos < (uint32_t)SR)
-              lds_insert(T, pos, SR - 1, hk[u], hv[u]);
+              lds_insert(T, pos, SR - 1, tk[u], tv[u]);
           }
         }
       } else {
@@ -604,14 +630,14 @@ __global__ __launch_bounds__(PL_NT) void bj_probe_lds(
           const uint64_t bal = ballot(in);
           if (lane == 0) s_red[w] = (int)__popcll(bal);
           __syncthreads();
-          int pre = 0, tot = 0;
+          int below = 0, tot = 0;
 #pragma unroll
           for (int v = 0; v < W; ++v) {
             const int c = s_red[v];
-            pre += v < w ? c : 0;
+            below += v < w ? c : 0;
             tot += c;
           }
-          const int rank = before + pre + mbcnt(bal);
+          const int rank = before + below + mbcnt(bal);
           if (in && rank >= pass * cap && rank < (pass + 1) * cap)
             lds_insert(T, pos, SR - 1, k, bv[r]);
           before += tot;
