@@ -304,7 +304,7 @@ __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_scatter(
 // tile scan, global base, cursor), and the tile's bucket ids staged as bytes.
 // Rows [c0, c1) of `keys`; bucket b's first output row is
 // offs_col[b * ostride] + delta.  NV value columns of `cy` ride along.
-template <typename K, typename KO, int MODE, int NT, int IT, int NV = 0, bool LDS_MATCH = false>
+template <typename K, typename KO, int MODE, int NT, int IT, int NV = 0>
 __device__ __forceinline__ void rp_scatter8_body(
     const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t c0, int64_t c1,
     int bits, int shift, const int64_t* __restrict__ offs_col, int64_t ostride,
@@ -320,17 +320,19 @@ __device__ __forceinline__ void rp_scatter8_body(
   __shared__ uint8_t sb[TILE];
   __shared__ int64_t stage_x[XV][XT];
   __shared__ int32_t tmp[W + 1];
-  // LDS_MATCH: a wave's lanes of one bucket found by OR-ing lane bits into
-  // the (wave, bucket) mask (order-free, so the ranks stay stable), read
-  // back and cleared by the bucket's first lane
-  __shared__ uint64_t pm[LDS_MATCH ? W * BMAX : 1];
+  // a wave's lanes of one bucket: each ORs its lane bit into the (wave,
+  // bucket) mask (order-free, so the ranks stay stable) and reads it back;
+  // the bucket's first lane clears it.  Three LDS operations a row where a
+  // ballot a digit bit costs four vector instructions a bit.
+  __shared__ uint64_t pm[W * BMAX];
 
   const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   const int B = 1 << bits;
   uint32_t cursor = tid < B ? (uint32_t)(offs_col[(int64_t)tid * ostride] + delta) : 0u;
-  for (int x = tid; x < W * B; x += NT) wh[x] = 0;
-  if (LDS_MATCH)
-    for (int x = tid; x < W * B; x += NT) pm[x] = 0;
+  for (int x = tid; x < W * B; x += NT) {
+    wh[x] = 0;
+    pm[x] = 0;
+  }
 
   K kn[IT];
   int32_t vn[IT];
@@ -378,19 +380,14 @@ __device__ __forceinline__ void rp_scatter8_body(
     for (int k = 0; k < IT; ++k) {
       const bool valid = d[k] != ~0u;
       const int at = w * B + (valid ? (int)d[k] : 0);
-      uint64_t peers;
-      if (LDS_MATCH) {
-        if (valid) atomicOr((unsigned long long*)&pm[at], 1ULL << lane);
-        peers = valid ? pm[at] : 0ULL;
-      } else {
-        peers = match_digit<8>(d[k], bits);
-      }
+      if (valid) atomicOr((unsigned long long*)&pm[at], 1ULL << lane);
+      const uint64_t peers = valid ? pm[at] : 0ULL;
       const int below = mbcnt(peers);
       const int base = valid ? (int)wh[at] : 0;
       rk[k] = base + below;
       if (valid && below == 0) {
         wh[at] = (uint16_t)(base + __popcll(peers));
-        if (LDS_MATCH) pm[at] = 0;
+        pm[at] = 0;
       }
     }
     __syncthreads();
@@ -470,7 +467,7 @@ __global__ __launch_bounds__(NT) void rp_scatter8_seg(const int32_t* __restrict_
   const RpSeg& g = segs[seg_of_block(segs, nseg)];
   const int64_t chunk = (int64_t)blockIdx.x - g.blk_start;
   const int64_t c0 = chunk * kSegChunkRows;
-  rp_scatter8_body<int32_t, int32_t, RP_HASH32, NT, IT, 0, true>(
+  rp_scatter8_body<int32_t, int32_t, RP_HASH32, NT, IT>(
       keys + g.in_off, nullptr, c0, c0 + kSegChunkRows < g.n ? c0 + kSegChunkRows : g.n, g.bits,
       g.shift, offs + g.hist_off + chunk, g.nchunks, kout, vout, g.delta, Carry{});
 }
