@@ -304,7 +304,7 @@ __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_scatter(
 // tile scan, global base, cursor), and the tile's bucket ids staged as bytes.
 // Rows [c0, c1) of `keys`; bucket b's first output row is
 // offs_col[b * ostride] + delta.  NV value columns of `cy` ride along.
-template <typename K, typename KO, int MODE, int NT, int IT, int NV = 0>
+template <typename K, typename KO, int MODE, int NT, int IT, int NV = 0, bool LDS_MATCH = true>
 __device__ __forceinline__ void rp_scatter8_body(
     const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t c0, int64_t c1,
     int bits, int shift, const int64_t* __restrict__ offs_col, int64_t ostride,
@@ -312,26 +312,34 @@ __device__ __forceinline__ void rp_scatter8_body(
   constexpr int W = NT / kWave, TILE = NT * IT, BMAX = 256, WROWS = kWave * IT;
   constexpr int XV = NV > 0 ? NV : 1, XT = NV > 0 ? TILE : 1;
   static_assert(BMAX <= NT, "a thread owns each bucket");
+  // wh: per (wave, bucket) counts while ranking, then the bucket's rows in
+  // the waves before; tstart: the bucket's run in the tile.  A 4-byte key
+  // and its row id are staged as one 8-byte word.
+  constexpr bool PACK = sizeof(KO) == 4;
   __shared__ uint16_t wh[W * BMAX];
-  __shared__ uint16_t tstart[BMAX];
   __shared__ uint32_t gb[BMAX];
-  __shared__ KO stage_k[TILE];
-  __shared__ int32_t stage_v[TILE];
+  __shared__ uint16_t tstart[BMAX];
+  __shared__ uint64_t stage_p[PACK ? TILE : 1];
+  __shared__ KO stage_k[PACK ? 1 : TILE];
+  __shared__ int32_t stage_v[PACK ? 1 : TILE];
   __shared__ uint8_t sb[TILE];
   __shared__ int64_t stage_x[XV][XT];
   __shared__ int32_t tmp[W + 1];
-  // a wave's lanes of one bucket: each ORs its lane bit into the (wave,
-  // bucket) mask (order-free, so the ranks stay stable) and reads it back;
-  // the bucket's first lane clears it.  Three LDS operations a row where a
-  // ballot a digit bit costs four vector instructions a bit.
-  __shared__ uint64_t pm[W * BMAX];
+  // LDS_MATCH: a wave's lanes of one bucket are found by each ORing its
+  // lane bit into the (wave, bucket) mask (order-free, so the ranks stay
+  // stable) and reading it back; the bucket's first lane clears it.  Three
+  // LDS operations a row, where a ballot a digit bit (match_digit) costs
+  // four vector instructions a bit.  Which is faster depends on the kernel
+  // (measured: LDS masks for 8-byte keys, ballots for the 4-byte segmented
+  // partition).
+  __shared__ uint64_t pm[LDS_MATCH ? W * BMAX : 1];
 
   const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   const int B = 1 << bits;
   uint32_t cursor = tid < B ? (uint32_t)(offs_col[(int64_t)tid * ostride] + delta) : 0u;
   for (int x = tid; x < W * B; x += NT) {
     wh[x] = 0;
-    pm[x] = 0;
+    if (LDS_MATCH) pm[x] = 0;
   }
 
   K kn[IT];
@@ -380,14 +388,19 @@ __device__ __forceinline__ void rp_scatter8_body(
     for (int k = 0; k < IT; ++k) {
       const bool valid = d[k] != ~0u;
       const int at = w * B + (valid ? (int)d[k] : 0);
-      if (valid) atomicOr((unsigned long long*)&pm[at], 1ULL << lane);
-      const uint64_t peers = valid ? pm[at] : 0ULL;
+      uint64_t peers;
+      if (LDS_MATCH) {
+        if (valid) atomicOr((unsigned long long*)&pm[at], 1ULL << lane);
+        peers = valid ? pm[at] : 0ULL;
+      } else {
+        peers = match_digit<8>(d[k], bits);
+      }
       const int below = mbcnt(peers);
       const int base = valid ? (int)wh[at] : 0;
       rk[k] = base + below;
       if (valid && below == 0) {
         wh[at] = (uint16_t)(base + __popcll(peers));
-        pm[at] = 0;
+        if (LDS_MATCH) pm[at] = 0;
       }
     }
     __syncthreads();
@@ -413,9 +426,9 @@ __device__ __forceinline__ void rp_scatter8_body(
       __syncthreads();
       if (tid < B) {
         const int ex = tmp[w] + incl - tot;
-        tstart[tid] = (uint16_t)ex;
         gb[tid] = cursor - (uint32_t)ex;
         cursor += (uint32_t)tot;
+        tstart[tid] = (uint16_t)ex;
       }
     }
     __syncthreads();
@@ -423,8 +436,12 @@ __device__ __forceinline__ void rp_scatter8_body(
     for (int k = 0; k < IT; ++k) {
       if (d[k] != ~0u) {
         const int pos = tstart[d[k]] + wh[w * B + d[k]] + rk[k];
-        stage_k[pos] = ov[k];
-        stage_v[pos] = vc[k];
+        if (PACK) {
+          stage_p[pos] = ((uint64_t)(uint32_t)vc[k] << 32) | (uint32_t)ov[k];
+        } else {
+          stage_k[pos] = ov[k];
+          stage_v[pos] = vc[k];
+        }
         sb[pos] = (uint8_t)d[k];
 #pragma unroll
         for (int c = 0; c < NV; ++c) stage_x[c][pos] = xc[c][k];
@@ -437,8 +454,14 @@ __device__ __forceinline__ void rp_scatter8_body(
     __syncthreads();
     for (int s = tid; s < tvalid; s += NT) {
       const uint32_t o = gb[sb[s]] + (uint32_t)s;
-      kout[o] = stage_k[s];
-      vout[o] = stage_v[s];
+      if (PACK) {
+        const uint64_t kv = stage_p[s];
+        kout[o] = (KO)(uint32_t)kv;
+        vout[o] = (int32_t)(kv >> 32);
+      } else {
+        kout[o] = stage_k[s];
+        vout[o] = stage_v[s];
+      }
 #pragma unroll
       for (int c = 0; c < NV; ++c) cy.dst[c][o] = stage_x[c][s];
     }
@@ -467,7 +490,7 @@ __global__ __launch_bounds__(NT) void rp_scatter8_seg(const int32_t* __restrict_
   const RpSeg& g = segs[seg_of_block(segs, nseg)];
   const int64_t chunk = (int64_t)blockIdx.x - g.blk_start;
   const int64_t c0 = chunk * kSegChunkRows;
-  rp_scatter8_body<int32_t, int32_t, RP_HASH32, NT, IT>(
+  rp_scatter8_body<int32_t, int32_t, RP_HASH32, NT, IT, 0, false>(
       keys + g.in_off, nullptr, c0, c0 + kSegChunkRows < g.n ? c0 + kSegChunkRows : g.n, g.bits,
       g.shift, offs + g.hist_off + chunk, g.nchunks, kout, vout, g.delta, Carry{});
 }

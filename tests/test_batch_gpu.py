@@ -153,3 +153,28 @@ def test_lds_table_passes(per_join, slice_kb, n_hot, hot_slots):
     torch.cuda.synchronize()
     assert bj.info()["sliced_joins"] == 1 and bj.info()["lds_overflow"] > 0
     _check_all(bj, nb, np_, torch.from_numpy(bk), torch.from_numpy(pk))
+
+
+@pytest.mark.parametrize("per_join", [False, True])
+def test_largest_slice_count(per_join):
+    # a 1.2M-row join: 2^22 slots, more than 256 slices of 64 KiB, so the
+    # slice partition takes its most bits (8: slices of 2^14 slots, one LDS
+    # table each); pairs come in (slice, probe row) order
+    nbj, npj = 1_200_000, 2_000_000
+    g = torch.Generator().manual_seed(11)
+    bkeys = (torch.randperm(nbj, generator=g) * 3 + 1).to(torch.int32)
+    pkeys = bkeys[torch.randint(0, nbj, (npj,), generator=g)].clone()
+    pkeys[::9] = 3  # absent (not 1 mod 3)
+    tune = [(2, 4096, 256, 0), (2, 4096, 512, 0), (2, 16384, 512, 64)]
+    bj = BatchJoin(np.array([nbj]), np.array([npj]), tune)
+    assert bj.info()["sliced_joins"] == 1
+    bj.run(bkeys.cuda(), pkeys.cuda(), per_join=per_join)
+    torch.cuda.synchronize()
+    _check_all(bj, np.array([nbj]), np.array([npj]), bkeys, pkeys)
+    p, _ = bj.pairs(0)
+    p = p.cpu().numpy()
+    cap = 1 << 22
+    h = oracle.fmix32_np(pkeys.numpy()[p]).astype(np.uint64)
+    sl = (h & np.uint64(cap - 1)) >> np.uint64(22 - 8)
+    key = sl.astype(np.int64) * (1 << 32) + p
+    assert np.all(np.diff(key) > 0)
