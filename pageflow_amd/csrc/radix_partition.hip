@@ -74,6 +74,17 @@ __device__ __forceinline__ uint32_t rp_digit(K key, int bits, int shift, KO& out
   return bits ? (uint32_t)((h << shift) >> (64 - bits)) : 0u;
 }
 
+// The digit again from the value written in the key's place (the hash
+// itself, or the raw key); RP_HASH_KEY would need the hash again.
+template <typename KO, int MODE>
+__device__ __forceinline__ uint32_t rp_digit_of_out(KO outv, int bits, int shift) {
+  static_assert(MODE != RP_HASH_KEY, "the key would be hashed again");
+  const uint64_t u = MODE == RP_HASH32 ? (uint64_t)(uint32_t)outv
+                                       : (uint64_t)(typename Unsigned<KO>::T)outv;
+  if (MODE == RP_RAW) return (uint32_t)((u >> shift) & ((1ULL << bits) - 1));
+  return bits ? (uint32_t)((u << shift) >> (64 - bits)) : 0u;
+}
+
 // Rows of a chunk: at least 64 a bucket on average, a multiple of the tile.
 int64_t chunk_rows(int bits, int tile) {
   int64_t c = (int64_t)64 << bits;
@@ -85,6 +96,10 @@ template <int LOGB>
 struct RpShape;
 template <>
 struct RpShape<8> {
+  static constexpr int NT = 256, IT = 8;
+};
+template <>
+struct RpShape<9> {
   static constexpr int NT = 256, IT = 8;
 };
 template <>
@@ -135,12 +150,13 @@ __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_hist(const K* __restrict
                                                blockIdx.x);
 }
 
-// The segment a block works for: the last whose first block is <= it.
-__device__ __forceinline__ int seg_of_block(const RpSeg* __restrict__ segs, int nseg) {
+// The segment block `blk` works for: the last whose first block is <= it.
+__device__ __forceinline__ int seg_of_block(const RpSeg* __restrict__ segs, int nseg,
+                                            int64_t blk) {
   int a = 0, b = nseg;
   while (b - a > 1) {
     const int mid = (a + b) >> 1;
-    if (segs[mid].blk_start <= (int64_t)blockIdx.x)
+    if (segs[mid].blk_start <= blk)
       a = mid;
     else
       b = mid;
@@ -148,12 +164,12 @@ __device__ __forceinline__ int seg_of_block(const RpSeg* __restrict__ segs, int 
   return a;
 }
 
-__global__ __launch_bounds__(RpShape<8>::NT) void rp_hist_seg(const int32_t* __restrict__ keys,
+__global__ __launch_bounds__(RpShape<kSegMaxBits>::NT) void rp_hist_seg(const int32_t* __restrict__ keys,
                                                              const RpSeg* __restrict__ segs,
                                                              int nseg,
                                                              int32_t* __restrict__ hist) {
-  const RpSeg& g = segs[seg_of_block(segs, nseg)];
-  rp_hist_body<int32_t, int32_t, RP_HASH32, 8>(keys + g.in_off, g.n, kSegChunkRows, g.bits,
+  const RpSeg& g = segs[seg_of_block(segs, nseg, blockIdx.x)];
+  rp_hist_body<int32_t, int32_t, RP_HASH32, kSegMaxBits>(keys + g.in_off, g.n, kSegChunkRows, g.bits,
                                                g.shift, g.nchunks, hist + g.hist_off,
                                                (int64_t)blockIdx.x - g.blk_start);
 }
@@ -304,14 +320,21 @@ __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_scatter(
 // tile scan, global base, cursor), and the tile's bucket ids staged as bytes.
 // Rows [c0, c1) of `keys`; bucket b's first output row is
 // offs_col[b * ostride] + delta.  NV value columns of `cy` ride along.
-template <typename K, typename KO, int MODE, int NT, int IT, int NV = 0, bool LDS_MATCH = true>
+template <typename K, typename KO, int MODE, int NT, int IT, int NV = 0, bool LDS_MATCH = true,
+          int BMAX = 256>
 __device__ __forceinline__ void rp_scatter8_body(
     const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t c0, int64_t c1,
     int bits, int shift, const int64_t* __restrict__ offs_col, int64_t ostride,
     KO* __restrict__ kout, int32_t* __restrict__ vout, int64_t delta, const Carry& cy) {
-  constexpr int W = NT / kWave, TILE = NT * IT, BMAX = 256, WROWS = kWave * IT;
+  constexpr int W = NT / kWave, TILE = NT * IT, WROWS = kWave * IT;
   constexpr int XV = NV > 0 ? NV : 1, XT = NV > 0 ? TILE : 1;
   static_assert(BMAX <= NT, "a thread owns each bucket");
+  // 9-bit digits (BMAX 512): ballot ranking, and no staged bucket bytes (the
+  // write-out takes each row's digit again from its staged hash)
+  static_assert(BMAX == 256 || (BMAX == 512 && !LDS_MATCH && MODE != RP_HASH_KEY &&
+                                 sizeof(KO) == 4 && NV == 0),
+                "digits of 8 bits, or of 9 with ballots, 4-byte keys and a re-derivable digit");
+  constexpr bool KEEP_B = BMAX == 256;
   // wh: per (wave, bucket) counts while ranking, then the bucket's rows in
   // the waves before; tstart: the bucket's run in the tile.  A 4-byte key
   // and its row id are staged as one 8-byte word.
@@ -322,7 +345,7 @@ __device__ __forceinline__ void rp_scatter8_body(
   __shared__ uint64_t stage_p[PACK ? TILE : 1];
   __shared__ KO stage_k[PACK ? 1 : TILE];
   __shared__ int32_t stage_v[PACK ? 1 : TILE];
-  __shared__ uint8_t sb[TILE];
+  __shared__ uint8_t sb[KEEP_B ? TILE : 1];
   __shared__ int64_t stage_x[XV][XT];
   __shared__ int32_t tmp[W + 1];
   // LDS_MATCH: a wave's lanes of one bucket are found by each ORing its
@@ -361,9 +384,14 @@ __device__ __forceinline__ void rp_scatter8_body(
       }
     }
   };
-  load_tile(a0);
+  // the next tile's rows are loaded while this one is ranked (registers
+  // double-buffered), except in 16-row tiles, whose registers would spill
+  // (measured: spilling, or loading during the write-out, is slower)
+  constexpr bool PREFETCH = IT <= 8;
+  if (PREFETCH) load_tile(a0);
   __syncthreads();
   for (int64_t t0 = a0; t0 < c1; t0 += TILE) {
+    if (!PREFETCH) load_tile(t0);
     K kc[IT];
     int32_t vc[IT];
     int64_t xc[XV][IT];
@@ -374,7 +402,7 @@ __device__ __forceinline__ void rp_scatter8_body(
 #pragma unroll
       for (int c = 0; c < NV; ++c) xc[c][k] = xn[c][k];
     }
-    if (t0 + TILE < c1) load_tile(t0 + TILE);
+    if (PREFETCH && t0 + TILE < c1) load_tile(t0 + TILE);
     const int64_t seg = t0 + (int64_t)w * WROWS;
     uint32_t d[IT];
     KO ov[IT];
@@ -393,7 +421,7 @@ __device__ __forceinline__ void rp_scatter8_body(
         if (valid) atomicOr((unsigned long long*)&pm[at], 1ULL << lane);
         peers = valid ? pm[at] : 0ULL;
       } else {
-        peers = match_digit<8>(d[k], bits);
+        peers = match_digit<(BMAX > 256 ? 9 : 8)>(d[k], bits);
       }
       const int below = mbcnt(peers);
       const int base = valid ? (int)wh[at] : 0;
@@ -442,7 +470,7 @@ __device__ __forceinline__ void rp_scatter8_body(
           stage_k[pos] = ov[k];
           stage_v[pos] = vc[k];
         }
-        sb[pos] = (uint8_t)d[k];
+        if (KEEP_B) sb[pos] = (uint8_t)d[k];
 #pragma unroll
         for (int c = 0; c < NV; ++c) stage_x[c][pos] = xc[c][k];
       }
@@ -453,9 +481,12 @@ __device__ __forceinline__ void rp_scatter8_body(
     const int tvalid = (int)(hi_row - lo_row);
     __syncthreads();
     for (int s = tid; s < tvalid; s += NT) {
-      const uint32_t o = gb[sb[s]] + (uint32_t)s;
+      const uint64_t kv = PACK ? stage_p[s] : 0;
+      const uint32_t b = KEEP_B ? (uint32_t)sb[s]
+                                : rp_digit_of_out<KO, (KEEP_B ? RP_RAW : MODE)>(
+                                      (KO)(uint32_t)kv, bits, shift);
+      const uint32_t o = gb[b] + (uint32_t)s;
       if (PACK) {
-        const uint64_t kv = stage_p[s];
         kout[o] = (KO)(uint32_t)kv;
         vout[o] = (int32_t)(kv >> 32);
       } else {
@@ -481,16 +512,21 @@ __global__ __launch_bounds__(NT) void rp_scatter8(
                                             0, cy);
 }
 
-template <int NT, int IT>
+// Blocks [blk0, blk0 + gridDim.x) of the segments' chunks; BMAX 512 for
+// segments of 9-bit digits (in 16K-row tiles: a bucket's run a tile is as
+// long as an 8-bit digit's in 8K-row tiles).
+template <int NT, int IT, int BMAX>
 __global__ __launch_bounds__(NT) void rp_scatter8_seg(const int32_t* __restrict__ keys,
                                                       const RpSeg* __restrict__ segs, int nseg,
+                                                      int64_t blk0,
                                                       const int64_t* __restrict__ offs,
                                                       int32_t* __restrict__ kout,
                                                       int32_t* __restrict__ vout) {
-  const RpSeg& g = segs[seg_of_block(segs, nseg)];
-  const int64_t chunk = (int64_t)blockIdx.x - g.blk_start;
+  const int64_t blk = blk0 + blockIdx.x;
+  const RpSeg& g = segs[seg_of_block(segs, nseg, blk)];
+  const int64_t chunk = blk - g.blk_start;
   const int64_t c0 = chunk * kSegChunkRows;
-  rp_scatter8_body<int32_t, int32_t, RP_HASH32, NT, IT, 0, false>(
+  rp_scatter8_body<int32_t, int32_t, RP_HASH32, NT, IT, 0, false, BMAX>(
       keys + g.in_off, nullptr, c0, c0 + kSegChunkRows < g.n ? c0 + kSegChunkRows : g.n, g.bits,
       g.shift, offs + g.hist_off + chunk, g.nchunks, kout, vout, g.delta, Carry{});
 }
@@ -1031,23 +1067,30 @@ int radix_partition_carry(const int64_t* keys, int64_t n, int bits, int nv,
 }
 
 // Host-side use by the other tiers of the library.
-int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
-                             int64_t nblocks, int64_t hist_entries, int32_t* hist, int64_t* offs,
-                             void* scan_ws, int32_t* kout, int32_t* vout, hipStream_t st,
-                             const char* hist_name, const char* scatter_name) {
+int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg, int nseg_wide,
+                             int64_t nblocks, int64_t blocks_wide, int64_t hist_entries,
+                             int32_t* hist, int64_t* offs, void* scan_ws, int32_t* kout,
+                             int32_t* vout, hipStream_t st, const char* hist_name,
+                             const char* scatter_name) {
   if (nseg <= 0 || nblocks <= 0) return 0;
+  if (nseg_wide < 0 || nseg_wide > nseg || blocks_wide < 0 || blocks_wide > nblocks)
+    return fail("radix_partition_segments: the 9-bit segments lead the list");
   // 1024 x 8 tiles: 7.24 -> 6.66 ms against 512 x 16 on the batch's slice partitions
-  static_assert(kSegChunkRows % (1024 * 8) == 0, "a segment chunk is whole scatter tiles");
+  static_assert(kSegChunkRows % (1024 * 16) == 0, "a segment chunk is whole scatter tiles");
   {
     KTimer _kt(hist_name, st);
-    rp_hist_seg<<<(unsigned)nblocks, RpShape<8>::NT, 0, st>>>(keys, d_segs, nseg, hist);
+    rp_hist_seg<<<(unsigned)nblocks, RpShape<kSegMaxBits>::NT, 0, st>>>(keys, d_segs, nseg, hist);
   }
   if (launch_check("segmented partition hist")) return 1;
   if (scan_i32_to_i64_excl(hist, offs, hist_entries, nullptr, scan_ws, st)) return 1;
   {
     KTimer _kt(scatter_name, st);
-    rp_scatter8_seg<1024, 8><<<(unsigned)nblocks, 1024, 0, st>>>(keys, d_segs, nseg, offs, kout,
-                                                               vout);
+    if (blocks_wide > 0)
+      rp_scatter8_seg<1024, 16, 512><<<(unsigned)blocks_wide, 1024, 0, st>>>(
+          keys, d_segs, nseg_wide, 0, offs, kout, vout);
+    if (nblocks > blocks_wide)
+      rp_scatter8_seg<1024, 8, 256><<<(unsigned)(nblocks - blocks_wide), 1024, 0, st>>>(
+          keys, d_segs, nseg, blocks_wide, offs, kout, vout);
   }
   return launch_check("segmented partition scatter");
 }

@@ -156,10 +156,10 @@ def test_lds_table_passes(per_join, slice_kb, n_hot, hot_slots):
 
 
 @pytest.mark.parametrize("per_join", [False, True])
-def test_largest_slice_count(per_join):
-    # a 1.2M-row join: 2^22 slots, more than 256 slices of 64 KiB, so the
-    # slice partition takes its most bits (8: slices of 2^14 slots, one LDS
-    # table each); pairs come in (slice, probe row) order
+def test_nine_bit_slices(per_join):
+    # a 1.2M-row join: 2^22 slots (32 MiB) in 64 KiB slices is 512 slices,
+    # a 9-bit slice partition of both sides (its own scatter launch, in
+    # 16K-row tiles); pairs come in (slice, probe row) order
     nbj, npj = 1_200_000, 2_000_000
     g = torch.Generator().manual_seed(11)
     bkeys = (torch.randperm(nbj, generator=g) * 3 + 1).to(torch.int32)
@@ -175,6 +175,30 @@ def test_largest_slice_count(per_join):
     p = p.cpu().numpy()
     cap = 1 << 22
     h = oracle.fmix32_np(pkeys.numpy()[p]).astype(np.uint64)
-    sl = (h & np.uint64(cap - 1)) >> np.uint64(22 - 8)
+    sl = (h & np.uint64(cap - 1)) >> np.uint64(22 - 9)
     key = sl.astype(np.int64) * (1 << 32) + p
     assert np.all(np.diff(key) > 0)
+
+
+@pytest.mark.parametrize("per_join", [False, True])
+def test_mixed_slice_widths(per_join):
+    # joins of 9-bit slices, of fewer slice bits and probed whole, in one
+    # batch: the 9-bit segments lead the partition's segment list
+    nb = np.array([3_000, 1_100_000, 90_000, 1_300_000, 700])
+    np_ = nb * 3
+    rng = np.random.default_rng(21)
+    bk, pk = [], []
+    for j in range(len(nb)):
+        b = rng.permutation(nb[j]).astype(np.int64) * 5 + 2
+        p = b[rng.integers(0, nb[j], np_[j])]
+        p[rng.random(np_[j]) < 0.1] = 4  # absent
+        bk.append(b)
+        pk.append(p)
+    bkeys = torch.from_numpy(np.concatenate(bk).astype(np.int32))
+    pkeys = torch.from_numpy(np.concatenate(pk).astype(np.int32))
+    tune = [(2, 4096, 256, 0), (2, 4096, 512, 64), (2, 16384, 512, 64)]
+    bj = BatchJoin(nb, np_, tune)
+    assert bj.info()["sliced_joins"] == 3
+    bj.run(bkeys.cuda(), pkeys.cuda(), per_join=per_join)
+    torch.cuda.synchronize()
+    _check_all(bj, nb, np_, bkeys, pkeys)
