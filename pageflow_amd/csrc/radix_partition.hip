@@ -321,7 +321,7 @@ __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_scatter(
 // Rows [c0, c1) of `keys`; bucket b's first output row is
 // offs_col[b * ostride] + delta.  NV value columns of `cy` ride along.
 template <typename K, typename KO, int MODE, int NT, int IT, int NV = 0, bool LDS_MATCH = true,
-          int BMAX = 256>
+          int BMAX = 256, bool ROW_IDS = false>
 __device__ __forceinline__ void rp_scatter8_body(
     const K* __restrict__ keys, const int32_t* __restrict__ vals, int64_t c0, int64_t c1,
     int bits, int shift, const int64_t* __restrict__ offs_col, int64_t ostride,
@@ -331,6 +331,7 @@ __device__ __forceinline__ void rp_scatter8_body(
   static_assert(BMAX <= NT, "a thread owns each bucket");
   // 9-bit digits (BMAX 512): ballot ranking, and no staged bucket bytes (the
   // write-out takes each row's digit again from its staged hash)
+  static_assert(!ROW_IDS || sizeof(KO) == 4, "row ids from indexes: 4-byte keys (staged in pairs)");
   static_assert(BMAX == 256 || (BMAX == 512 && !LDS_MATCH && MODE != RP_HASH_KEY &&
                                  sizeof(KO) == 4 && NV == 0),
                 "digits of 8 bits, or of 9 with ballots, 4-byte keys and a re-derivable digit");
@@ -378,7 +379,7 @@ __device__ __forceinline__ void rp_scatter8_body(
       const int64_t i = seg + k * kWave + lane;
       if (i >= c0 && i < c1) {
         kn[k] = keys[i];
-        vn[k] = vals ? vals[i] : (int32_t)i;
+        if (!ROW_IDS) vn[k] = vals ? vals[i] : (int32_t)i;
 #pragma unroll
         for (int c = 0; c < NV; ++c) xn[c][k] = rf_widen(cy.src[c], cy.type[c], i);
       }
@@ -387,7 +388,8 @@ __device__ __forceinline__ void rp_scatter8_body(
   // the next tile's rows are loaded while this one is ranked (registers
   // double-buffered), except in 16-row tiles, whose registers would spill
   // (measured: spilling, or loading during the write-out, is slower)
-  constexpr bool PREFETCH = IT <= 8;
+  constexpr bool REDERIVE = !KEEP_B;  // digits taken again from ov, not held
+  constexpr bool PREFETCH = IT <= 8 || REDERIVE;
   if (PREFETCH) load_tile(a0);
   __syncthreads();
   for (int64_t t0 = a0; t0 < c1; t0 += TILE) {
@@ -398,30 +400,41 @@ __device__ __forceinline__ void rp_scatter8_body(
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
       kc[k] = kn[k];
-      vc[k] = vn[k];
+      vc[k] = ROW_IDS ? 0 : vn[k];
 #pragma unroll
       for (int c = 0; c < NV; ++c) xc[c][k] = xn[c][k];
     }
     if (PREFETCH && t0 + TILE < c1) load_tile(t0 + TILE);
     const int64_t seg = t0 + (int64_t)w * WROWS;
-    uint32_t d[IT];
+    uint32_t dh[REDERIVE ? 1 : IT];
     KO ov[IT];
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
       const int64_t i = seg + k * kWave + lane;
-      d[k] = (i >= c0 && i < c1) ? rp_digit<K, KO, MODE>(kc[k], bits, shift, ov[k]) : ~0u;
+      const uint32_t dk =
+          (i >= c0 && i < c1) ? rp_digit<K, KO, MODE>(kc[k], bits, shift, ov[k]) : ~0u;
+      if (!REDERIVE) dh[k] = dk;
     }
+    // row k's digit, or ~0u past the rows
+    auto dig = [&](int k) -> uint32_t {
+      if (!REDERIVE) return dh[k];
+      const int64_t i = seg + k * kWave + lane;
+      return (i >= c0 && i < c1)
+                 ? rp_digit_of_out<KO, (REDERIVE ? MODE : RP_RAW)>(ov[k], bits, shift)
+                 : ~0u;
+    };
     int rk[IT];
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
-      const bool valid = d[k] != ~0u;
-      const int at = w * B + (valid ? (int)d[k] : 0);
+      const uint32_t dk = dig(k);
+      const bool valid = dk != ~0u;
+      const int at = w * B + (valid ? (int)dk : 0);
       uint64_t peers;
       if (LDS_MATCH) {
         if (valid) atomicOr((unsigned long long*)&pm[at], 1ULL << lane);
         peers = valid ? pm[at] : 0ULL;
       } else {
-        peers = match_digit<(BMAX > 256 ? 9 : 8)>(d[k], bits);
+        peers = match_digit<(BMAX > 256 ? 9 : 8)>(dk, bits);
       }
       const int below = mbcnt(peers);
       const int base = valid ? (int)wh[at] : 0;
@@ -462,15 +475,18 @@ __device__ __forceinline__ void rp_scatter8_body(
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
-      if (d[k] != ~0u) {
-        const int pos = tstart[d[k]] + wh[w * B + d[k]] + rk[k];
+      const uint32_t dk = dig(k);
+      if (dk != ~0u) {
+        const int pos = tstart[dk] + wh[w * B + dk] + rk[k];
         if (PACK) {
-          stage_p[pos] = ((uint64_t)(uint32_t)vc[k] << 32) | (uint32_t)ov[k];
+          // ROW_IDS: the row id is the row's index (vals is null)
+          const int32_t v = ROW_IDS ? (int32_t)(seg + k * kWave + lane) : vc[k];
+          stage_p[pos] = ((uint64_t)(uint32_t)v << 32) | (uint32_t)ov[k];
         } else {
           stage_k[pos] = ov[k];
           stage_v[pos] = vc[k];
         }
-        if (KEEP_B) sb[pos] = (uint8_t)d[k];
+        if (KEEP_B) sb[pos] = (uint8_t)dk;
 #pragma unroll
         for (int c = 0; c < NV; ++c) stage_x[c][pos] = xc[c][k];
       }
@@ -526,7 +542,7 @@ __global__ __launch_bounds__(NT) void rp_scatter8_seg(const int32_t* __restrict_
   const RpSeg& g = segs[seg_of_block(segs, nseg, blk)];
   const int64_t chunk = blk - g.blk_start;
   const int64_t c0 = chunk * kSegChunkRows;
-  rp_scatter8_body<int32_t, int32_t, RP_HASH32, NT, IT, 0, false, BMAX>(
+  rp_scatter8_body<int32_t, int32_t, RP_HASH32, NT, IT, 0, false, BMAX, true>(
       keys + g.in_off, nullptr, c0, c0 + kSegChunkRows < g.n ? c0 + kSegChunkRows : g.n, g.bits,
       g.shift, offs + g.hist_off + chunk, g.nchunks, kout, vout, g.delta, Carry{});
 }
