@@ -115,8 +115,6 @@ struct Plan {
   std::vector<int32_t> seg_join;
   std::vector<int> seg_of;  // per join: its segment, or -1
   int64_t seg_blocks = 0, seg_entries = 0;
-  int nseg_wide = 0;  // leading segments of 9-bit digits, in both lists
-  int64_t seg_wide_blocks = 0, bseg_wide_blocks = 0;
   RpSeg *d_segs = nullptr, *d_segs1 = nullptr;
   int32_t *d_seg_join = nullptr, *d_shist = nullptr;
   int64_t* d_soffs = nullptr;
@@ -826,15 +824,11 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
   }
   P.plain_slots = P.total_slots;
   P.t_off[nj] = P.total_slots;
-  // segments: the sliced joins, those of 9-bit slice digits first (they
-  // take a scatter launch of their own), each group in join order
-  std::vector<int> seg_order;
-  for (int wide = 1; wide >= 0; --wide)
-    for (int j = 0; j < nj; ++j)
-      if (P.sbits[j] > 0 && (P.sbits[j] == kSegMaxBits) == (wide == 1)) seg_order.push_back(j);
+  // segments: the sliced joins in join order
   P.seg_of.assign(nj, -1);
   int64_t cum = 0;
-  for (const int j : seg_order) {
+  for (int j = 0; j < nj; ++j) {
+    if (P.sbits[j] == 0) continue;
     RpSeg g{};
     g.in_off = P.p_off[j];
     g.n = np[j];
@@ -854,10 +848,6 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
     P.seg_join.push_back(j);
     P.seg_blocks += g.nchunks;
     P.seg_entries += g.nchunks << g.bits;
-    if (g.bits == kSegMaxBits) {
-      ++P.nseg_wide;
-      P.seg_wide_blocks = P.seg_blocks;
-    }
     cum += np[j];
   }
   P.bsoff_base.assign(nj, 0);
@@ -867,7 +857,8 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
     P.bsoff_total += (1 << P.sbits[j]) + 1;
   }
   int64_t bcum = 0;
-  for (const int j : seg_order) {
+  for (int j = 0; j < nj; ++j) {
+    if (P.sbits[j] == 0) continue;
     RpSeg g{};
     g.in_off = P.b_off[j];
     g.n = nb[j];
@@ -885,7 +876,6 @@ int plan_build(Plan& P, int nj, const int64_t* nb, const int64_t* np, const int3
     P.bsegs1.push_back(one);
     P.bseg_blocks += g.nchunks;
     P.bseg_entries += g.nchunks << g.bits;
-    if (g.bits == kSegMaxBits) P.bseg_wide_blocks = P.bseg_blocks;
     bcum += nb[j];
   }
   // the (join, slice) pairs of each queue, join-major, a one-entry gap after
@@ -1146,13 +1136,10 @@ int build_part_stage(Plan& P, const int32_t* bkeys, int only, hipStream_t st) {
   const RpSeg* segs = only < 0 ? P.d_bsegs : P.d_bsegs1 + k;
   const int64_t blocks = only < 0 ? P.bseg_blocks : P.bsegs1[k].nchunks;
   const int64_t entries = only < 0 ? P.bseg_entries : P.bsegs1[k].nchunks << P.bsegs1[k].bits;
-  const bool wide1 = only >= 0 && P.bsegs1[k].bits == kSegMaxBits;
-  const int nwide = only < 0 ? P.nseg_wide : (int)wide1;
-  const int64_t bwide = only < 0 ? P.bseg_wide_blocks : (wide1 ? blocks : 0);
   if (fill_i64(P.d_overflow, 1, 0, st)) return 1;
-  if (radix_partition_segments(bkeys, segs, nseg, nwide, blocks, bwide, entries, P.d_shist,
-                               P.d_soffs, P.d_sscan, P.d_bk_part, P.d_bv_part, st,
-                               "bj_slice_hist", "bj_slice_scatter"))
+  if (radix_partition_segments(bkeys, segs, nseg, blocks, entries, P.d_shist, P.d_soffs,
+                               P.d_sscan, P.d_bk_part, P.d_bv_part, st, "bj_slice_hist",
+                               "bj_slice_scatter"))
     return 1;
   bj_slice_bounds<<<nseg, 256, 0, st>>>(segs, P.d_seg_join + k, P.d_soffs, P.d_b_off,
                                         P.d_bsoff_base, P.d_bsoff);
@@ -1174,12 +1161,9 @@ int slice_stage(Plan& P, const int32_t* pkeys, const SliceQueues& q, int only, h
   const RpSeg* segs = only < 0 ? P.d_segs : P.d_segs1 + k;
   const int64_t blocks = only < 0 ? P.seg_blocks : P.segs1[k].nchunks;
   const int64_t entries = only < 0 ? P.seg_entries : P.segs1[k].nchunks << P.segs1[k].bits;
-  const bool wide1 = only >= 0 && P.segs1[k].bits == kSegMaxBits;
-  const int nwide = only < 0 ? P.nseg_wide : (int)wide1;
-  const int64_t bwide = only < 0 ? P.seg_wide_blocks : (wide1 ? blocks : 0);
-  if (radix_partition_segments(pkeys, segs, nseg, nwide, blocks, bwide, entries, P.d_shist,
-                               P.d_soffs, P.d_sscan, P.d_pk_part, P.d_pv_part, st,
-                               "bj_slice_hist", "bj_slice_scatter"))
+  if (radix_partition_segments(pkeys, segs, nseg, blocks, entries, P.d_shist, P.d_soffs,
+                               P.d_sscan, P.d_pk_part, P.d_pv_part, st, "bj_slice_hist",
+                               "bj_slice_scatter"))
     return 1;
   bj_slice_bounds<<<nseg, 256, 0, st>>>(segs, P.d_seg_join + k, P.d_soffs, P.d_p_off,
                                         P.d_soff_base, P.d_soff);

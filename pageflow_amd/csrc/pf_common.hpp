@@ -226,15 +226,12 @@ constexpr int kSegMaxBits = 9;            // digit bits of a segment, at most
 
 // Stable segmented partition: one hist, one scan and one scatter launch
 // over the nseg segments in d_segs (nblocks chunks in all, hist_entries
-// histogram entries); hist/offs/scan_ws sized for hist_entries.  The
-// segments of 9-bit digits come first (nseg_wide of them, blocks_wide
-// chunks) and take a scatter launch of their own.  kout gets each row's
-// fmix32(key), vout its segment-local row id.
-int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg, int nseg_wide,
-                             int64_t nblocks, int64_t blocks_wide, int64_t hist_entries,
-                             int32_t* hist, int64_t* offs, void* scan_ws, int32_t* kout,
-                             int32_t* vout, hipStream_t st, const char* hist_name,
-                             const char* scatter_name);
+// histogram entries); hist/offs/scan_ws sized for hist_entries.  kout gets
+// each row's fmix32(key), vout its segment-local row id.
+int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
+                             int64_t nblocks, int64_t hist_entries, int32_t* hist, int64_t* offs,
+                             void* scan_ws, int32_t* kout, int32_t* vout, hipStream_t st,
+                             const char* hist_name, const char* scatter_name);
 
 // Fills n values of a device array (a kernel, never hipMemsetAsync, so that
 // graph capture records no memset node).

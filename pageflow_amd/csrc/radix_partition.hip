@@ -528,19 +528,16 @@ __global__ __launch_bounds__(NT) void rp_scatter8(
                                             0, cy);
 }
 
-// Blocks [blk0, blk0 + gridDim.x) of the segments' chunks; BMAX 512 for
-// segments of 9-bit digits (in 16K-row tiles: a bucket's run a tile is as
-// long as an 8-bit digit's in 8K-row tiles).
+// Digits of up to 9 bits (BMAX 512) in 16K-row tiles: a bucket's run a
+// tile is 32 rows or more; row ids are the rows' indexes.
 template <int NT, int IT, int BMAX>
 __global__ __launch_bounds__(NT) void rp_scatter8_seg(const int32_t* __restrict__ keys,
                                                       const RpSeg* __restrict__ segs, int nseg,
-                                                      int64_t blk0,
                                                       const int64_t* __restrict__ offs,
                                                       int32_t* __restrict__ kout,
                                                       int32_t* __restrict__ vout) {
-  const int64_t blk = blk0 + blockIdx.x;
-  const RpSeg& g = segs[seg_of_block(segs, nseg, blk)];
-  const int64_t chunk = blk - g.blk_start;
+  const RpSeg& g = segs[seg_of_block(segs, nseg, blockIdx.x)];
+  const int64_t chunk = (int64_t)blockIdx.x - g.blk_start;
   const int64_t c0 = chunk * kSegChunkRows;
   rp_scatter8_body<int32_t, int32_t, RP_HASH32, NT, IT, 0, false, BMAX, true>(
       keys + g.in_off, nullptr, c0, c0 + kSegChunkRows < g.n ? c0 + kSegChunkRows : g.n, g.bits,
@@ -1083,15 +1080,14 @@ int radix_partition_carry(const int64_t* keys, int64_t n, int bits, int nv,
 }
 
 // Host-side use by the other tiers of the library.
-int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg, int nseg_wide,
-                             int64_t nblocks, int64_t blocks_wide, int64_t hist_entries,
-                             int32_t* hist, int64_t* offs, void* scan_ws, int32_t* kout,
-                             int32_t* vout, hipStream_t st, const char* hist_name,
-                             const char* scatter_name) {
+int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
+                             int64_t nblocks, int64_t hist_entries, int32_t* hist, int64_t* offs,
+                             void* scan_ws, int32_t* kout, int32_t* vout, hipStream_t st,
+                             const char* hist_name, const char* scatter_name) {
   if (nseg <= 0 || nblocks <= 0) return 0;
-  if (nseg_wide < 0 || nseg_wide > nseg || blocks_wide < 0 || blocks_wide > nblocks)
-    return fail("radix_partition_segments: the 9-bit segments lead the list");
-  // 1024 x 8 tiles: 7.24 -> 6.66 ms against 512 x 16 on the batch's slice partitions
+  // 1024 x 16 tiles with digits of up to 9 bits: 0.17 ms less than 1024 x 8
+  // tiles on the batch's 8-bit segments, and one launch for 8- and 9-bit
+  // segments costs no more than one each
   static_assert(kSegChunkRows % (1024 * 16) == 0, "a segment chunk is whole scatter tiles");
   {
     KTimer _kt(hist_name, st);
@@ -1101,12 +1097,8 @@ int radix_partition_segments(const int32_t* keys, const RpSeg* d_segs, int nseg,
   if (scan_i32_to_i64_excl(hist, offs, hist_entries, nullptr, scan_ws, st)) return 1;
   {
     KTimer _kt(scatter_name, st);
-    if (blocks_wide > 0)
-      rp_scatter8_seg<1024, 16, 512><<<(unsigned)blocks_wide, 1024, 0, st>>>(
-          keys, d_segs, nseg_wide, 0, offs, kout, vout);
-    if (nblocks > blocks_wide)
-      rp_scatter8_seg<1024, 8, 256><<<(unsigned)(nblocks - blocks_wide), 1024, 0, st>>>(
-          keys, d_segs, nseg, blocks_wide, offs, kout, vout);
+    rp_scatter8_seg<1024, 16, 1 << kSegMaxBits><<<(unsigned)nblocks, 1024, 0, st>>>(
+        keys, d_segs, nseg, offs, kout, vout);
   }
   return launch_check("segmented partition scatter");
 }

@@ -158,8 +158,8 @@ def test_lds_table_passes(per_join, slice_kb, n_hot, hot_slots):
 @pytest.mark.parametrize("per_join", [False, True])
 def test_nine_bit_slices(per_join):
     # a 1.2M-row join: 2^22 slots (32 MiB) in 64 KiB slices is 512 slices,
-    # a 9-bit slice partition of both sides (its own scatter launch, in
-    # 16K-row tiles); pairs come in (slice, probe row) order
+    # a 9-bit slice partition of both sides; pairs come in (slice, probe
+    # row) order
     nbj, npj = 1_200_000, 2_000_000
     g = torch.Generator().manual_seed(11)
     bkeys = (torch.randperm(nbj, generator=g) * 3 + 1).to(torch.int32)
@@ -183,7 +183,7 @@ def test_nine_bit_slices(per_join):
 @pytest.mark.parametrize("per_join", [False, True])
 def test_mixed_slice_widths(per_join):
     # joins of 9-bit slices, of fewer slice bits and probed whole, in one
-    # batch: the 9-bit segments lead the partition's segment list
+    # batch (one segmented partition launch for every slice width)
     nb = np.array([3_000, 1_100_000, 90_000, 1_300_000, 700])
     np_ = nb * 3
     rng = np.random.default_rng(21)
