@@ -388,8 +388,9 @@ __device__ __forceinline__ void rp_scatter8_body(
   // the next tile's rows are loaded while this one is ranked (registers
   // double-buffered), except in 16-row tiles, whose registers would spill
   // (measured: spilling, or loading during the write-out, is slower)
-  constexpr bool REDERIVE = !KEEP_B;  // digits taken again from ov, not held
-  constexpr bool PREFETCH = IT <= 8 || REDERIVE;
+  // digits taken again from ov where that is cheap, not held in registers
+  constexpr bool REDERIVE = MODE != RP_HASH_KEY;
+  constexpr bool PREFETCH = IT <= 8 || !KEEP_B;
   if (PREFETCH) load_tile(a0);
   __syncthreads();
   for (int64_t t0 = a0; t0 < c1; t0 += TILE) {
