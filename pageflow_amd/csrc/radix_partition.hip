@@ -385,16 +385,14 @@ __device__ __forceinline__ void rp_scatter8_body(
       }
     }
   };
-  // the next tile's rows are loaded while this one is ranked (registers
-  // double-buffered), except in 16-row tiles, whose registers would spill
-  // (measured: spilling, or loading during the write-out, is slower)
   // digits taken again from ov where that is cheap, not held in registers
   constexpr bool REDERIVE = MODE != RP_HASH_KEY;
-  constexpr bool PREFETCH = IT <= 8 || !KEEP_B;
-  if (PREFETCH) load_tile(a0);
+  // the next tile's rows are loaded while this one is ranked (registers
+  // double-buffered; the 4-byte 16-row tiles fit with their row ids taken
+  // from indexes and their digits re-derived)
+  load_tile(a0);
   __syncthreads();
   for (int64_t t0 = a0; t0 < c1; t0 += TILE) {
-    if (!PREFETCH) load_tile(t0);
     K kc[IT];
     int32_t vc[IT];
     int64_t xc[XV][IT];
@@ -405,7 +403,7 @@ __device__ __forceinline__ void rp_scatter8_body(
 #pragma unroll
       for (int c = 0; c < NV; ++c) xc[c][k] = xn[c][k];
     }
-    if (PREFETCH && t0 + TILE < c1) load_tile(t0 + TILE);
+    if (t0 + TILE < c1) load_tile(t0 + TILE);
     const int64_t seg = t0 + (int64_t)w * WROWS;
     uint32_t dh[REDERIVE ? 1 : IT];
     KO ov[IT];
