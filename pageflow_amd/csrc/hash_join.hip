@@ -405,9 +405,9 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
       int hidx[IT];
       int m[IT];
       uint64_t hv[IT];
-      // the unique write pass loads the round's probe rows here too: a
-      // load among the stores below would wait for every store before it
-      constexpr bool PR = W1 && !MULTI;
+      // the write passes load the round's probe rows here too: a load among
+      // the stores below would wait for every store before it
+      constexpr bool PR = W1;
       int32_t pr[PR ? IT : 1];
 #pragma unroll
       for (int k = 0; k < IT; ++k) {
@@ -484,10 +484,9 @@ __global__ __launch_bounds__(HJ_NT) void hj_probe(HjParams p) {
         const int wb = w * kWave;
 #pragma unroll
         for (int k = 0; k < HJ_IT; ++k) {
-          const int i = i0 + k * HJ_NT + tid;
           s_ex[tid] = inw[k];
           s_hi[tid] = hidx[k];
-          s_pr[tid] = m[k] ? prw[i] : 0;
+          s_pr[tid] = m[k] ? pr[k] : 0;
           const int32_t T = (int32_t)__shfl(inw[k] + m[k], kWave - 1, kWave);
           wave_lds_sync();  // a wave reads only its own lanes' entries
           const int64_t base = pos + cnt[k * HJ_W + w];
