@@ -314,12 +314,16 @@ __global__ __launch_bounds__(RpShape<LOGB>::NT) void rp_scatter(
   }
 }
 
-// Up to 256 buckets: the same stable partition with the next tile's keys
-// loaded while the current one is ranked (registers double-buffered), each
-// bucket owned by one thread across the per-tile bookkeeping (wave prefix,
-// tile scan, global base, cursor), and the tile's bucket ids staged as bytes.
-// Rows [c0, c1) of `keys`; bucket b's first output row is
-// offs_col[b * ostride] + delta.  NV value columns of `cy` ride along.
+// Up to 256 buckets (512 for 4-byte keys): the same stable partition with
+// the next tile's keys loaded while the current one is ranked (registers
+// double-buffered), each bucket owned by one thread across the per-tile
+// bookkeeping (wave prefix, tile scan, global base, cursor).  Rows [c0, c1)
+// of `keys`; bucket b's first output row is offs_col[b * ostride] + delta.
+// NV value columns of `cy` ride along.
+//   LDS_MATCH  a wave's lanes of one bucket by LDS lane masks, else ballots
+//   BMAX 512   9-bit digits: ballot ranking, and no staged bucket bytes (the
+//              write-out takes each row's digit again from its staged hash)
+//   ROW_IDS    vals is null: a row's id is its index, taken at staging
 template <typename K, typename KO, int MODE, int NT, int IT, int NV = 0, bool LDS_MATCH = true,
           int BMAX = 256, bool ROW_IDS = false>
 __device__ __forceinline__ void rp_scatter8_body(
@@ -329,13 +333,11 @@ __device__ __forceinline__ void rp_scatter8_body(
   constexpr int W = NT / kWave, TILE = NT * IT, WROWS = kWave * IT;
   constexpr int XV = NV > 0 ? NV : 1, XT = NV > 0 ? TILE : 1;
   static_assert(BMAX <= NT, "a thread owns each bucket");
-  // 9-bit digits (BMAX 512): ballot ranking, and no staged bucket bytes (the
-  // write-out takes each row's digit again from its staged hash)
   static_assert(!ROW_IDS || sizeof(KO) == 4, "row ids from indexes: 4-byte keys (staged in pairs)");
   static_assert(BMAX == 256 || (BMAX == 512 && !LDS_MATCH && MODE != RP_HASH_KEY &&
                                  sizeof(KO) == 4 && NV == 0),
                 "digits of 8 bits, or of 9 with ballots, 4-byte keys and a re-derivable digit");
-  constexpr bool KEEP_B = BMAX == 256;
+  constexpr bool KEEP_B = BMAX == 256;  // the staged rows' buckets as bytes
   // wh: per (wave, bucket) counts while ranking, then the bucket's rows in
   // the waves before; tstart: the bucket's run in the tile.  A 4-byte key
   // and its row id are staged as one 8-byte word.
@@ -349,13 +351,12 @@ __device__ __forceinline__ void rp_scatter8_body(
   __shared__ uint8_t sb[KEEP_B ? TILE : 1];
   __shared__ int64_t stage_x[XV][XT];
   __shared__ int32_t tmp[W + 1];
-  // LDS_MATCH: a wave's lanes of one bucket are found by each ORing its
-  // lane bit into the (wave, bucket) mask (order-free, so the ranks stay
-  // stable) and reading it back; the bucket's first lane clears it.  Three
-  // LDS operations a row, where a ballot a digit bit (match_digit) costs
-  // four vector instructions a bit.  Which is faster depends on the kernel
-  // (measured: LDS masks for 8-byte keys, ballots for the 4-byte segmented
-  // partition).
+  // LDS_MATCH: each lane ORs its lane bit into the (wave, bucket) mask
+  // (order-free, so the ranks stay stable) and reads it back; the bucket's
+  // first lane clears it.  Three LDS operations a row, where a ballot a
+  // digit bit (match_digit) costs four vector instructions a bit.  Which is
+  // faster depends on the kernel (measured: LDS masks for 8-byte keys,
+  // ballots for the 4-byte segmented partition).
   __shared__ uint64_t pm[LDS_MATCH ? W * BMAX : 1];
 
   const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
