@@ -221,7 +221,9 @@ struct RpSeg {
   int64_t blk_start, in_off, n, nchunks, hist_off, delta;
   int32_t bits, shift;
 };
-constexpr int64_t kSegChunkRows = 32768;  // rows a chunk, every segment
+// rows a chunk, every segment: four 16K-row scatter tiles, so a workgroup's
+// start (bucket bases, first tile's loads) is spread over 64K rows
+constexpr int64_t kSegChunkRows = 65536;
 constexpr int kSegMaxBits = 9;            // digit bits of a segment, at most
 
 // Stable segmented partition: one hist, one scan and one scatter launch
