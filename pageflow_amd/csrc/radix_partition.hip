@@ -578,12 +578,17 @@ bool shape8_valid(int shape) { return (shape >= 0 && shape <= 3) || shape == kSh
 
 // The largest tile of any shape fixes the chunk rows, so one workspace size
 // serves every shape.
-int64_t chunk_rows_of(int bits) { return chunk_rows(bits, 8192); }
+// From 64M rows a chunk is at least 128K rows (16 or more 8K-row tiles a
+// workgroup, its start spread over them; still thousands of chunks).
+int64_t chunk_rows_of(int64_t n, int bits) {
+  const int64_t c = chunk_rows(bits, 8192);
+  return n >= (64LL << 20) && c < 131072 ? 131072 : c;
+}
 
 RpWs carve_rp(void* ws, int64_t n, int bits) {
   Carve cv(ws);
   RpWs r;
-  const int64_t crows = chunk_rows_of(bits);
+  const int64_t crows = chunk_rows_of(n, bits);
   const int64_t nchunks = n > 0 ? (n + crows - 1) / crows : 1;
   const int64_t entries = nchunks << bits;
   r.hist = cv.take<int32_t>(entries);
@@ -615,7 +620,7 @@ template <typename K, typename KO, int MODE, int LOGB>
 int rp_launch(const K* keys, const int32_t* vals, int64_t n, int bits, int shift, KO* kout,
               int32_t* vout, int64_t* bucket_offsets, const RpWs& r, hipStream_t st,
               int who, int shape) {
-  const int64_t crows = chunk_rows_of(bits);
+  const int64_t crows = chunk_rows_of(n, bits);
   const int64_t nchunks = n > 0 ? (n + crows - 1) / crows : 1;
   const int64_t entries = nchunks << bits;
   {
@@ -990,7 +995,7 @@ int carry_run(const int64_t* keys, int64_t n, int bits, const Carry& first, cons
               void* ws, hipStream_t st, int who) {
   if (bits <= kOnePassMax) {
     RpWs r = carve_rp(ws, n, bits);
-    const int64_t crows = chunk_rows_of(bits);
+    const int64_t crows = chunk_rows_of(n, bits);
     const int64_t nchunks = n > 0 ? (n + crows - 1) / crows : 1;
     {
       KTimer _kt(kHistName[who], st);
