@@ -202,3 +202,28 @@ def test_mixed_slice_widths(per_join):
     bj.run(bkeys.cuda(), pkeys.cuda(), per_join=per_join)
     torch.cuda.synchronize()
     _check_all(bj, nb, np_, bkeys, pkeys)
+
+
+@pytest.mark.parametrize("per_join", [False, True])
+def test_segment_chunk_edges(per_join):
+    # sliced joins whose build and probe sides end one row before, on and
+    # one row after a segment chunk boundary (64K rows a chunk), and one
+    # spanning two chunks and a row
+    nb = np.array([65_535, 65_536, 65_537, 131_073])
+    np_ = np.array([65_536, 131_071, 131_073, 196_608])
+    rng = np.random.default_rng(33)
+    bk, pk = [], []
+    for j in range(len(nb)):
+        b = rng.permutation(nb[j]).astype(np.int64) * 7 - 3
+        p = b[rng.integers(0, nb[j], np_[j])]
+        p[rng.random(np_[j]) < 0.2] = 5  # absent (not 4 mod 7)
+        bk.append(b)
+        pk.append(p)
+    bkeys = torch.from_numpy(np.concatenate(bk).astype(np.int32))
+    pkeys = torch.from_numpy(np.concatenate(pk).astype(np.int32))
+    tune = [(2, 4096, 256, 0), (2, 4096, 512, 64), (2, 16384, 512, 64)]
+    bj = BatchJoin(nb, np_, tune)
+    assert bj.info()["sliced_joins"] == 4
+    bj.run(bkeys.cuda(), pkeys.cuda(), per_join=per_join)
+    torch.cuda.synchronize()
+    _check_all(bj, nb, np_, bkeys, pkeys)
